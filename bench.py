@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the ghost-free flux-divergence hot path on one MI355X.
+
+Workload (default): 3D 256³ periodic smooth turbulence-like state, 5 conserved
+fields SoA, γ = 5/3, PLM-minmod + HLLC + SSP-RK2 at CFL 0.3 with the CFL dt
+reduced on the device every step, Float64. One "step" = one full SSP-RK2 step
+over the whole grid (one zone-update per cell); inputs are resident in HBM
+before the timed region starts.
+
+Prints ONE JSON line on stdout (progress goes to stderr):
+  value      = zone-updates/s (cells × steps / wall seconds of the timed region)
+  roofline   = the fused stage-2 kernel: algorithmic bytes per launch (15 values
+               per cell: read U¹ and Uⁿ, write Uⁿ⁺¹) ÷ its mean HIP-event duration
+               inside the timed region, against 8 TB/s
+  cpu_baseline = the oracle (C restatement of the RefMesh path, gather form,
+               OpenMP on the host cores) on 2 steps of the same 256³ workload
+Also reports the FP32 form (value, ratio to FP64, L1 error vs FP64).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def load_package():
+    if "vespa_jl_amd" in sys.modules:
+        return sys.modules["vespa_jl_amd"]
+    d = os.path.join(ROOT, "vespa.jl_amd")
+    spec = importlib.util.spec_from_file_location(
+        "vespa_jl_amd", os.path.join(d, "__init__.py"), submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["vespa_jl_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def make_sim(V, n, dtype, U, tile, zchunk):
+    prob = V.Problem("turbulence256", (n, n, n), ((0.0, 1.0),) * 3, 5.0 / 3.0, "periodic",
+                     lambda x, y, z: (1.0, 0.0, 0.0, 0.0, 1.0), 1.0, 0.3)
+    sim = V.Simulation(V.UniformMesh((n, n, n), prob.domain), prob, eltype=dtype)
+    if tile >= 0 or zchunk > 0:
+        sim.set_launch(max(tile, 0), zchunk)
+    sim.upload(U)
+    return sim
+
+
+def sync_all():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def timed_run(V, sim, steps, warmup):
+    """Warm up, then time exactly `steps` steps (wall clock, synchronised on both
+    sides) with per-stage HIP events recorded on the library's stream."""
+    if warmup > 0:
+        V.run_steps(sim, warmup)
+    sim.sync()
+    sync_all()
+    t0 = time.perf_counter()
+    ev = sim.time_steps(steps)
+    sync_all()
+    wall = time.perf_counter() - t0
+    return wall, ev
+
+
+def cpu_baseline(U, n, steps=2):
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    g = O.make_grid((n, n, n), [(0.0, 1.0)] * 3, "periodic", 5.0 / 3.0)
+    Uc = np.ascontiguousarray(U.copy())
+    work = np.empty(10 * n ** 3, dtype=Uc.dtype)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = O.compute_dt_omp(g, Uc, 0.3, threads=threads)
+        O.step_omp(g, Uc, dt, threads=threads, work=work)
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} SSP-RK2 steps (CFL dt each) of the same {n}^3 FP64 workload, oracle in "
+                      f"gather form with OpenMP on {threads} host threads, {el:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--tile", type=int, default=-1, help="tile variant (-1: library default)")
+    ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU only: the flux-divergence path is "
+                                   "benchmarked on a single MI355X; multi-GPU sharding is design-only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    V = load_package()
+    if V.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    from vespa_jl_amd import problems as P
+
+    n = args.n
+    ncell = n ** 3
+    log(f"[bench] building {n}^3 turbulence-like initial state")
+    U64 = P.turbulence_state(n)
+
+    log("[bench] FP64 run")
+    sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk)
+    launch = sim.launch()
+    wall, ev = timed_run(V, sim, args.steps, args.warmup)
+    value = ncell * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
+    stage2_bytes = 15 * 8 * ncell
+    stage1_bytes = 10 * 8 * ncell
+    ach = stage2_bytes / (ev["stage2_ms"] * 1e-3) / 1e9
+    log(f"[bench] FP64 {value / 1e9:.3f} G zone-updates/s, {ms_per_step:.3f} ms/step, "
+        f"stage1 {ev['stage1_ms']:.3f} ms, stage2 {ev['stage2_ms']:.3f} ms, stage2 {ach:.0f} GB/s")
+    U64_end = sim.state() if not args.no_fp32 else None
+    sim.close()
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_rk_stage2_256cubed_f64.json")
+    if os.path.exists(tfile) and n == 256:
+        try:
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
+    out = {
+        "metric": "zone-updates/s & %HBM peak, 3D 256³ PLM+HLLC RK2; adjoint/forward time ratio",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D {n}^3 periodic smooth turbulence-like state (rho=1+0.1*delta, "
+                               f"|k|<=8 random-phase modes, v_rms=0.1 c_s, p=1), gamma=5/3, "
+                               f"PLM-minmod+HLLC SSP-RK2, CFL 0.3 with device dt every step, FP64",
+                   "grid": [n, n, n], "fields": "5 conserved, SoA", "tile": launch["tile"],
+                   "zchunk": launch["zchunk"], "workgroups_per_stage": launch["workgroups"]},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "rk_stage_kernel<double,3,...,STAGE2=true>",
+                     "bytes_per_launch": stage2_bytes, "mean_launch_ms": ev["stage2_ms"]},
+        "step_detail": {"stage1_ms": ev["stage1_ms"], "stage2_ms": ev["stage2_ms"],
+                        "event_step_ms": ev["step_ms"],
+                        "stage1_GBps": stage1_bytes / (ev["stage1_ms"] * 1e-3) / 1e9,
+                        "step_algorithmic_GBps": 200 * ncell / (ms_per_step * 1e-3) / 1e9},
+    }
+
+    if not args.no_fp32:
+        log("[bench] FP32 run")
+        U32 = U64.astype(np.float32)
+        s32 = make_sim(V, n, np.float32, U32, args.tile, args.zchunk)
+        wall32, ev32 = timed_run(V, s32, args.steps, args.warmup)
+        v32 = ncell * args.steps / wall32
+        U32_end = s32.state().astype(np.float64)
+        s32.close()
+        den = np.abs(U64_end).mean(axis=(1, 2, 3))
+        l1 = (np.abs(U32_end - U64_end).mean(axis=(1, 2, 3)) / den).tolist()
+        out["fp32"] = {"value": v32, "unit": "zone-updates/s", "ratio_vs_fp64": v32 / value,
+                       "ms_per_step": 1e3 * wall32 / args.steps, "stage1_ms": ev32["stage1_ms"],
+                       "stage2_ms": ev32["stage2_ms"],
+                       "stage2_GBps": 15 * 4 * ncell / (ev32["stage2_ms"] * 1e-3) / 1e9,
+                       "rel_L1_vs_fp64_per_field": dict(zip(
+                           ["density", "momentum_x", "momentum_y", "momentum_z", "energy"], l1)),
+                       "after_steps": args.warmup + args.steps}
+        log(f"[bench] FP32 {v32 / 1e9:.3f} G zone-updates/s (x{v32 / value:.2f}), rel L1 {l1}")
+
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle, OpenMP)")
+        out["cpu_baseline"] = cpu_baseline(U64, n)
+        log(f"[bench] CPU {out['cpu_baseline']['value'] / 1e6:.2f} M zone-updates/s")
+    else:
+        out["cpu_baseline"] = None
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

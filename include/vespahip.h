@@ -1,0 +1,143 @@
+/*
+ * vespahip.h — C ABI of the MI355X (gfx950) flux-divergence library.
+ *
+ * The library replaces the device flux backend of Vespa.jl behind its
+ * MeshInterface seam. Plain pointers and sizes only; every entry point returns
+ * 0 on success and a negative code on failure, with a message available from
+ * vh_last_error() (the reference raises Julia exceptions instead:
+ * src/ka_flux.jl:63-66, src/problem.jl:39-46).
+ *
+ * Conventions
+ *   - Primitive W = (rho, vx, vy, vz, p); conserved U = (rho, rho*vx, rho*vy,
+ *     rho*vz, E) (src/eos.jl:1-5).
+ *   - Face batches are AoS: 5 contiguous values per face (the reinterpret of
+ *     Vector{NTuple{5,T}} in ext/VespaKAFluxExt.jl:46-48,60).
+ *   - Grid state is SoA: 5 fields, each a dense column-major array (x fastest),
+ *     the RefMesh SoA layout (lib/RefMesh/src/RefMesh.jl:220-226), stored one
+ *     after another: field f of cell c at U[f*ncell + c].
+ *   - dtype: VH_F64 (Float64 state) or VH_F32 (Float32 state with Float64
+ *     geometry, clock, CFL reduction, totals and pressure floor).
+ *   - Boundary codes per (axis, side): VH_BC_PERIODIC, VH_BC_OUTFLOW,
+ *     VH_BC_REFLECTING (lib/MeshInterface/src/MeshInterface.jl:73-89).
+ *   - Calls on one context are stream-ordered on the context's own HIP stream;
+ *     functions that return host data synchronise that stream first.
+ */
+#ifndef VESPAHIP_H
+#define VESPAHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { VH_F64 = 0, VH_F32 = 1 };
+enum { VH_BC_PERIODIC = 0, VH_BC_OUTFLOW = 1, VH_BC_REFLECTING = 2 };
+
+/* Last error message of the calling thread ("" if none). */
+const char* vh_last_error(void);
+
+/* ABI version of this header (bumped on any signature change). */
+int vh_abi_version(void);
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int vh_device_count(void);
+
+/* ---------------------------------------------------------------------------
+ * Tier 1 — exact drop-in for compute_face_fluxes!(::KAFlux, model, F, WL, WR, axisv)
+ * (ext/VespaKAFluxExt.jl:37-63; contract src/ka_flux.jl:40-53): HLLC flux
+ * (src/riemann.jl:33-88) for nf faces. WL, WR, F: [nf][5]; axis: [nf] in {1,2,3}.
+ * nf == 0 is a no-op (ext/VespaKAFluxExt.jl:40).
+ *
+ * Host-pointer form: copies in, runs on the device, copies back (the reference
+ * kernel's own host-to-device crossing, ext/VespaKAFluxExt.jl:51-61), reusing
+ * device staging buffers across calls.
+ */
+int vh_face_flux_f64(const double* WL, const double* WR, const int32_t* axis, int64_t nf,
+                     double gamma, double* F);
+int vh_face_flux_f32(const float* WL, const float* WR, const int32_t* axis, int64_t nf,
+                     double gamma, float* F);
+
+/* Device-pointer form of the same contract, enqueued on `stream` (a
+ * hipStream_t, NULL = the default stream). No synchronisation. */
+int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf,
+                         double gamma, double* dF, void* stream);
+int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf,
+                         double gamma, float* dF, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Tier 2 — device-resident uniform grid (the RefMesh single-level path).
+ * Replaces the whole-stage unit step!(sim, dt) (src/driver.jl:333-359) with its
+ * accumulate_flux! (src/driver.jl:209-223), _euler_apply!/_rk2_combine!
+ * (src/driver.jl:297-321), compute_dt (src/driver.jl:169-199), evolve!
+ * (src/driver.jl:483-508) and conserved_totals (src/diagnostics.jl:16-31).
+ */
+typedef struct vh_grid vh_grid;
+
+/* Create a context over a UniformMesh(dims, domain) (lib/RefMesh/src/RefMesh.jl:95-100)
+ * of rank 1..3. dims[d], lo[d], hi[d] for d < rank; bc[2*d + side] (side 0 lo, 1 hi).
+ * Periodic must be set on both sides of an axis, and a periodic axis needs >= 2
+ * cells. Allocates the state and the stage buffer on the current device. */
+int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const double* lo,
+                   const double* hi, const int32_t* bc, double gamma, int32_t dtype);
+int vh_grid_destroy(vh_grid* g);
+
+/* Number of cells and the element size (8 or 4 bytes) of the state. */
+int64_t vh_grid_ncell(const vh_grid* g);
+
+/* Copy the conserved state (5*ncell values, layout above) from / to the host.
+ * Replaces set_initial_conditions!/field views (src/driver.jl:101-112). */
+int vh_grid_upload(vh_grid* g, const void* U);
+int vh_grid_download(vh_grid* g, void* U);
+
+/* Device address of the state (5*ncell values) for zero-copy callers. */
+int vh_grid_state_device_ptr(vh_grid* g, void** dptr);
+
+/* compute_dt(sim) (src/driver.jl:169-199): cfl / max(|v_d|+c)/w_d, +Inf if the
+ * max is 0. Synchronises. */
+int vh_grid_compute_dt(vh_grid* g, double cfl, double* dt);
+
+/* step!(sim, dt) (src/driver.jl:333-359): one SSP-RK2 step with the given dt.
+ * Stream-ordered; does not synchronise. */
+int vh_grid_step(vh_grid* g, double dt);
+
+/* evolve!-style loop with the CFL dt computed on the device each step
+ * (src/driver.jl:493-506). tfinal > 0: run while t < tfinal*(1-1e-12) with
+ * dt = min(cfl/invdt, tfinal - t), synchronising once per step to test t;
+ * max_steps > 0 bounds the count. tfinal <= 0: exactly max_steps steps with no
+ * clip and no host synchronisation inside the loop. *t_io is the clock in/out,
+ * *steps receives the number of steps taken. */
+int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double* t_io,
+                int64_t* steps);
+
+/* conserved_totals(sim) (src/diagnostics.jl:16-31): sum of U*V per field in
+ * Float64 (tree order). Synchronises. */
+int vh_grid_totals(vh_grid* g, double* out5);
+
+/* Net flux out, acc = accumulate_flux!(sim) (src/driver.jl:209-223), for the
+ * current state, copied to the host (5*ncell values of the state dtype).
+ * Diagnostic/parity entry point; synchronises. */
+int vh_grid_accumulate_flux(vh_grid* g, void* acc_host);
+
+/* Wait for all work queued on the context's stream. */
+int vh_grid_sync(vh_grid* g);
+
+/* Launch shape of the fused stage kernels: tile variant (0 = library default)
+ * and z planes per workgroup (0 = default). See vh_grid_tile_count for the
+ * available tile variants of the grid's rank. Used by the autotuner. */
+int vh_grid_set_launch(vh_grid* g, int32_t tile_variant, int32_t zchunk);
+int vh_grid_tile_count(const vh_grid* g);
+int vh_grid_get_launch(const vh_grid* g, int32_t* tile_x, int32_t* tile_y, int32_t* zchunk,
+                       int64_t* workgroups);
+
+/* Time the fused stage kernels with HIP events on the context's stream: runs
+ * `nsteps` steps (device dt) and returns the mean duration in milliseconds of
+ * each stage kernel and of the whole step. */
+int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1, double* ms_stage2,
+                       double* ms_step);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VESPAHIP_H */

@@ -1,0 +1,281 @@
+"""ctypes wrapper over the CPU oracle ``liboracle.so`` (TEST INFRASTRUCTURE ONLY).
+
+The oracle is the checker: only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module. The product path
+(``vespa.jl_amd``) never imports it.
+
+Arrays follow the reference's SoA storage (lib/RefMesh/src/RefMesh.jl:220-226):
+one contiguous block of 5 fields, each field column-major (x fastest), so a numpy
+array of shape ``(5, nz, ny, nx)`` in C order is exactly that layout.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
+_BC_NAMES = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW, "reflecting": BC_REFLECTING}
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("rank", C.c_int32),
+        ("bc", C.c_int32 * 6),
+        ("dims", C.c_int64 * 3),
+        ("lo", C.c_double * 3),
+        ("hi", C.c_double * 3),
+        ("gamma", C.c_double),
+    ]
+
+
+def build() -> None:
+    """Compile liboracle.so from oracle/vespa_oracle.c (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp, fp = C.POINTER(C.c_double), C.POINTER(C.c_float)
+        gp = C.POINTER(Grid)
+        for suf, tp in (("f64", dp), ("f32", fp)):
+            getattr(L, f"vo_hllc_flux_{suf}").argtypes = [tp, tp, C.c_double, C.c_int, tp]
+            getattr(L, f"vo_euler_flux_{suf}").argtypes = [tp, C.c_double, C.c_int, tp]
+            getattr(L, f"vo_cons2prim_{suf}").argtypes = [tp, C.c_double, C.c_int, tp]
+            getattr(L, f"vo_prim2cons_{suf}").argtypes = [tp, C.c_double, tp]
+            getattr(L, f"vo_face_flux_{suf}").argtypes = [tp, tp, C.POINTER(C.c_int32), C.c_int64,
+                                                          C.c_double, tp]
+            getattr(L, f"vo_accumulate_flux_{suf}").argtypes = [gp, tp, tp, C.c_int]
+            getattr(L, f"vo_accumulate_flux_{suf}").restype = C.c_int
+            getattr(L, f"vo_step_{suf}").argtypes = [gp, tp, C.c_double, tp, C.c_int, tp]
+            getattr(L, f"vo_step_{suf}").restype = C.c_int
+            getattr(L, f"vo_compute_dt_{suf}").argtypes = [gp, tp, C.c_double, C.c_int]
+            getattr(L, f"vo_compute_dt_{suf}").restype = C.c_double
+            getattr(L, f"vo_totals_{suf}").argtypes = [gp, tp, dp]
+            getattr(L, f"vo_evolve_{suf}").argtypes = [gp, tp, C.c_double, C.c_double, C.c_int64,
+                                                       C.c_int, dp]
+            getattr(L, f"vo_evolve_{suf}").restype = C.c_int64
+            getattr(L, f"vo_run_steps_{suf}").argtypes = [gp, tp, C.c_int64, C.c_double, C.c_int, dp]
+            getattr(L, f"vo_run_steps_{suf}").restype = C.c_int
+            getattr(L, f"vo_accumulate_flux_gather_{suf}").argtypes = [gp, tp, tp, C.c_int]
+            getattr(L, f"vo_accumulate_flux_gather_{suf}").restype = C.c_int
+            getattr(L, f"vo_step_omp_{suf}").argtypes = [gp, tp, C.c_double, tp, C.c_int, C.c_int]
+            getattr(L, f"vo_step_omp_{suf}").restype = C.c_int
+            getattr(L, f"vo_compute_dt_omp_{suf}").argtypes = [gp, tp, C.c_double, C.c_int, C.c_int]
+            getattr(L, f"vo_compute_dt_omp_{suf}").restype = C.c_double
+        L.vo_minmod_f64.argtypes = [C.c_double, C.c_double]
+        L.vo_minmod_f64.restype = C.c_double
+        L.vo_minmod_f32.argtypes = [C.c_float, C.c_float]
+        L.vo_minmod_f32.restype = C.c_float
+        L.vo_bface_count.argtypes = [gp]
+        L.vo_bface_count.restype = C.c_int64
+        L.vo_star_state.argtypes = [dp, dp, C.c_double, dp, dp]
+        L.vo_exact_riemann_sample.argtypes = [dp, dp, C.c_double, C.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def _suf(dtype) -> str:
+    dtype = np.dtype(dtype)
+    if dtype == np.float64:
+        return "f64"
+    if dtype == np.float32:
+        return "f32"
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags.c_contiguous
+    tp = C.c_double if a.dtype == np.float64 else C.c_float
+    return a.ctypes.data_as(C.POINTER(tp))
+
+
+def make_grid(dims, domain, bcs, gamma) -> Grid:
+    """dims: tuple of 1..3 ints (x first); domain: ((lo, hi), ...); bcs: one name
+    for every side, or a list of (lo, hi) names per axis."""
+    g = Grid()
+    rank = len(dims)
+    g.rank = rank
+    for d in range(3):
+        g.dims[d] = int(dims[d]) if d < rank else 1
+        g.lo[d] = float(domain[d][0]) if d < rank else 0.0
+        g.hi[d] = float(domain[d][1]) if d < rank else 1.0
+    if isinstance(bcs, str):
+        bcs = [(bcs, bcs)] * rank
+    for d in range(3):
+        lo, hi = bcs[d] if d < rank else ("outflow", "outflow")
+        g.bc[2 * d] = _BC_NAMES[lo] if isinstance(lo, str) else int(lo)
+        g.bc[2 * d + 1] = _BC_NAMES[hi] if isinstance(hi, str) else int(hi)
+    g.gamma = float(gamma)
+    return g
+
+
+def ncell(g: Grid) -> int:
+    return int(g.dims[0] * g.dims[1] * g.dims[2])
+
+
+# ---------------- scalar kernels ----------------
+
+def hllc_flux(WL, WR, gamma, axis, dtype=np.float64):
+    WL = np.ascontiguousarray(WL, dtype=dtype)
+    WR = np.ascontiguousarray(WR, dtype=dtype)
+    F = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"vo_hllc_flux_{_suf(dtype)}")(_ptr(WL), _ptr(WR), gamma, int(axis), _ptr(F))
+    return F
+
+
+def euler_flux(W, gamma, axis, dtype=np.float64):
+    W = np.ascontiguousarray(W, dtype=dtype)
+    F = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"vo_euler_flux_{_suf(dtype)}")(_ptr(W), gamma, int(axis), _ptr(F))
+    return F
+
+
+def cons2prim(U, gamma, dtype=np.float64, floor64=False):
+    U = np.ascontiguousarray(U, dtype=dtype)
+    W = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"vo_cons2prim_{_suf(dtype)}")(_ptr(U), gamma, int(floor64), _ptr(W))
+    return W
+
+
+def prim2cons(W, gamma, dtype=np.float64):
+    W = np.ascontiguousarray(W, dtype=dtype)
+    U = np.zeros(5, dtype=dtype)
+    getattr(lib(), f"vo_prim2cons_{_suf(dtype)}")(_ptr(W), gamma, _ptr(U))
+    return U
+
+
+def minmod(a, b, dtype=np.float64):
+    f = lib().vo_minmod_f64 if np.dtype(dtype) == np.float64 else lib().vo_minmod_f32
+    return f(a, b)
+
+
+def face_flux(WL, WR, axis, gamma):
+    """Batched HLLC (src/ka_flux.jl:48-53): WL, WR (nf, 5); axis (nf,) in 1..3."""
+    WL = np.ascontiguousarray(WL)
+    WR = np.ascontiguousarray(WR, dtype=WL.dtype)
+    ax = np.ascontiguousarray(axis, dtype=np.int32)
+    nf = WL.shape[0]
+    F = np.zeros_like(WL)
+    getattr(lib(), f"vo_face_flux_{_suf(WL.dtype)}")(
+        _ptr(WL), _ptr(WR), ax.ctypes.data_as(C.POINTER(C.c_int32)), nf, gamma, _ptr(F))
+    return F
+
+
+# ---------------- uniform-grid path ----------------
+
+def accumulate_flux(g: Grid, U: np.ndarray, floor64=False) -> np.ndarray:
+    U = np.ascontiguousarray(U)
+    acc = np.zeros_like(U)
+    rc = getattr(lib(), f"vo_accumulate_flux_{_suf(U.dtype)}")(C.byref(g), _ptr(U), _ptr(acc),
+                                                               int(floor64))
+    if rc != 0:
+        raise ValueError("invalid grid")
+    return acc
+
+
+def step(g: Grid, U: np.ndarray, dt: float, floor64=False, bflux=None) -> None:
+    """In-place SSP-RK2 step of U (src/driver.jl:333-359)."""
+    assert U.flags.c_contiguous
+    work = np.empty(10 * ncell(g), dtype=U.dtype)
+    bp = _ptr(bflux) if bflux is not None else None
+    rc = getattr(lib(), f"vo_step_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(dt), _ptr(work),
+                                                   int(floor64), bp)
+    if rc != 0:
+        raise ValueError("invalid grid")
+
+
+def accumulate_flux_gather(g: Grid, U: np.ndarray, floor64=False) -> np.ndarray:
+    U = np.ascontiguousarray(U)
+    acc = np.zeros_like(U)
+    rc = getattr(lib(), f"vo_accumulate_flux_gather_{_suf(U.dtype)}")(C.byref(g), _ptr(U), _ptr(acc),
+                                                                      int(floor64))
+    if rc != 0:
+        raise ValueError("invalid grid")
+    return acc
+
+
+def step_omp(g: Grid, U: np.ndarray, dt: float, floor64=False, threads=0, work=None) -> None:
+    """In-place SSP-RK2 step in gather form on `threads` OpenMP threads."""
+    assert U.flags.c_contiguous
+    if work is None:
+        work = np.empty(10 * ncell(g), dtype=U.dtype)
+    rc = getattr(lib(), f"vo_step_omp_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(dt), _ptr(work),
+                                                       int(floor64), int(threads))
+    if rc != 0:
+        raise ValueError("invalid grid")
+
+
+def compute_dt_omp(g: Grid, U: np.ndarray, cfl: float, floor64=False, threads=0) -> float:
+    U = np.ascontiguousarray(U)
+    return getattr(lib(), f"vo_compute_dt_omp_{_suf(U.dtype)}")(C.byref(g), _ptr(U), cfl, int(floor64),
+                                                               int(threads))
+
+
+def compute_dt(g: Grid, U: np.ndarray, cfl: float, floor64=False) -> float:
+    U = np.ascontiguousarray(U)
+    return getattr(lib(), f"vo_compute_dt_{_suf(U.dtype)}")(C.byref(g), _ptr(U), cfl, int(floor64))
+
+
+def totals(g: Grid, U: np.ndarray) -> np.ndarray:
+    U = np.ascontiguousarray(U)
+    out = np.zeros(5)
+    getattr(lib(), f"vo_totals_{_suf(U.dtype)}")(C.byref(g), _ptr(U),
+                                                out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def evolve(g: Grid, U: np.ndarray, tfinal: float, cfl: float, max_steps=0, floor64=False, t0=0.0):
+    """In-place evolve! (src/driver.jl:483-508); returns (t, steps)."""
+    assert U.flags.c_contiguous
+    t = C.c_double(t0)
+    k = getattr(lib(), f"vo_evolve_{_suf(U.dtype)}")(C.byref(g), _ptr(U), tfinal, cfl,
+                                                    int(max_steps), int(floor64), C.byref(t))
+    if k < 0:
+        raise ValueError("invalid grid")
+    return t.value, int(k)
+
+
+def run_steps(g: Grid, U: np.ndarray, nsteps: int, cfl: float, floor64=False):
+    """In-place: nsteps steps, dt from the CFL condition before each. Returns dts."""
+    assert U.flags.c_contiguous
+    dts = np.zeros(max(nsteps, 1))
+    rc = getattr(lib(), f"vo_run_steps_{_suf(U.dtype)}")(
+        C.byref(g), _ptr(U), int(nsteps), cfl, int(floor64), dts.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise ValueError("invalid grid")
+    return dts[:nsteps]
+
+
+def bface_count(g: Grid) -> int:
+    return int(lib().vo_bface_count(C.byref(g)))
+
+
+# ---------------- exact Riemann ----------------
+
+def star_state(WL, WR, gamma):
+    a = np.ascontiguousarray(WL, dtype=np.float64)
+    b = np.ascontiguousarray(WR, dtype=np.float64)
+    p, u = C.c_double(), C.c_double()
+    lib().vo_star_state(_ptr(a), _ptr(b), gamma, C.byref(p), C.byref(u))
+    return p.value, u.value
+
+
+def exact_riemann_sample(WL, WR, gamma, s):
+    a = np.ascontiguousarray(WL, dtype=np.float64)
+    b = np.ascontiguousarray(WR, dtype=np.float64)
+    out = np.zeros(3)
+    lib().vo_exact_riemann_sample(_ptr(a), _ptr(b), gamma, float(s), _ptr(out))
+    return out

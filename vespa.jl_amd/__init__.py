@@ -1,0 +1,21 @@
+"""vespa.jl_amd — MI355X-native (gfx950) ghost-free flux-divergence hot path of Vespa.jl.
+
+The product is ``libvespahip.so`` (hand-written HIP kernels behind the C ABI in
+``include/vespahip.h``); this package is its host-side mirror of the reference's
+flux-backend interface (see ``hydro.py``). The directory name contains a dot, so
+load it with ``importlib`` under the module name ``vespa_jl_amd`` (as the tests,
+``bench.py`` and ``__graft_entry__.py`` do).
+"""
+from ._lib import (BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_F32, VH_F64,
+                   VespaHIPError, check, device_count, load_library)
+from .hydro import (HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
+                    UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
+                    dump_fields, evolve, prim2cons_array, run_steps, sod_problem, step)
+
+__all__ = [
+    "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_F32", "VH_F64",
+    "VespaHIPError", "check", "device_count", "load_library",
+    "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",
+    "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",
+    "dump_fields", "evolve", "prim2cons_array", "run_steps", "sod_problem", "step",
+]

@@ -1,0 +1,92 @@
+"""ctypes binding of libvespahip.so (the C ABI declared in include/vespahip.h).
+
+The library is built in-tree by ``vespa.jl_amd/build.py`` (hipcc, gfx950). If it
+is missing this module raises: there is no CPU fallback for the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libvespahip.so")
+
+VH_F64, VH_F32 = 0, 1
+BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
+
+# Every symbol include/vespahip.h declares (checked by the CPU test suite).
+EXPORTED = (
+    "vh_last_error", "vh_abi_version", "vh_device_count",
+    "vh_face_flux_f64", "vh_face_flux_f32", "vh_face_flux_dev_f64", "vh_face_flux_dev_f32",
+    "vh_grid_create", "vh_grid_destroy", "vh_grid_ncell", "vh_grid_upload", "vh_grid_download",
+    "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
+    "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
+    "vh_grid_tile_count", "vh_grid_get_launch", "vh_grid_time_steps",
+)
+
+
+class VespaHIPError(RuntimeError):
+    """A C-ABI call returned a non-zero status (message from vh_last_error)."""
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load and type the HIP library. Raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"libvespahip.so not found at {p}: build it with vespa.jl_amd/build.py "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(p)
+    dp, fp, vp = C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_void_p
+    i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.vh_last_error.restype = C.c_char_p
+    L.vh_abi_version.restype = C.c_int
+    L.vh_device_count.restype = C.c_int
+    for name, tp in (("f64", dp), ("f32", fp)):
+        f = getattr(L, f"vh_face_flux_{name}")
+        f.argtypes = [tp, tp, i32p, C.c_int64, C.c_double, tp]
+        f.restype = C.c_int
+        f = getattr(L, f"vh_face_flux_dev_{name}")
+        f.argtypes = [vp, vp, vp, C.c_int64, C.c_double, vp, vp]
+        f.restype = C.c_int
+    L.vh_grid_create.argtypes = [C.POINTER(vp), C.c_int32, i64p, dp, dp, i32p, C.c_double, C.c_int32]
+    L.vh_grid_destroy.argtypes = [vp]
+    L.vh_grid_ncell.argtypes = [vp]
+    L.vh_grid_ncell.restype = C.c_int64
+    L.vh_grid_upload.argtypes = [vp, vp]
+    L.vh_grid_download.argtypes = [vp, vp]
+    L.vh_grid_state_device_ptr.argtypes = [vp, C.POINTER(vp)]
+    L.vh_grid_compute_dt.argtypes = [vp, C.c_double, dp]
+    L.vh_grid_step.argtypes = [vp, C.c_double]
+    L.vh_grid_run.argtypes = [vp, C.c_double, C.c_double, C.c_int64, dp, i64p]
+    L.vh_grid_totals.argtypes = [vp, dp]
+    L.vh_grid_accumulate_flux.argtypes = [vp, vp]
+    L.vh_grid_sync.argtypes = [vp]
+    L.vh_grid_set_launch.argtypes = [vp, C.c_int32, C.c_int32]
+    L.vh_grid_tile_count.argtypes = [vp]
+    L.vh_grid_get_launch.argtypes = [vp, i32p, i32p, i32p, i64p]
+    L.vh_grid_time_steps.argtypes = [vp, C.c_double, C.c_int64, dp, dp, dp]
+    for n in ("vh_grid_create", "vh_grid_destroy", "vh_grid_upload", "vh_grid_download",
+              "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
+              "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
+              "vh_grid_tile_count", "vh_grid_get_launch", "vh_grid_time_steps"):
+        getattr(L, n).restype = C.c_int
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().vh_last_error().decode(errors="replace")
+        raise VespaHIPError(f"vespahip call failed ({rc}): {msg}")
+
+
+def device_count() -> int:
+    return int(load_library().vh_device_count())

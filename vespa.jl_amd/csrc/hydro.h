@@ -1,0 +1,179 @@
+// Device-side ideal-gas hydro kernels for the flux-divergence path (gfx950).
+//
+// These are the per-face / per-cell building blocks shared by the batched
+// face-flux kernel and the fused stencil stage kernels. Each expression keeps
+// the evaluation order of the reference so that, compiled without FMA
+// contraction, results are bit-identical to the CPU restatement:
+//   cons2prim      src/eos.jl:8-25
+//   prim2cons / E  src/eos.jl:28-32 (E = p/(γ-1) + (½ρ)((vx²+vy²)+vz²), a division)
+//   minmod         src/reconstruct.jl:6-16 (a·b ≤ 0 → 0; |a| < |b| ? a : b)
+//   euler_flux     src/riemann.jl:9-20
+//   hllc_flux      src/riemann.jl:33-66, _hllc_star src/riemann.jl:69-88
+//
+// The HLLC evaluation is restructured, not re-derived: the upwind side K is
+// selected first (SL ≥ 0 → L; SR ≤ 0 → R; else sign of S*), and only that
+// side's E, physical flux and star state are formed. Every value that reaches
+// the output is produced by the same operations as in the reference, so the
+// result is the reference's bit for bit; the other side's flux is never needed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vh {
+
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
+
+template <typename T>
+struct Prim {
+  T r, vx, vy, vz, p;
+};
+
+template <typename T>
+struct Cons {
+  T d, mx, my, mz, e;
+};
+
+template <typename T>
+__device__ __forceinline__ T vabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
+template <>
+__device__ __forceinline__ double vabs<double>(double x) { return __builtin_fabs(x); }
+template <>
+__device__ __forceinline__ float vabs<float>(float x) { return __builtin_fabsf(x); }
+
+__device__ __forceinline__ double vsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float vsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+
+// cons2prim (src/eos.jl:8-25). With FLOOR64 (the FP32 form) the pressure and its
+// positivity floor are evaluated in double from the Tf energy and kinetic terms.
+template <typename T, bool FLOOR64>
+__device__ __forceinline__ Prim<T> cons2prim(const Cons<T>& U, T g, double gd) {
+  Prim<T> W;
+  const T rho = U.d;
+  const T inv = T(1) / rho;
+  W.r = rho;
+  W.vx = U.mx * inv;
+  W.vy = U.my * inv;
+  W.vz = U.mz * inv;
+  const T kinetic = (T(0.5) * rho) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
+  if constexpr (FLOOR64) {
+    const double pd = (gd - 1.0) * (double(U.e) - double(kinetic));
+    W.p = T(vmax(pd, 1e-12 * double(kinetic)));
+  } else {
+    const T p = (g - T(1)) * (U.e - kinetic);
+    W.p = vmax(p, T(1e-12) * kinetic);
+  }
+  return W;
+}
+
+// Total energy from primitives (the E of prim2cons and euler_flux).
+template <typename T>
+__device__ __forceinline__ T energy(const Prim<T>& W, T g) {
+  return W.p / (g - T(1)) + (T(0.5) * W.r) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
+}
+
+template <typename T>
+__device__ __forceinline__ T minmod(T a, T b) {
+  if (a * b <= T(0)) return T(0);
+  return vabs(a) < vabs(b) ? a : b;
+}
+
+// limited_slope(l, c, r) = minmod(c - l, r - c) per primitive component.
+template <typename T>
+__device__ __forceinline__ Prim<T> slope(const Prim<T>& l, const Prim<T>& c, const Prim<T>& r) {
+  Prim<T> s;
+  s.r = minmod(c.r - l.r, r.r - c.r);
+  s.vx = minmod(c.vx - l.vx, r.vx - c.vx);
+  s.vy = minmod(c.vy - l.vy, r.vy - c.vy);
+  s.vz = minmod(c.vz - l.vz, r.vz - c.vz);
+  s.p = minmod(c.p - l.p, r.p - c.p);
+  return s;
+}
+
+// _face_value: W + h·slope with h = ±½ (src/driver.jl:227-233).
+template <typename T>
+__device__ __forceinline__ Prim<T> face_value(const Prim<T>& w, const Prim<T>& s, T h) {
+  Prim<T> o;
+  o.r = w.r + h * s.r;
+  o.vx = w.vx + h * s.vx;
+  o.vy = w.vy + h * s.vy;
+  o.vz = w.vz + h * s.vz;
+  o.p = w.p + h * s.p;
+  return o;
+}
+
+// ghost_state (src/driver.jl:117-122). axis is 1-based.
+template <typename T>
+__device__ __forceinline__ Prim<T> ghost(const Prim<T>& w, int bc, int axis) {
+  Prim<T> o = w;
+  if (bc == BC_REFLECTING) {
+    if (axis == 1) o.vx = -w.vx;
+    else if (axis == 2) o.vy = -w.vy;
+    else o.vz = -w.vz;
+  }
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T normal_v(const Prim<T>& w, int dim) {
+  return dim == 1 ? w.vx : (dim == 2 ? w.vy : w.vz);
+}
+
+// HLLC flux across a face normal to `dim` (1-based), src/riemann.jl:33-88.
+template <typename T>
+__device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T g, int dim) {
+  const T rL = WL.r, rR = WR.r, pL = WL.p, pR = WR.p;
+  const T unL = normal_v(WL, dim), unR = normal_v(WR, dim);
+  const T cL = vsqrt(g * pL / rL);
+  const T cR = vsqrt(g * pR / rR);
+  const T SL = vmin(unL - cL, unR - cR);
+  const T SR = vmax(unL + cL, unR + cR);
+  const bool supL = SL >= T(0);
+  const bool supR = !supL && (SR <= T(0));
+  const T Ss = (pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR)) /
+               (rL * (SL - unL) - rR * (SR - unR));
+  const bool useL = supL || (!supR && Ss >= T(0));
+
+  const Prim<T> W = useL ? WL : WR;
+  const T SK = useL ? SL : SR;
+  const T un = useL ? unL : unR;
+  const T E = energy(W, g);
+  const T m1 = W.r * W.vx, m2 = W.r * W.vy, m3 = W.r * W.vz;
+  Cons<T> F;
+  F.d = W.r * un;
+  F.mx = m1 * un + (dim == 1 ? W.p : T(0));
+  F.my = m2 * un + (dim == 2 ? W.p : T(0));
+  F.mz = m3 * un + (dim == 3 ? W.p : T(0));
+  F.e = (E + W.p) * un;
+  if (supL || supR) return F;
+
+  // F*K = FK + SK (U*K − UK), src/riemann.jl:69-88.
+  const T q = W.r * (SK - un);
+  const T factor = q / (SK - Ss);
+  const T Es = E / W.r + (Ss - un) * (Ss + W.p / q);
+  const T vsx = dim == 1 ? Ss : W.vx;
+  const T vsy = dim == 2 ? Ss : W.vy;
+  const T vsz = dim == 3 ? Ss : W.vz;
+  Cons<T> o;
+  o.d = F.d + SK * (factor - W.r);
+  o.mx = F.mx + SK * (factor * vsx - m1);
+  o.my = F.my + SK * (factor * vsy - m2);
+  o.mz = F.mz + SK * (factor * vsz - m3);
+  o.e = F.e + SK * (factor * Es - E);
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ void acc_add(Cons<T>& a, const Cons<T>& F, T s) {
+  a.d += s * F.d;
+  a.mx += s * F.mx;
+  a.my += s * F.my;
+  a.mz += s * F.mz;
+  a.e += s * F.e;
+}
+
+}  // namespace vh

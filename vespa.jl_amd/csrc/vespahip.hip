@@ -1,0 +1,683 @@
+// vespahip — C-ABI entry points of the gfx950 flux-divergence library.
+//
+// Kernels:
+//   face_flux_kernel   one lane per face, HLLC (tier-1 drop-in for the
+//                      KernelAbstractions kernel, ext/VespaKAFluxExt.jl:23-35)
+//   rk_stage_kernel    fused RK stage over a uniform grid (stencil.h)
+//   cfl_kernel         per-workgroup max of the CFL term (src/driver.jl:178-185)
+//   dt_finalize_kernel reduce the partial maxima into dt on the device
+//   totals_kernel      per-workgroup Σ U·V partial sums (src/diagnostics.jl:16-31)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/vespahip.h"
+#include "stencil.h"
+
+namespace vh {
+
+// ----------------------------------------------------------------------------
+// error plumbing
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
+  char buf[512];
+  snprintf(buf, sizeof buf, fmt, a, b);
+  g_err = buf;
+  return code;
+}
+
+#define VH_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) return fail(-2, "HIP error %s (code %lld) at " #call,          \
+                                      hipGetErrorString(e_), (long long)e_);             \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// tier-1 face flux
+
+template <typename T>
+__global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL, const T* __restrict__ WR,
+                                                        const int32_t* __restrict__ axis, long long nf,
+                                                        T g, T* __restrict__ F) {
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  Prim<T> L, R;
+  const T* l = WL + 5 * f;
+  const T* r = WR + 5 * f;
+  L.r = l[0]; L.vx = l[1]; L.vy = l[2]; L.vz = l[3]; L.p = l[4];
+  R.r = r[0]; R.vx = r[1]; R.vy = r[2]; R.vz = r[3]; R.p = r[4];
+  int d = axis[f];
+  d = d < 1 ? 1 : (d > 3 ? 3 : d);
+  const Cons<T> o = hllc(L, R, g, d);
+  T* out = F + 5 * f;
+  out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
+}
+
+// ----------------------------------------------------------------------------
+// reductions
+
+template <typename T, int RANK, bool FLOOR64>
+__global__ void __launch_bounds__(256) cfl_kernel(const T* __restrict__ U, GridDev G,
+                                                  double* __restrict__ partials) {
+  __shared__ double red[16];
+  const T g = T(G.gamma);
+  double m = 0.0;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < G.ncell; c += (long long)gridDim.x * 256)
+    m = vmax(m, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g, G));
+  m = block_max<256>(m, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// clock: [0] t, [1] dt, [2] invdt, [3] steps (as double)
+__global__ void __launch_bounds__(256) dt_finalize_kernel(const double* __restrict__ partials, int np,
+                                                          double cfl, double tfinal, int clip,
+                                                          double* clock) {
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) m = vmax(m, partials[i]);
+  m = block_max<256>(m, red);
+  if (threadIdx.x == 0) {
+    double dt = m == 0.0 ? HUGE_VAL : cfl / m;
+    if (clip) dt = vmin(dt, tfinal - clock[0]);
+    clock[1] = dt;
+    clock[2] = m;
+  }
+}
+
+__global__ void set_dt_kernel(double* clock, double dt) { clock[1] = dt; }
+
+__global__ void advance_clock_kernel(double* clock) {
+  clock[0] = clock[0] + clock[1];
+  clock[3] = clock[3] + 1.0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) totals_kernel(const T* __restrict__ U, long long n, double V,
+                                                     double* __restrict__ partials) {
+  __shared__ double red[16];
+  double s[5] = {0, 0, 0, 0, 0};
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) s[f] += double(U[f * n + c]) * V;
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    const double t = block_sum<256>(s[f], red);
+    if (threadIdx.x == 0) partials[5 * blockIdx.x + f] = t;
+  }
+}
+
+// Zero the accumulator output used by the diagnostic accumulate path.
+template <typename T>
+__global__ void fill_kernel(T* p, long long n, T v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// acc = (U − U1)/invV reconstructed exactly is not possible; the diagnostic
+// accumulate path instead runs stage 1 with dt = −V (invV = −1 in Tf), so
+// U1 = U + acc·1, then subtracts U in a second kernel. With invV = −1 the
+// multiply is exact, but the add/subtract round. The parity tests therefore use
+// the step path; this entry point is kept for inspection only.
+template <typename T>
+__global__ void sub_kernel(const T* __restrict__ a, const T* __restrict__ b, T* __restrict__ out,
+                           long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+
+// ----------------------------------------------------------------------------
+// launch table
+
+struct TileShape {
+  int tx, ty;
+};
+
+static const TileShape kTiles3[] = {{32, 8}, {64, 4}, {16, 16}, {64, 8}};
+static const TileShape kTiles2[] = {{32, 8}, {64, 4}, {16, 16}};
+static const TileShape kTiles1[] = {{256, 1}, {128, 1}};
+
+template <typename T, int RANK, int TX, int TY, bool FLOOR64>
+static void launch_stage_t(bool stage2, int blocks, hipStream_t s, const T* Us, const T* U0, T* Uout,
+                           const double* dtp, double* partials, const GridDev& G) {
+  if (stage2)
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
+                       Us, U0, Uout, dtp, partials, G);
+  else
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
+                       Us, U0, Uout, dtp, partials, G);
+}
+
+template <typename T, bool FLOOR64>
+static void launch_stage(int rank, int variant, bool stage2, int blocks, hipStream_t s, const T* Us,
+                         const T* U0, T* Uout, const double* dtp, double* partials, const GridDev& G) {
+  if (rank == 3) {
+    switch (variant) {
+      case 0: launch_stage_t<T, 3, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 1: launch_stage_t<T, 3, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 2: launch_stage_t<T, 3, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: launch_stage_t<T, 3, 64, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  } else if (rank == 2) {
+    switch (variant) {
+      case 0: launch_stage_t<T, 2, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 1: launch_stage_t<T, 2, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: launch_stage_t<T, 2, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  } else {
+    switch (variant) {
+      case 0: launch_stage_t<T, 1, 256, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: launch_stage_t<T, 1, 128, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  }
+}
+
+template <typename T, bool FLOOR64>
+static void launch_cfl(int rank, int blocks, hipStream_t s, const T* U, const GridDev& G, double* partials) {
+  if (rank == 3) hipLaunchKernelGGL((cfl_kernel<T, 3, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
+  else if (rank == 2) hipLaunchKernelGGL((cfl_kernel<T, 2, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
+  else hipLaunchKernelGGL((cfl_kernel<T, 1, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
+}
+
+}  // namespace vh
+
+using namespace vh;
+
+// ----------------------------------------------------------------------------
+// context
+
+struct vh_grid {
+  int dtype;
+  int device;
+  GridDev G;
+  int variant;
+  int tx, ty;
+  int stage_blocks;
+  int cfl_blocks;
+  size_t esize;
+  void* U;
+  void* U1;
+  double* partials;
+  int npartials;
+  double* clock;
+  double* totals_partials;
+  hipStream_t stream;
+};
+
+static const int kMaxCflBlocks = 2048;
+
+static int tile_count_for_rank(int rank) {
+  return rank == 3 ? 4 : (rank == 2 ? 3 : 2);
+}
+
+static TileShape tile_for(int rank, int variant) {
+  if (rank == 3) return kTiles3[variant];
+  if (rank == 2) return kTiles2[variant];
+  return kTiles1[variant];
+}
+
+static int configure(vh_grid* g, int variant, int zchunk) {
+  GridDev& G = g->G;
+  if (variant < 0 || variant >= tile_count_for_rank(G.rank))
+    return fail(-1, "tile variant %s%lld out of range", "", variant);
+  const TileShape t = tile_for(G.rank, variant);
+  g->variant = variant;
+  g->tx = t.tx;
+  g->ty = t.ty;
+  G.tiles_x = (G.n[0] + t.tx - 1) / t.tx;
+  G.tiles_y = (G.n[1] + t.ty - 1) / t.ty;
+  if (G.rank == 3) {
+    if (zchunk <= 0) zchunk = 32;
+    if (zchunk > G.n[2]) zchunk = G.n[2];
+  } else {
+    zchunk = 1;
+  }
+  G.zchunk = zchunk;
+  G.chunks_z = (G.n[2] + zchunk - 1) / zchunk;
+  G.xcd_remap = 1;
+  long long blocks = (long long)G.tiles_x * G.tiles_y * G.chunks_z;
+  if (blocks > (1LL << 30)) return fail(-1, "grid too large for one launch%s (%lld workgroups)", "", blocks);
+  g->stage_blocks = (int)blocks;
+  const int need = g->stage_blocks > kMaxCflBlocks ? g->stage_blocks : kMaxCflBlocks;
+  if (need > g->npartials) {
+    if (g->partials) VH_HIP(hipFree(g->partials));
+    g->partials = nullptr;
+    VH_HIP(hipMalloc(&g->partials, sizeof(double) * need));
+    g->npartials = need;
+  }
+  return 0;
+}
+
+template <typename T>
+static int face_flux_dev(const T* dWL, const T* dWR, const int32_t* dax, int64_t nf, double gamma, T* dF,
+                         void* stream) {
+  if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
+  if (nf == 0) return 0;
+  if (!dWL || !dWR || !dax || !dF) return fail(-1, "null pointer%s", "");
+  const long long blocks = (nf + 255) / 256;
+  if (blocks > 0x7fffffffLL) return fail(-1, "too many faces%s (%lld)", "", nf);
+  hipLaunchKernelGGL((face_flux_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dWL,
+                     dWR, dax, (long long)nf, T(gamma), dF);
+  VH_HIP(hipGetLastError());
+  return 0;
+}
+
+// Host-pointer form: persistent device staging per thread (grown on demand).
+struct Staging {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  ~Staging() {
+    // Process teardown: the runtime may already be gone; do not call into it.
+  }
+};
+static thread_local Staging g_stage;
+
+template <typename T>
+static int face_flux_host(const T* WL, const T* WR, const int32_t* ax, int64_t nf, double gamma, T* F) {
+  if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
+  if (nf == 0) return 0;
+  if (!WL || !WR || !ax || !F) return fail(-1, "null pointer%s", "");
+  int dev = 0;
+  VH_HIP(hipGetDevice(&dev));
+  const size_t nv = (size_t)nf * 5 * sizeof(T);
+  const size_t na = ((size_t)nf * sizeof(int32_t) + 255) & ~(size_t)255;
+  const size_t nv_al = (nv + 255) & ~(size_t)255;
+  const size_t need = 3 * nv_al + na;
+  if (g_stage.device != dev || g_stage.bytes < need) {
+    if (g_stage.buf && g_stage.device == dev) VH_HIP(hipFree(g_stage.buf));
+    g_stage.buf = nullptr;
+    VH_HIP(hipMalloc(&g_stage.buf, need));
+    g_stage.bytes = need;
+    if (g_stage.device != dev || !g_stage.stream) VH_HIP(hipStreamCreateWithFlags(&g_stage.stream, hipStreamNonBlocking));
+    g_stage.device = dev;
+  }
+  char* base = (char*)g_stage.buf;
+  T* dWL = (T*)base;
+  T* dWR = (T*)(base + nv_al);
+  T* dF = (T*)(base + 2 * nv_al);
+  int32_t* dax = (int32_t*)(base + 3 * nv_al);
+  hipStream_t s = g_stage.stream;
+  VH_HIP(hipMemcpyAsync(dWL, WL, nv, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dWR, WR, nv, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dax, ax, (size_t)nf * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  int rc = face_flux_dev<T>(dWL, dWR, dax, nf, gamma, dF, s);
+  if (rc) return rc;
+  VH_HIP(hipMemcpyAsync(F, dF, nv, hipMemcpyDeviceToHost, s));
+  VH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" {
+
+const char* vh_last_error(void) { return g_err.c_str(); }
+int vh_abi_version(void) { return 1; }
+
+int vh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ---- tier 1 ----
+
+int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf, double gamma,
+                         double* dF, void* stream) {
+  return face_flux_dev<double>(dWL, dWR, daxis, nf, gamma, dF, stream);
+}
+int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf, double gamma,
+                         float* dF, void* stream) {
+  return face_flux_dev<float>(dWL, dWR, daxis, nf, gamma, dF, stream);
+}
+
+int vh_face_flux_f64(const double* WL, const double* WR, const int32_t* axis, int64_t nf, double gamma,
+                     double* F) {
+  return face_flux_host<double>(WL, WR, axis, nf, gamma, F);
+}
+int vh_face_flux_f32(const float* WL, const float* WR, const int32_t* axis, int64_t nf, double gamma,
+                     float* F) {
+  return face_flux_host<float>(WL, WR, axis, nf, gamma, F);
+}
+
+// ---- tier 2 ----
+
+int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const double* lo, const double* hi,
+                   const int32_t* bc, double gamma, int32_t dtype) {
+  if (!out || !dims || !lo || !hi || !bc) return fail(-1, "null argument%s", "");
+  *out = nullptr;
+  if (rank < 1 || rank > 3) return fail(-1, "rank must be 1..3%s (got %lld)", "", rank);
+  if (dtype != VH_F64 && dtype != VH_F32) return fail(-1, "dtype must be VH_F64 or VH_F32%s (got %lld)", "", dtype);
+  if (!(gamma > 1.0)) return fail(-1, "gamma must exceed 1%s", "");
+  vh_grid* g = new vh_grid();
+  memset(g, 0, sizeof(*g));
+  g->dtype = dtype;
+  g->esize = dtype == VH_F64 ? 8 : 4;
+  GridDev& G = g->G;
+  G.rank = rank;
+  long long n = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < rank) {
+      if (dims[d] < 1 || dims[d] > (1 << 30)) { delete g; return fail(-1, "dims[%s] out of range (%lld)", "", dims[d]); }
+      if (!(hi[d] > lo[d])) { delete g; return fail(-1, "domain needs hi > lo%s (axis %lld)", "", d); }
+      G.n[d] = (int)dims[d];
+      G.w[d] = (hi[d] - lo[d]) / (double)dims[d];
+      for (int s = 0; s < 2; ++s) {
+        const int b = bc[2 * d + s];
+        if (b < 0 || b > 2) { delete g; return fail(-1, "unknown boundary code%s %lld", "", b); }
+        G.bc[2 * d + s] = b;
+      }
+      if ((G.bc[2 * d] == BC_PERIODIC) != (G.bc[2 * d + 1] == BC_PERIODIC)) {
+        delete g;
+        return fail(-1, "periodic must be set on both sides of an axis%s (axis %lld)", "", d);
+      }
+      if (G.bc[2 * d] == BC_PERIODIC && dims[d] < 2) {
+        delete g;
+        return fail(-1, "a periodic axis needs at least 2 cells%s (axis %lld)", "", d);
+      }
+    } else {
+      G.n[d] = 1;
+      G.w[d] = 1.0;
+      G.bc[2 * d] = G.bc[2 * d + 1] = BC_OUTFLOW;
+    }
+    n *= G.n[d];
+  }
+  G.ncell = n;
+  G.gamma = gamma;
+  // lib/RefMesh/src/RefMesh.jl:127 and :135-142 (left folds in Float64).
+  double V = G.w[0];
+  for (int d = 1; d < rank; ++d) V = V * G.w[d];
+  G.V = V;
+  for (int a = 0; a < 3; ++a) {
+    double ar = 1.0;
+    for (int d = 0; d < rank; ++d)
+      if (d != a) ar *= G.w[d];
+    G.area[a] = ar;
+  }
+  if (hipGetDevice(&g->device) != hipSuccess) { delete g; return fail(-2, "no HIP device%s", ""); }
+  int rc = configure(g, 0, 0);
+  if (rc) { vh_grid_destroy(g); return rc; }
+  const size_t bytes = (size_t)5 * n * g->esize;
+  if (hipMalloc(&g->U, bytes) != hipSuccess || hipMalloc(&g->U1, bytes) != hipSuccess ||
+      hipMalloc(&g->clock, 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&g->totals_partials, 5 * kMaxCflBlocks * sizeof(double)) != hipSuccess) {
+    vh_grid_destroy(g);
+    return fail(-2, "device allocation of %s%lld bytes failed", "", (long long)(2 * bytes));
+  }
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+    vh_grid_destroy(g);
+    return fail(-2, "stream creation failed%s", "");
+  }
+  VH_HIP(hipMemsetAsync(g->clock, 0, 4 * sizeof(double), g->stream));
+  VH_HIP(hipMemsetAsync(g->U, 0, bytes, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  const long long cb = (n + 255) / 256;
+  g->cfl_blocks = (int)(cb < kMaxCflBlocks ? cb : kMaxCflBlocks);
+  *out = g;
+  return 0;
+}
+
+int vh_grid_destroy(vh_grid* g) {
+  if (!g) return 0;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->U) (void)hipFree(g->U);
+  if (g->U1) (void)hipFree(g->U1);
+  if (g->partials) (void)hipFree(g->partials);
+  if (g->clock) (void)hipFree(g->clock);
+  if (g->totals_partials) (void)hipFree(g->totals_partials);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return 0;
+}
+
+int64_t vh_grid_ncell(const vh_grid* g) { return g ? g->G.ncell : -1; }
+
+int vh_grid_upload(vh_grid* g, const void* U) {
+  if (!g || !U) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipMemcpyAsync(g->U, U, (size_t)5 * g->G.ncell * g->esize, hipMemcpyHostToDevice, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_download(vh_grid* g, void* U) {
+  if (!g || !U) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipMemcpyAsync(U, g->U, (size_t)5 * g->G.ncell * g->esize, hipMemcpyDeviceToHost, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_state_device_ptr(vh_grid* g, void** dptr) {
+  if (!g || !dptr) return fail(-1, "null argument%s", "");
+  *dptr = g->U;
+  return 0;
+}
+
+static void enqueue_cfl(vh_grid* g) {
+  if (g->dtype == VH_F64)
+    launch_cfl<double, false>(g->G.rank, g->cfl_blocks, g->stream, (const double*)g->U, g->G, g->partials);
+  else
+    launch_cfl<float, true>(g->G.rank, g->cfl_blocks, g->stream, (const float*)g->U, g->G, g->partials);
+}
+
+// Enqueue one RK2 step reading dt from clock[1]; stage 2 leaves the partial CFL
+// maxima of the new state in g->partials (g->stage_blocks entries).
+static int enqueue_step(vh_grid* g) {
+  const double* dtp = g->clock + 1;
+  if (g->dtype == VH_F64) {
+    double* U = (double*)g->U;
+    double* U1 = (double*)g->U1;
+    launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, U1, dtp,
+                                g->partials, g->G);
+    launch_stage<double, false>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1, U, U, dtp,
+                                g->partials, g->G);
+  } else {
+    float* U = (float*)g->U;
+    float* U1 = (float*)g->U1;
+    launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, U1, dtp,
+                              g->partials, g->G);
+    launch_stage<float, true>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1, U, U, dtp,
+                              g->partials, g->G);
+  }
+  hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
+  VH_HIP(hipGetLastError());
+  return 0;
+}
+
+int vh_grid_compute_dt(vh_grid* g, double cfl, double* dt) {
+  if (!g || !dt) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  enqueue_cfl(g);
+  hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, g->cfl_blocks, cfl, 0.0,
+                     0, g->clock);
+  VH_HIP(hipGetLastError());
+  double clk[4];
+  VH_HIP(hipMemcpyAsync(clk, g->clock, sizeof clk, hipMemcpyDeviceToHost, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  *dt = clk[1];
+  return 0;
+}
+
+int vh_grid_step(vh_grid* g, double dt) {
+  if (!g) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, dt);
+  return enqueue_step(g);
+}
+
+int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double* t_io, int64_t* steps) {
+  if (!g || !t_io) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  double clk[4] = {*t_io, 0.0, 0.0, 0.0};
+  VH_HIP(hipMemcpyAsync(g->clock, clk, sizeof clk, hipMemcpyHostToDevice, g->stream));
+  int64_t k = 0;
+  const bool clip = tfinal > 0.0;
+  if (!clip && max_steps <= 0) return fail(-1, "need tfinal > 0 or max_steps > 0%s", "");
+  // Initial CFL partials from the current state.
+  enqueue_cfl(g);
+  int np = g->cfl_blocks;
+  double t = *t_io;
+  while (true) {
+    if (max_steps > 0 && k >= max_steps) break;
+    if (clip && !(t < tfinal * (1 - 1e-12))) break;
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, tfinal,
+                       clip ? 1 : 0, g->clock);
+    int rc = enqueue_step(g);
+    if (rc) return rc;
+    np = g->stage_blocks;
+    ++k;
+    if (clip) {
+      VH_HIP(hipMemcpyAsync(clk, g->clock, sizeof clk, hipMemcpyDeviceToHost, g->stream));
+      VH_HIP(hipStreamSynchronize(g->stream));
+      t = clk[0];
+    }
+  }
+  VH_HIP(hipMemcpyAsync(clk, g->clock, sizeof clk, hipMemcpyDeviceToHost, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  *t_io = clk[0];
+  if (steps) *steps = k;
+  return 0;
+}
+
+int vh_grid_totals(vh_grid* g, double* out5) {
+  if (!g || !out5) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  const int blocks = g->cfl_blocks;
+  if (g->dtype == VH_F64)
+    hipLaunchKernelGGL(totals_kernel<double>, dim3(blocks), dim3(256), 0, g->stream, (const double*)g->U, g->G.ncell,
+                       g->G.V, g->totals_partials);
+  else
+    hipLaunchKernelGGL(totals_kernel<float>, dim3(blocks), dim3(256), 0, g->stream, (const float*)g->U, g->G.ncell,
+                       g->G.V, g->totals_partials);
+  VH_HIP(hipGetLastError());
+  double* h = new double[5 * blocks];
+  hipError_t e = hipMemcpyAsync(h, g->totals_partials, sizeof(double) * 5 * blocks, hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) { delete[] h; return fail(-2, "HIP error %s in totals%s", hipGetErrorString(e)); }
+  for (int f = 0; f < 5; ++f) out5[f] = 0.0;
+  for (int b = 0; b < blocks; ++b)
+    for (int f = 0; f < 5; ++f) out5[f] += h[5 * b + f];
+  delete[] h;
+  return 0;
+}
+
+int vh_grid_accumulate_flux(vh_grid* g, void* acc_host) {
+  if (!g || !acc_host) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  // Stage 1 with dt = −V gives invV = Tf(−V)/(Tf(1)·Tf(V)) = −1 exactly, so
+  // U1 = U + acc; acc is recovered as U1 − U (rounded; inspection only).
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, -g->G.V);
+  const double* dtp = g->clock + 1;
+  const long long n5 = 5 * g->G.ncell;
+  const unsigned blocks = (unsigned)((n5 + 255) / 256);
+  if (g->dtype == VH_F64) {
+    launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (const double*)g->U,
+                                (const double*)g->U, (double*)g->U1, dtp, g->partials, g->G);
+    hipLaunchKernelGGL(sub_kernel<double>, dim3(blocks), dim3(256), 0, g->stream, (const double*)g->U1,
+                       (const double*)g->U, (double*)g->U1, n5);
+  } else {
+    launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (const float*)g->U,
+                              (const float*)g->U, (float*)g->U1, dtp, g->partials, g->G);
+    hipLaunchKernelGGL(sub_kernel<float>, dim3(blocks), dim3(256), 0, g->stream, (const float*)g->U1,
+                       (const float*)g->U, (float*)g->U1, n5);
+  }
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(acc_host, g->U1, (size_t)n5 * g->esize, hipMemcpyDeviceToHost, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_sync(vh_grid* g) {
+  if (!g) return fail(-1, "null argument%s", "");
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_set_launch(vh_grid* g, int32_t tile_variant, int32_t zchunk) {
+  if (!g) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return configure(g, tile_variant, zchunk);
+}
+
+int vh_grid_tile_count(const vh_grid* g) { return g ? tile_count_for_rank(g->G.rank) : -1; }
+
+int vh_grid_get_launch(const vh_grid* g, int32_t* tile_x, int32_t* tile_y, int32_t* zchunk, int64_t* workgroups) {
+  if (!g) return fail(-1, "null argument%s", "");
+  if (tile_x) *tile_x = g->tx;
+  if (tile_y) *tile_y = g->ty;
+  if (zchunk) *zchunk = g->G.zchunk;
+  if (workgroups) *workgroups = g->stage_blocks;
+  return 0;
+}
+
+int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1, double* ms_stage2,
+                       double* ms_step) {
+  if (!g || nsteps <= 0 || nsteps > 100000) return fail(-1, "bad argument%s (nsteps %lld)", "", nsteps);
+  VH_HIP(hipSetDevice(g->device));
+  // Three events per step, recorded asynchronously; one synchronisation at the end.
+  const int64_t ne = 3 * nsteps;
+  hipEvent_t* ev = new hipEvent_t[ne]();
+  int rc = 0;
+  for (int64_t i = 0; i < ne && !rc; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(-2, "hipEventCreate failed%s", "");
+  if (!rc) {
+    enqueue_cfl(g);
+    int np = g->cfl_blocks;
+    const double* dtp = g->clock + 1;
+    for (int64_t k = 0; k < nsteps; ++k) {
+      hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, 0.0, 0,
+                         g->clock);
+      (void)hipEventRecord(ev[3 * k], g->stream);
+      if (g->dtype == VH_F64) {
+        launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (double*)g->U,
+                                    (double*)g->U, (double*)g->U1, dtp, g->partials, g->G);
+        (void)hipEventRecord(ev[3 * k + 1], g->stream);
+        launch_stage<double, false>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, (double*)g->U1,
+                                    (double*)g->U, (double*)g->U, dtp, g->partials, g->G);
+      } else {
+        launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (float*)g->U,
+                                  (float*)g->U, (float*)g->U1, dtp, g->partials, g->G);
+        (void)hipEventRecord(ev[3 * k + 1], g->stream);
+        launch_stage<float, true>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, (float*)g->U1,
+                                  (float*)g->U, (float*)g->U, dtp, g->partials, g->G);
+      }
+      (void)hipEventRecord(ev[3 * k + 2], g->stream);
+      hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
+      np = g->stage_blocks;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = fail(-2, "HIP error %s while timing%s", hipGetErrorString(e));
+  }
+  double s1 = 0, s2 = 0;
+  float tot = 0;
+  if (!rc) {
+    for (int64_t k = 0; k < nsteps; ++k) {
+      float a = 0, b = 0;
+      (void)hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+      s1 += a;
+      s2 += b;
+    }
+    (void)hipEventElapsedTime(&tot, ev[0], ev[ne - 1]);
+  }
+  for (int64_t i = 0; i < ne; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  delete[] ev;
+  if (rc) return rc;
+  if (ms_stage1) *ms_stage1 = s1 / nsteps;
+  if (ms_stage2) *ms_stage2 = s2 / nsteps;
+  // First stage-1 start to last stage-2 end, per step (includes the dt reduction
+  // and clock kernels between steps).
+  if (ms_step) *ms_step = (double)tot / nsteps;
+  return 0;
+}
+
+}  // extern "C"
