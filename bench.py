@@ -53,25 +53,15 @@ def make_sim(V, n, dtype, U, tile, zchunk):
     return sim
 
 
-def sync_all():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except ImportError:
-        pass
-
-
 def timed_run(V, sim, steps, warmup):
     """Warm up, then time exactly `steps` steps (wall clock, synchronised on both
     sides) with per-stage HIP events recorded on the library's stream."""
     if warmup > 0:
         V.run_steps(sim, warmup)
     sim.sync()
-    sync_all()
     t0 = time.perf_counter()
-    ev = sim.time_steps(steps)
-    sync_all()
+    ev = sim.time_steps(steps)          # synchronises the library's stream at the end
+    sim.sync()
     wall = time.perf_counter() - t0
     return wall, ev
 

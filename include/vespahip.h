@@ -65,6 +65,13 @@ int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* da
 int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf,
                          double gamma, float* dF, void* stream);
 
+/* Device buffers for callers without a GPU array package of their own (the
+ * reference's KA.allocate / copyto!, ext/VespaKAFluxExt.jl:51-54,59). */
+int vh_buffer_alloc(void** dptr, int64_t bytes);
+int vh_buffer_free(void* dptr);
+int vh_buffer_upload(void* dptr, const void* host, int64_t bytes);
+int vh_buffer_download(void* host, const void* dptr, int64_t bytes);
+
 /* ---------------------------------------------------------------------------
  * Tier 2 — device-resident uniform grid (the RefMesh single-level path).
  * Replaces the whole-stage unit step!(sim, dt) (src/driver.jl:333-359) with its

@@ -60,19 +60,17 @@ def test_face_flux_empty_batch_is_noop(V):
 
 
 def test_face_flux_device_pointer_form(V, O):
-    import torch
     WL, WR, ax = random_faces(10_000, seed=5)
-    dev = torch.device("cuda:0")
-    tWL = torch.from_numpy(WL).to(dev)
-    tWR = torch.from_numpy(WR).to(dev)
-    tax = torch.from_numpy(ax).to(dev)
-    tF = torch.empty_like(tWL)
+    bWL = V.DeviceBuffer(WL.nbytes).upload(WL)
+    bWR = V.DeviceBuffer(WR.nbytes).upload(WR)
+    bax = V.DeviceBuffer(ax.nbytes).upload(ax)
+    bF = V.DeviceBuffer(WL.nbytes)
     L = V.load_library()
-    torch.cuda.synchronize()
-    V.check(L.vh_face_flux_dev_f64(tWL.data_ptr(), tWR.data_ptr(), tax.data_ptr(), len(ax), 1.4,
-                                   tF.data_ptr(), None))
-    torch.cuda.synchronize()
-    assert np.array_equal(tF.cpu().numpy(), O.face_flux(WL, WR, ax, 1.4))
+    V.check(L.vh_face_flux_dev_f64(bWL.ptr, bWR.ptr, bax.ptr, len(ax), 1.4, bF.ptr, None))
+    F = bF.download(np.empty_like(WL))          # hipMemcpy orders after the null-stream kernel
+    assert np.array_equal(F, O.face_flux(WL, WR, ax, 1.4))
+    for b in (bWL, bWR, bax, bF):
+        b.free()
 
 
 # --------------------------------------------------------------------------

@@ -6,13 +6,14 @@ flux-backend interface (see ``hydro.py``). The directory name contains a dot, so
 load it with ``importlib`` under the module name ``vespa_jl_amd`` (as the tests,
 ``bench.py`` and ``__graft_entry__.py`` do).
 """
-from ._lib import (BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_F32, VH_F64,
+from ._lib import (DeviceBuffer, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_F32, VH_F64,
                    VespaHIPError, check, device_count, load_library)
 from .hydro import (HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
                     dump_fields, evolve, prim2cons_array, run_steps, sod_problem, step)
 
 __all__ = [
+    "DeviceBuffer",
     "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_F32", "VH_F64",
     "VespaHIPError", "check", "device_count", "load_library",
     "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",

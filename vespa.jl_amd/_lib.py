@@ -18,6 +18,7 @@ BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
 EXPORTED = (
     "vh_last_error", "vh_abi_version", "vh_device_count",
     "vh_face_flux_f64", "vh_face_flux_f32", "vh_face_flux_dev_f64", "vh_face_flux_dev_f32",
+    "vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download",
     "vh_grid_create", "vh_grid_destroy", "vh_grid_ncell", "vh_grid_upload", "vh_grid_download",
     "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
     "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
@@ -55,6 +56,12 @@ def load_library(path: str | None = None):
         f = getattr(L, f"vh_face_flux_dev_{name}")
         f.argtypes = [vp, vp, vp, C.c_int64, C.c_double, vp, vp]
         f.restype = C.c_int
+    L.vh_buffer_alloc.argtypes = [C.POINTER(vp), C.c_int64]
+    L.vh_buffer_free.argtypes = [vp]
+    L.vh_buffer_upload.argtypes = [vp, vp, C.c_int64]
+    L.vh_buffer_download.argtypes = [vp, vp, C.c_int64]
+    for n in ("vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download"):
+        getattr(L, n).restype = C.c_int
     L.vh_grid_create.argtypes = [C.POINTER(vp), C.c_int32, i64p, dp, dp, i32p, C.c_double, C.c_int32]
     L.vh_grid_destroy.argtypes = [vp]
     L.vh_grid_ncell.argtypes = [vp]
@@ -86,6 +93,41 @@ def check(rc: int) -> None:
     if rc != 0:
         msg = load_library().vh_last_error().decode(errors="replace")
         raise VespaHIPError(f"vespahip call failed ({rc}): {msg}")
+
+
+class DeviceBuffer:
+    """A raw device allocation owned by the library (hipMalloc/hipFree)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load_library().vh_buffer_alloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    def upload(self, arr) -> "DeviceBuffer":
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        check(load_library().vh_buffer_upload(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return self
+
+    def download(self, out):
+        if out.nbytes > self.nbytes or not out.flags.c_contiguous:
+            raise ValueError("output must be contiguous and fit the buffer")
+        check(load_library().vh_buffer_download(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            check(load_library().vh_buffer_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_count() -> int:

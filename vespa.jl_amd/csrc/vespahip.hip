@@ -345,6 +345,28 @@ int vh_face_flux_f32(const float* WL, const float* WR, const int32_t* axis, int6
   return face_flux_host<float>(WL, WR, axis, nf, gamma, F);
 }
 
+int vh_buffer_alloc(void** dptr, int64_t bytes) {
+  if (!dptr || bytes < 0) return fail(-1, "bad argument%s", "");
+  *dptr = nullptr;
+  if (bytes == 0) return 0;
+  VH_HIP(hipMalloc(dptr, (size_t)bytes));
+  return 0;
+}
+int vh_buffer_free(void* dptr) {
+  if (dptr) VH_HIP(hipFree(dptr));
+  return 0;
+}
+int vh_buffer_upload(void* dptr, const void* host, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!dptr || !host))) return fail(-1, "bad argument%s", "");
+  if (bytes) VH_HIP(hipMemcpy(dptr, host, (size_t)bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+int vh_buffer_download(void* host, const void* dptr, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!dptr || !host))) return fail(-1, "bad argument%s", "");
+  if (bytes) VH_HIP(hipMemcpy(host, dptr, (size_t)bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // ---- tier 2 ----
 
 int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const double* lo, const double* hi,
