@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also time 3 device-to-device copies of 1 GiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -113,6 +115,20 @@ def main():
     ncell = n ** 3
     log(f"[bench] building {n}^3 turbulence-like initial state")
     U64 = P.turbulence_state(n)
+
+    calib = None
+    if args.calib_copy:
+        nb = 1 << 30
+        a, b = V.DeviceBuffer(nb), V.DeviceBuffer(nb)
+        b.copy_from(a)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            b.copy_from(a)
+        el = (time.perf_counter() - t0) / 3
+        calib = {"copy_bytes": nb, "copy_GBps_read_plus_write": 2 * nb / el / 1e9}
+        a.free()
+        b.free()
+        log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
 
     log("[bench] FP64 run")
     sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk)
@@ -163,6 +179,9 @@ def main():
                         "stage1_GBps": stage1_bytes / (ev["stage1_ms"] * 1e-3) / 1e9,
                         "step_algorithmic_GBps": 200 * ncell / (ms_per_step * 1e-3) / 1e9},
     }
+
+    if calib is not None:
+        out["calibration"] = calib
 
     if not args.no_fp32:
         log("[bench] FP32 run")

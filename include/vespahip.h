@@ -33,6 +33,14 @@ extern "C" {
 
 enum { VH_F64 = 0, VH_F32 = 1 };
 enum { VH_BC_PERIODIC = 0, VH_BC_OUTFLOW = 1, VH_BC_REFLECTING = 2 };
+/* Arithmetic profile of a grid context's stage kernels:
+ *   VH_EXACT  the reference's evaluation order, no FMA contraction, correctly
+ *             rounded division and square root: bit-identical to the reference
+ *             arithmetic (the default);
+ *   VH_FAST   FMA contraction, refined hardware reciprocals/rsqrt and fewer
+ *             divisions in the HLLC star state: agrees with VH_EXACT to a few
+ *             ulps per face. */
+enum { VH_EXACT = 0, VH_FAST = 1 };
 
 /* Last error message of the calling thread ("" if none). */
 const char* vh_last_error(void);
@@ -71,6 +79,7 @@ int vh_buffer_alloc(void** dptr, int64_t bytes);
 int vh_buffer_free(void* dptr);
 int vh_buffer_upload(void* dptr, const void* host, int64_t bytes);
 int vh_buffer_download(void* host, const void* dptr, int64_t bytes);
+int vh_buffer_copy(void* ddst, const void* dsrc, int64_t bytes);
 
 /* ---------------------------------------------------------------------------
  * Tier 2 — device-resident uniform grid (the RefMesh single-level path).
@@ -89,8 +98,13 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
                    const double* hi, const int32_t* bc, double gamma, int32_t dtype);
 int vh_grid_destroy(vh_grid* g);
 
-/* Number of cells and the element size (8 or 4 bytes) of the state. */
+/* Number of cells of the grid. */
 int64_t vh_grid_ncell(const vh_grid* g);
+
+/* Select / query the arithmetic profile (VH_EXACT or VH_FAST) of the stage
+ * kernels of this context. The tier-1 face-flux entry points are always exact. */
+int vh_grid_set_mode(vh_grid* g, int32_t mode);
+int vh_grid_get_mode(const vh_grid* g);
 
 /* Copy the conserved state (5*ncell values, layout above) from / to the host.
  * Replaces set_initial_conditions!/field views (src/driver.jl:101-112). */

@@ -168,6 +168,52 @@ def test_sod_400_full_run_bitwise_and_exact(V, O):
     assert np.abs(Ug[0] - ex).mean() < 0.01
 
 
+def _field_rel(a, b):
+    """Per-field max |a − b| / max |b| (the 1e-12 relative bar of the fast profile)."""
+    a = a.reshape(5, -1).astype(np.float64)
+    b = b.reshape(5, -1).astype(np.float64)
+    return np.abs(a - b).max(axis=1) / np.maximum(np.abs(b).max(axis=1), 1e-300)
+
+
+FAST_TOL = 1e-12
+
+
+@pytest.mark.parametrize("case", [1, 3, 5, 6, 7, 8])
+def test_fast_profile_within_1e12_of_oracle(V, O, case):
+    dims, bcs = CASES[case]
+    _log(f"fast profile dims={dims} bcs={bcs}")
+    U = random_state(tuple(reversed(dims)), seed=case, dtype=np.float64, vmax=0.3)
+    sim = make_sim(V, dims, bcs, 1.4, np.float64, U)
+    sim.set_mode("fast")
+    assert sim.mode == "fast"
+    g = oracle_grid(O, dims, bcs, 1.4)
+    Uo = U.copy()
+    for k in range(3):
+        dt = O.compute_dt(g, Uo, 0.4)
+        V.step(sim, dt)
+        O.step(g, Uo, dt)
+    rel = _field_rel(sim.state(), Uo)
+    _log(f"  per-field rel diff {rel}")
+    assert (rel <= FAST_TOL).all(), rel
+    sim.close()
+
+
+def test_fast_profile_sod_400(V, O):
+    """Fast profile through the whole Sod run (shocks, limiter switches)."""
+    prob = V.sod_problem(400)
+    sim = V.Simulation(V.UniformMesh(prob.dims, prob.domain), prob)
+    sim.set_mode("fast")
+    U0 = sim.state().copy()
+    V.evolve(sim)
+    g = O.make_grid((400,), ((0.0, 1.0),), "outflow", 1.4)
+    t, k = O.evolve(g, U0, 0.2, 0.4)
+    assert sim.step_count == k and sim.t == pytest.approx(t, rel=1e-12)
+    rel = _field_rel(sim.state(), U0)
+    _log(f"fast Sod-400 per-field rel diff {rel}")
+    assert (rel <= 1e-10).all(), rel
+    sim.close()
+
+
 def test_totals_match_oracle(V, O):
     dims = (30, 20, 10)
     U = random_state((10, 20, 30), seed=2)
@@ -214,6 +260,38 @@ def test_256cubed_one_step_bitwise_vs_oracle(V, O, turb256):
         rel = np.abs(Ug - Uo) / np.maximum(np.abs(Uo), 1e-300)
         pytest.fail(f"not bit-identical: max rel diff {rel.max():.3e}")
     sim.close()
+
+
+def test_256cubed_fast_profile(V, O, turb256):
+    """Fast profile at the full size: one step against the oracle, and 100
+    device-dt steps against the exact profile (which is bit-identical to the
+    oracle step by step), both within 1e-12 relative per field."""
+    _log("256^3 fast profile: 1 step vs oracle, 100 steps vs exact")
+    dims = (256, 256, 256)
+    g = O.make_grid(dims, [(0.0, 1.0)] * 3, "periodic", 5.0 / 3.0)
+    dt = O.compute_dt_omp(g, turb256, 0.3, threads=16)
+    Uo = turb256.copy()
+    O.step_omp(g, Uo, dt, threads=16)
+    fast = make_sim(V, dims, [(0, 0)] * 3, 5.0 / 3.0, np.float64, turb256)
+    fast.problem.cfl = 0.3
+    fast.set_mode("fast")
+    V.step(fast, dt)
+    rel1 = _field_rel(fast.state(), Uo)
+    _log(f"  1 step: {rel1}")
+    assert (rel1 <= FAST_TOL).all()
+    del Uo
+    exact = make_sim(V, dims, [(0, 0)] * 3, 5.0 / 3.0, np.float64, turb256)
+    exact.problem.cfl = 0.3
+    fast.upload(turb256)
+    fast.t = 0.0
+    V.run_steps(fast, 100)
+    V.run_steps(exact, 100)
+    assert fast.t == pytest.approx(exact.t, rel=1e-12)
+    rel = _field_rel(fast.state(), exact.state())
+    _log(f"  100 steps: {rel}")
+    assert (rel <= FAST_TOL).all()
+    fast.close()
+    exact.close()
 
 
 def test_256cubed_conservation_and_fp32_error(V, O, turb256):

@@ -6,7 +6,7 @@ flux-backend interface (see ``hydro.py``). The directory name contains a dot, so
 load it with ``importlib`` under the module name ``vespa_jl_amd`` (as the tests,
 ``bench.py`` and ``__graft_entry__.py`` do).
 """
-from ._lib import (DeviceBuffer, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_F32, VH_F64,
+from ._lib import (DeviceBuffer, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_EXACT, VH_F32, VH_F64, VH_FAST,
                    VespaHIPError, check, device_count, load_library)
 from .hydro import (HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
@@ -14,7 +14,7 @@ from .hydro import (HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting,
 
 __all__ = [
     "DeviceBuffer",
-    "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_F32", "VH_F64",
+    "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_EXACT", "VH_F32", "VH_F64", "VH_FAST",
     "VespaHIPError", "check", "device_count", "load_library",
     "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",
     "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",

@@ -9,9 +9,11 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libvespahip.so")
+# VESPAHIP_LIB overrides the library location (e.g. an alternative build).
+LIB_PATH = os.environ.get("VESPAHIP_LIB") or os.path.join(PKG_DIR, "libvespahip.so")
 
 VH_F64, VH_F32 = 0, 1
+VH_EXACT, VH_FAST = 0, 1
 BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
 
 # Every symbol include/vespahip.h declares (checked by the CPU test suite).
@@ -19,7 +21,8 @@ EXPORTED = (
     "vh_last_error", "vh_abi_version", "vh_device_count",
     "vh_face_flux_f64", "vh_face_flux_f32", "vh_face_flux_dev_f64", "vh_face_flux_dev_f32",
     "vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download",
-    "vh_grid_create", "vh_grid_destroy", "vh_grid_ncell", "vh_grid_upload", "vh_grid_download",
+    "vh_buffer_copy",
+    "vh_grid_create", "vh_grid_destroy", "vh_grid_ncell", "vh_grid_set_mode", "vh_grid_get_mode", "vh_grid_upload", "vh_grid_download",
     "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
     "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
     "vh_grid_tile_count", "vh_grid_get_launch", "vh_grid_time_steps",
@@ -60,12 +63,18 @@ def load_library(path: str | None = None):
     L.vh_buffer_free.argtypes = [vp]
     L.vh_buffer_upload.argtypes = [vp, vp, C.c_int64]
     L.vh_buffer_download.argtypes = [vp, vp, C.c_int64]
-    for n in ("vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download"):
+    L.vh_buffer_copy.argtypes = [vp, vp, C.c_int64]
+    for n in ("vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download",
+              "vh_buffer_copy"):
         getattr(L, n).restype = C.c_int
     L.vh_grid_create.argtypes = [C.POINTER(vp), C.c_int32, i64p, dp, dp, i32p, C.c_double, C.c_int32]
     L.vh_grid_destroy.argtypes = [vp]
     L.vh_grid_ncell.argtypes = [vp]
     L.vh_grid_ncell.restype = C.c_int64
+    L.vh_grid_set_mode.argtypes = [vp, C.c_int32]
+    L.vh_grid_set_mode.restype = C.c_int
+    L.vh_grid_get_mode.argtypes = [vp]
+    L.vh_grid_get_mode.restype = C.c_int
     L.vh_grid_upload.argtypes = [vp, vp]
     L.vh_grid_download.argtypes = [vp, vp]
     L.vh_grid_state_device_ptr.argtypes = [vp, C.POINTER(vp)]
@@ -117,6 +126,11 @@ class DeviceBuffer:
             raise ValueError("output must be contiguous and fit the buffer")
         check(load_library().vh_buffer_download(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
         return out
+
+    def copy_from(self, other: "DeviceBuffer", nbytes: int | None = None) -> "DeviceBuffer":
+        n = min(self.nbytes, other.nbytes) if nbytes is None else int(nbytes)
+        check(load_library().vh_buffer_copy(self.ptr, other.ptr, n))
+        return self
 
     def free(self):
         if self.ptr:

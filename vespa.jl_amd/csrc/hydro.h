@@ -1,52 +1,93 @@
 // Device-side ideal-gas hydro kernels for the flux-divergence path (gfx950).
 //
 // These are the per-face / per-cell building blocks shared by the batched
-// face-flux kernel and the fused stencil stage kernels. Each expression keeps
-// the evaluation order of the reference so that, compiled without FMA
-// contraction, results are bit-identical to the CPU restatement:
-//   cons2prim      src/eos.jl:8-25
-//   prim2cons / E  src/eos.jl:28-32 (E = p/(γ-1) + (½ρ)((vx²+vy²)+vz²), a division)
-//   minmod         src/reconstruct.jl:6-16 (a·b ≤ 0 → 0; |a| < |b| ? a : b)
-//   euler_flux     src/riemann.jl:9-20
-//   hllc_flux      src/riemann.jl:33-66, _hllc_star src/riemann.jl:69-88
+// face-flux kernel and the fused stencil stage kernels.
 //
-// The HLLC evaluation is restructured, not re-derived: the upwind side K is
-// selected first (SL ≥ 0 → L; SR ≤ 0 → R; else sign of S*), and only that
-// side's E, physical flux and star state are formed. Every value that reaches
-// the output is produced by the same operations as in the reference, so the
-// result is the reference's bit for bit; the other side's flux is never needed.
+// Two arithmetic profiles are compiled from this one source (VH_FAST_PROFILE = 0 / 1):
+//
+// * exact (VH_FAST_PROFILE = 0, built with -ffp-contract=off): every expression keeps
+//   the reference's evaluation order, so results are bit-identical to the CPU
+//   restatement:
+//     cons2prim      src/eos.jl:8-25
+//     prim2cons / E  src/eos.jl:28-32 (E = p/(γ-1) + (½ρ)((vx²+vy²)+vz²), a division)
+//     minmod         src/reconstruct.jl:6-16 (a·b ≤ 0 → 0; |a| < |b| ? a : b)
+//     euler_flux     src/riemann.jl:9-20
+//     hllc_flux      src/riemann.jl:33-66, _hllc_star src/riemann.jl:69-88
+//   The HLLC evaluation selects the upwind side K first (SL ≥ 0 → L; SR ≤ 0 → R;
+//   else the sign of S*) and forms only that side's E, physical flux and star
+//   state; every value reaching the output is produced by the same operations
+//   as in the reference.
+//
+// * fast (VH_FAST_PROFILE = 1, built with FMA contraction): the same formulas with
+//   a*b+c fused, E = p·(1/(γ−1)), sound speeds c = (γp)·rsqrt(γpρ) refined by
+//   Newton steps, divisions through a refined reciprocal, and the star state
+//   rearranged so that one reciprocal 1/(S_K − S*) replaces the three
+//   divisions by ρ, by ρ(S_K − u_n) and by (S_K − S*):
+//     F*_K = F_K + S_K·(U*_K − U_K),   U*_K = r·(ρ, ρv*, ·) with r = (S_K − u_n)/(S_K − S*),
+//     ρ_*E_* = r·E + ρ·r·(S* − u_n)·(S* + p/(ρ(S_K − u_n)))
+//            = [(S_K − u_n)·E + (S* − u_n)·(ρ(S_K − u_n)·S* + p)] / (S_K − S*).
+//   Results agree with the exact profile to a few ulps per face.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "common.h"
+
 namespace vh {
+namespace VH_PROFILE_NS {
 
-enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
-
-template <typename T>
-struct Prim {
-  T r, vx, vy, vz, p;
-};
-
-template <typename T>
-struct Cons {
-  T d, mx, my, mz, e;
-};
-
-template <typename T>
-__device__ __forceinline__ T vabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
-template <>
-__device__ __forceinline__ double vabs<double>(double x) { return __builtin_fabs(x); }
-template <>
-__device__ __forceinline__ float vabs<float>(float x) { return __builtin_fabsf(x); }
-
+__device__ __forceinline__ double vabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double vsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float vsqrt(float x) { return __builtin_sqrtf(x); }
-__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
-__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
-__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Reciprocal from the hardware estimate refined by two Newton steps
+// (e = 1 − b·r; r += r·e), then a/b as a·r with one residual correction.
+__device__ __forceinline__ double fast_rcp(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = vfma(vfma(-b, r, 1.0), r, r);
+  r = vfma(vfma(-b, r, 1.0), r, r);
+  return r;
+}
+__device__ __forceinline__ float fast_rcp(float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  return vfma(vfma(-b, r, 1.0f), r, r);
+}
+template <typename T>
+__device__ __forceinline__ T fast_div(T a, T b) {
+  const T r = fast_rcp(b);
+  const T q = a * r;
+  return vfma(vfma(-b, q, a), r, q);
+}
+// sqrt(x/y) for positive x, y: x·rsqrt(x·y) with two Newton steps on the rsqrt.
+__device__ __forceinline__ double fast_sqrt_ratio(double x, double y) {
+  const double a = x * y;
+  double s = __builtin_amdgcn_rsq(a);
+  const double h = 0.5 * a;
+  s = s * vfma(-h * s, s, 1.5);
+  s = s * vfma(-h * s, s, 1.5);
+  return x * s;
+}
+__device__ __forceinline__ float fast_sqrt_ratio(float x, float y) {
+  const float a = x * y;
+  float s = __builtin_amdgcn_rsqf(a);
+  const float h = 0.5f * a;
+  s = s * vfma(-h * s, s, 1.5f);
+  return x * s;
+}
+
+// Reciprocal used by cons2prim (1/ρ).
+template <typename T>
+__device__ __forceinline__ T recip(T x) {
+#if VH_FAST_PROFILE
+  return fast_rcp(x);
+#else
+  return T(1) / x;
+#endif
+}
 
 // cons2prim (src/eos.jl:8-25). With FLOOR64 (the FP32 form) the pressure and its
 // positivity floor are evaluated in double from the Tf energy and kinetic terms.
@@ -54,7 +95,7 @@ template <typename T, bool FLOOR64>
 __device__ __forceinline__ Prim<T> cons2prim(const Cons<T>& U, T g, double gd) {
   Prim<T> W;
   const T rho = U.d;
-  const T inv = T(1) / rho;
+  const T inv = recip(rho);
   W.r = rho;
   W.vx = U.mx * inv;
   W.vy = U.my * inv;
@@ -70,10 +111,16 @@ __device__ __forceinline__ Prim<T> cons2prim(const Cons<T>& U, T g, double gd) {
   return W;
 }
 
-// Total energy from primitives (the E of prim2cons and euler_flux).
+// Total energy from primitives (the E of prim2cons and euler_flux). `igm1` is
+// 1/(γ−1) (used by the fast profile only).
 template <typename T>
-__device__ __forceinline__ T energy(const Prim<T>& W, T g) {
+__device__ __forceinline__ T energy(const Prim<T>& W, T g, T igm1) {
+#if VH_FAST_PROFILE
+  return W.p * igm1 + (T(0.5) * W.r) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
+#else
+  (void)igm1;
   return W.p / (g - T(1)) + (T(0.5) * W.r) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
+#endif
 }
 
 template <typename T>
@@ -123,25 +170,39 @@ __device__ __forceinline__ T normal_v(const Prim<T>& w, int dim) {
   return dim == 1 ? w.vx : (dim == 2 ? w.vy : w.vz);
 }
 
+template <typename T>
+__device__ __forceinline__ T sound(T g, T p, T rho) {
+#if VH_FAST_PROFILE
+  return fast_sqrt_ratio(g * p, rho);
+#else
+  return vsqrt(g * p / rho);
+#endif
+}
+
 // HLLC flux across a face normal to `dim` (1-based), src/riemann.jl:33-88.
 template <typename T>
-__device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T g, int dim) {
+__device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T g, T igm1, int dim) {
   const T rL = WL.r, rR = WR.r, pL = WL.p, pR = WR.p;
   const T unL = normal_v(WL, dim), unR = normal_v(WR, dim);
-  const T cL = vsqrt(g * pL / rL);
-  const T cR = vsqrt(g * pR / rR);
+  const T cL = sound(g, pL, rL);
+  const T cR = sound(g, pR, rR);
   const T SL = vmin(unL - cL, unR - cR);
   const T SR = vmax(unL + cL, unR + cR);
   const bool supL = SL >= T(0);
   const bool supR = !supL && (SR <= T(0));
+#if VH_FAST_PROFILE
+  const T Ss = fast_div(pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR),
+                        rL * (SL - unL) - rR * (SR - unR));
+#else
   const T Ss = (pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR)) /
                (rL * (SL - unL) - rR * (SR - unR));
+#endif
   const bool useL = supL || (!supR && Ss >= T(0));
 
   const Prim<T> W = useL ? WL : WR;
   const T SK = useL ? SL : SR;
   const T un = useL ? unL : unR;
-  const T E = energy(W, g);
+  const T E = energy(W, g, igm1);
   const T m1 = W.r * W.vx, m2 = W.r * W.vy, m3 = W.r * W.vz;
   Cons<T> F;
   F.d = W.r * un;
@@ -153,17 +214,24 @@ __device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T 
 
   // F*K = FK + SK (U*K − UK), src/riemann.jl:69-88.
   const T q = W.r * (SK - un);
-  const T factor = q / (SK - Ss);
-  const T Es = E / W.r + (Ss - un) * (Ss + W.p / q);
   const T vsx = dim == 1 ? Ss : W.vx;
   const T vsy = dim == 2 ? Ss : W.vy;
   const T vsz = dim == 3 ? Ss : W.vz;
+#if VH_FAST_PROFILE
+  const T inv = fast_rcp(SK - Ss);
+  const T factor = q * inv;
+  const T fE = ((SK - un) * E + (Ss - un) * (q * Ss + W.p)) * inv;
+#else
+  const T factor = q / (SK - Ss);
+  const T Es = E / W.r + (Ss - un) * (Ss + W.p / q);
+  const T fE = factor * Es;
+#endif
   Cons<T> o;
   o.d = F.d + SK * (factor - W.r);
   o.mx = F.mx + SK * (factor * vsx - m1);
   o.my = F.my + SK * (factor * vsy - m2);
   o.mz = F.mz + SK * (factor * vsz - m3);
-  o.e = F.e + SK * (factor * Es - E);
+  o.e = F.e + SK * (fE - E);
   return o;
 }
 
@@ -176,4 +244,5 @@ __device__ __forceinline__ void acc_add(Cons<T>& a, const Cons<T>& F, T s) {
   a.e += s * F.e;
 }
 
+}  // namespace VH_PROFILE_NS
 }  // namespace vh

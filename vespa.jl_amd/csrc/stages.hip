@@ -1,0 +1,98 @@
+// Kernel launchers of one arithmetic profile. This file is compiled twice:
+// VH_FAST_PROFILE=0 with -ffp-contract=off (exact, bit-identical to the reference's
+// evaluation order) and VH_FAST_PROFILE=1 with FMA contraction (fast).
+#include "profiles.h"
+#include "stencil.h"
+
+namespace vh {
+namespace VH_PROFILE_NS {
+
+// Tier-1 face batch: one lane per face (ext/VespaKAFluxExt.jl:23-35 contract).
+template <typename T>
+__global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL, const T* __restrict__ WR,
+                                                        const int32_t* __restrict__ axis, long long nf, T g,
+                                                        T igm1, T* __restrict__ F) {
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  Prim<T> L, R;
+  const T* l = WL + 5 * f;
+  const T* r = WR + 5 * f;
+  L.r = l[0]; L.vx = l[1]; L.vy = l[2]; L.vz = l[3]; L.p = l[4];
+  R.r = r[0]; R.vx = r[1]; R.vy = r[2]; R.vz = r[3]; R.p = r[4];
+  int d = axis[f];
+  d = d < 1 ? 1 : (d > 3 ? 3 : d);
+  const Cons<T> o = hllc(L, R, g, igm1, d);
+  T* out = F + 5 * f;
+  out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
+}
+
+template <typename T, int RANK, int TX, int TY, bool FLOOR64>
+static void stage_t(bool stage2, int blocks, hipStream_t s, const void* Us, const void* U0, void* Uout,
+                    const double* dtp, double* partials, const GridDev& G) {
+  if (stage2)
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
+                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+  else
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
+                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+}
+
+template <typename T, bool FLOOR64>
+static void stage_dispatch(int rank, int variant, bool stage2, int blocks, hipStream_t s, const void* Us,
+                           const void* U0, void* Uout, const double* dtp, double* partials, const GridDev& G) {
+  if (rank == 3) {
+    switch (variant) {
+      case 0: stage_t<T, 3, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 1: stage_t<T, 3, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 2: stage_t<T, 3, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: stage_t<T, 3, 64, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  } else if (rank == 2) {
+    switch (variant) {
+      case 0: stage_t<T, 2, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      case 1: stage_t<T, 2, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: stage_t<T, 2, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  } else {
+    switch (variant) {
+      case 0: stage_t<T, 1, 256, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+      default: stage_t<T, 1, 128, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
+    }
+  }
+}
+
+void launch_stage(int dtype, int rank, int variant, bool stage2, int blocks, hipStream_t s, const void* Us,
+                  const void* U0, void* Uout, const double* dtp, double* partials, const GridDev& G) {
+  if (dtype == 0)
+    stage_dispatch<double, false>(rank, variant, stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  else
+    stage_dispatch<float, true>(rank, variant, stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+}
+
+void launch_cfl(int dtype, int rank, int blocks, hipStream_t s, const void* U, const GridDev& G,
+                double* partials) {
+  if (dtype == 0) {
+    if (rank == 3) hipLaunchKernelGGL((cfl_kernel<double, 3, false>), dim3(blocks), dim3(256), 0, s, (const double*)U, G, partials);
+    else if (rank == 2) hipLaunchKernelGGL((cfl_kernel<double, 2, false>), dim3(blocks), dim3(256), 0, s, (const double*)U, G, partials);
+    else hipLaunchKernelGGL((cfl_kernel<double, 1, false>), dim3(blocks), dim3(256), 0, s, (const double*)U, G, partials);
+  } else {
+    if (rank == 3) hipLaunchKernelGGL((cfl_kernel<float, 3, true>), dim3(blocks), dim3(256), 0, s, (const float*)U, G, partials);
+    else if (rank == 2) hipLaunchKernelGGL((cfl_kernel<float, 2, true>), dim3(blocks), dim3(256), 0, s, (const float*)U, G, partials);
+    else hipLaunchKernelGGL((cfl_kernel<float, 1, true>), dim3(blocks), dim3(256), 0, s, (const float*)U, G, partials);
+  }
+}
+
+void launch_face_flux(int dtype, long long nf, hipStream_t s, const void* WL, const void* WR, const int32_t* axis,
+                      double gamma, void* F) {
+  const unsigned blocks = (unsigned)((nf + 255) / 256);
+  const double igm1 = 1.0 / (gamma - 1.0);
+  if (dtype == 0)
+    hipLaunchKernelGGL((face_flux_kernel<double>), dim3(blocks), dim3(256), 0, s, (const double*)WL,
+                       (const double*)WR, axis, nf, gamma, igm1, (double*)F);
+  else
+    hipLaunchKernelGGL((face_flux_kernel<float>), dim3(blocks), dim3(256), 0, s, (const float*)WL,
+                       (const float*)WR, axis, nf, float(gamma), float(igm1), (float*)F);
+}
+
+}  // namespace VH_PROFILE_NS
+}  // namespace vh

@@ -6,12 +6,20 @@
 // also fuses the SSP combine and the FP64 CFL reduction of the new state.
 //
 // Work decomposition (2.5-D march): a workgroup owns a TX×TY column of cells and
-// marches through a chunk of z planes. For the current plane it stages the
-// primitive state with a two-cell halo in x and y in LDS (a cross-shaped
-// footprint; corners are never read), computes every x- and y-face flux of the
-// tile once into LDS, while each lane carries its own column's z pipeline
-// (W(k), W(k+1), W(k+2), the z slope and the z-lo face flux) in registers, so
-// every face flux in the grid is evaluated exactly once per chunk plane.
+// marches through a chunk of z planes. Per plane:
+//   P0  stage the primitive state W of the plane with a two-cell halo in x and y
+//       in LDS (cross-shaped footprint; corners are never read);
+//   PA  each lane computes its cell's x and y slopes once, keeps the lo-side
+//       face states in registers and publishes the hi-side ones to LDS; a few
+//       lanes do the same for the one-cell halo rim;
+//   PB  each lane evaluates the HLLC flux of its cell's x-lo and y-lo faces and
+//       writes it over the face state it just consumed (every published face
+//       state is read by exactly one face); the tile's hi-edge faces are spread
+//       over the lanes; meanwhile each lane advances its own z pipeline
+//       (W(k+1), W(k+2), the z slope and the z-lo flux live in registers);
+//   PC  each lane gathers its six face fluxes in the reference order and
+//       writes the updated state.
+// Every face flux and every slope of the tile is evaluated once per plane.
 //
 // Accumulation order. The reference walks faces cell by cell, column-major,
 // axis by axis (lib/RefMesh/src/RefMesh.jl:187-197) and scatters ±F·area into
@@ -20,94 +28,24 @@
 //   2. per axis x, y, z: its own hi face, then the lo face if it is a domain
 //      boundary (emitted right after the hi face of the same axis);
 //   3. lo faces wrapped by a periodic boundary (partner visited later), x, y, z.
-// The gather below adds the six face fluxes in exactly that order, so the
-// floating-point sum matches the reference scatter bit for bit.
+// The gather adds the six fluxes in exactly that order, so in the exact profile
+// the floating-point sum matches the reference scatter bit for bit.
 #pragma once
 
 #include "hydro.h"
 
+#ifndef VH_MIN_WAVES
+#define VH_MIN_WAVES 3
+#endif
+// 1: issue the loads of the cell's own state at the start of the plane (more
+// latency hidden, more registers live across the plane); 0: load it in the
+// update phase.
+#ifndef VH_EARLY_OWN
+#define VH_EARLY_OWN 0
+#endif
+
 namespace vh {
-
-struct GridDev {
-  int n[3];        // cells per axis (axes beyond rank: 1)
-  int bc[6];       // [2*axis + side], side 0 = lo, 1 = hi
-  int rank;
-  int zchunk;      // planes per workgroup along z
-  int tiles_x;     // tiles along x
-  int tiles_y;     // tiles along y
-  int chunks_z;    // z chunks
-  int xcd_remap;   // 1: contiguous tile ranges per XCD
-  long long ncell;
-  double gamma;
-  double V;        // cell volume (geometry precision)
-  double w[3];     // cell widths
-  double area[3];  // face areas
-};
-
-// Map a (possibly out-of-range) coordinate on one axis to the cell that
-// supplies its state, following neighbor() (lib/RefMesh/src/RefMesh.jl:147-161).
-// Returns the source coordinate and sets `gh` to the boundary side whose ghost
-// applies (0: none, 1: lo, 2: hi).
-__device__ __forceinline__ int resolve(int i, int n, int bclo, int &gh) {
-  gh = 0;
-  if (i >= 0 && i < n) return i;
-  if (bclo == BC_PERIODIC) {
-    int r = i % n;
-    return r < 0 ? r + n : r;
-  }
-  if (i < 0) { gh = 1; return 0; }
-  gh = 2;
-  return n - 1;
-}
-
-template <typename T>
-__device__ __forceinline__ Cons<T> load_cons(const T* __restrict__ U, long long n, long long c) {
-  Cons<T> u;
-  u.d = U[c];
-  u.mx = U[n + c];
-  u.my = U[2 * n + c];
-  u.mz = U[3 * n + c];
-  u.e = U[4 * n + c];
-  return u;
-}
-
-template <typename T>
-__device__ __forceinline__ void store_cons(T* __restrict__ U, long long n, long long c, const Cons<T>& u) {
-  U[c] = u.d;
-  U[n + c] = u.mx;
-  U[2 * n + c] = u.my;
-  U[3 * n + c] = u.mz;
-  U[4 * n + c] = u.e;
-}
-
-// Lane-level block maximum of a double (wave64 butterfly + LDS across waves).
-template <int NT>
-__device__ __forceinline__ double block_max(double v, double* red) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = vmax(v, __shfl_xor(v, o, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double r = red[0];
-#pragma unroll
-  for (int i = 1; i < NT / 64; ++i) r = vmax(r, red[i]);
-  return r;
-}
-
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* red) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double r = red[0];
-#pragma unroll
-  for (int i = 1; i < NT / 64; ++i) r += red[i];
-  return r;
-}
+namespace VH_PROFILE_NS {
 
 // Per-cell CFL term of compute_dt (src/driver.jl:178-185), Tr = double.
 template <typename T, int RANK>
@@ -119,85 +57,102 @@ __device__ __forceinline__ double cfl_term(const Prim<T>& W, T g, const GridDev&
   return m;
 }
 
-// LDS footprint of one plane (primitive state with halo 2 in x and y) plus the
-// x- and y-face flux tiles.
 template <typename T, int TX, int TY>
 struct PlaneLDS {
-  T w[5][TY + 4][TX + 4];
-  T fx[5][TY][TX + 1];
-  T fy[5][TY + 1][TX];
+  T w[5][TY + 4][TX + 4];   // primitive state, halo 2 in x and y
+  T xh[5][TY][TX + 1];      // x: hi face state of cell s-1 → then flux of x face s
+  T yh[5][TY + 1][TX];      // y: hi face state of cell s-1 → then flux of y face s
+  T xl[5][TY];              // x: lo face state of halo cell TX
+  T yl[5][TX];              // y: lo face state of halo cell TY
   double red[16];
 };
 
+#define VH_PUT5(arr, idx, v) \
+  do {                       \
+    arr[0] idx = (v).r;      \
+    arr[1] idx = (v).vx;     \
+    arr[2] idx = (v).vy;     \
+    arr[3] idx = (v).vz;     \
+    arr[4] idx = (v).p;      \
+  } while (0)
+#define VH_GET5(v, arr, idx) \
+  do {                       \
+    (v).r = arr[0] idx;      \
+    (v).vx = arr[1] idx;     \
+    (v).vy = arr[2] idx;     \
+    (v).vz = arr[3] idx;     \
+    (v).p = arr[4] idx;      \
+  } while (0)
+#define VH_PUTF(arr, idx, f) \
+  do {                       \
+    arr[0] idx = (f).d;      \
+    arr[1] idx = (f).mx;     \
+    arr[2] idx = (f).my;     \
+    arr[3] idx = (f).mz;     \
+    arr[4] idx = (f).e;      \
+  } while (0)
+#define VH_GETF(f, arr, idx) \
+  do {                       \
+    (f).d = arr[0] idx;      \
+    (f).mx = arr[1] idx;     \
+    (f).my = arr[2] idx;     \
+    (f).mz = arr[3] idx;     \
+    (f).e = arr[4] idx;      \
+  } while (0)
+
 template <typename T, int TX, int TY>
-__device__ __forceinline__ void lds_put_w(PlaneLDS<T, TX, TY>& S, int sy, int sx, const Prim<T>& w) {
-  S.w[0][sy][sx] = w.r;
-  S.w[1][sy][sx] = w.vx;
-  S.w[2][sy][sx] = w.vy;
-  S.w[3][sy][sx] = w.vz;
-  S.w[4][sy][sx] = w.p;
-}
-template <typename T, int TX, int TY>
-__device__ __forceinline__ Prim<T> lds_get_w(const PlaneLDS<T, TX, TY>& S, int sy, int sx) {
+__device__ __forceinline__ Prim<T> lds_w(const PlaneLDS<T, TX, TY>& S, int sy, int sx) {
   Prim<T> w;
-  w.r = S.w[0][sy][sx];
-  w.vx = S.w[1][sy][sx];
-  w.vy = S.w[2][sy][sx];
-  w.vz = S.w[3][sy][sx];
-  w.p = S.w[4][sy][sx];
+  VH_GET5(w, S.w, [sy][sx]);
   return w;
 }
 
 // State of a column cell at plane index kk (any integer) with the z boundary
 // rule applied: periodic wrap, or the ghost of the nearest boundary cell.
 template <typename T, bool FLOOR64>
-__device__ __forceinline__ Prim<T> column_w(const T* __restrict__ U, const GridDev& G, long long col,
-                                            int kk, T g) {
+__device__ __forceinline__ Prim<T> column_w(const Cons<T>& u, const GridDev& G, int kk, T g) {
   int gh;
-  const int src = resolve(kk, G.n[2], G.bc[4], gh);
-  const long long plane = (long long)G.n[0] * G.n[1];
-  Prim<T> w = cons2prim<T, FLOOR64>(load_cons(U, G.ncell, col + (long long)src * plane), g, G.gamma);
+  (void)resolve(kk, G.n[2], G.bc[4], gh);
+  Prim<T> w = cons2prim<T, FLOOR64>(u, g, G.gamma);
   if (gh) w = ghost(w, G.bc[4 + gh - 1], 3);
   return w;
 }
+__device__ __forceinline__ long long column_src(const GridDev& G, long long col, int kk) {
+  int gh;
+  const int src = resolve(kk, G.n[2], G.bc[4], gh);
+  return col + (long long)src * ((long long)G.n[0] * G.n[1]);
+}
 
-// z-face flux between column cells k and k+1 given W and z slopes of both.
-// Handles the domain-boundary faces at k = -1 (lo) and k = nz-1 (hi).
+// Flux of the face between cells gpos-1 and gpos along an axis with n cells,
+// given the hi state of gpos-1 (L) and the lo state of gpos (R); applies the
+// domain-boundary rule of _flux_face! (src/driver.jl:255-281).
 template <typename T>
-__device__ __forceinline__ Cons<T> zface(int k, const GridDev& G, const Prim<T>& wa, const Prim<T>& sa,
-                                         const Prim<T>& wb, const Prim<T>& sb, T g) {
-  const int nz = G.n[2];
-  const bool per = G.bc[4] == BC_PERIODIC;
-  Prim<T> L, R;
-  if (!per && k < 0) {
-    R = face_value(wb, sb, T(-0.5));
-    L = ghost(R, G.bc[4], 3);
-  } else if (!per && k >= nz - 1) {
-    L = face_value(wa, sa, T(0.5));
-    R = ghost(L, G.bc[5], 3);
-  } else {
-    L = face_value(wa, sa, T(0.5));
-    R = face_value(wb, sb, T(-0.5));
+__device__ __forceinline__ Cons<T> axis_face(int gpos, int n, int bclo, int bchi, Prim<T> L, Prim<T> R, T g,
+                                             T igm1, int dim) {
+  if (bclo != BC_PERIODIC) {
+    if (gpos == 0) L = ghost(R, bclo, dim);
+    else if (gpos == n) R = ghost(L, bchi, dim);
   }
-  return hllc(L, R, g, 3);
+  return hllc(L, R, g, igm1, dim);
 }
 
 // One RK stage. STAGE2 = false: Uout = Us − (dt/V)·acc(Us).
 // STAGE2 = true: Uout = ½·U0 + ½·(Us − (dt/V)·acc(Us)) written over U0 (point
 // reads only), plus the per-workgroup max of the CFL term of the new state.
 template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64>
-__global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
+__global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
                                                           T* Uout, const double* __restrict__ dtp,
                                                           double* __restrict__ partials, GridDev G) {
   constexpr int NT = TX * TY;
   __shared__ PlaneLDS<T, TX, TY> S;
   const T g = T(G.gamma);
+  const T igm1 = T(G.igm1);
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
 
-  // Workgroup → (tile, z chunk). Consecutive hardware dispatch slots go round
-  // robin over the 8 XCDs; remapping gives each XCD a contiguous tile range so
-  // tiles that share halos share an L2.
+  // Workgroup → (tile, z chunk). Consecutive dispatch slots go round robin over
+  // the 8 XCDs; the remap gives each XCD a contiguous tile range so tiles that
+  // share halos share an L2.
   const int nblk = G.tiles_x * G.tiles_y * G.chunks_z;
   int b = blockIdx.x;
   if (G.xcd_remap && (nblk % 8) == 0) b = (b % 8) * (nblk / 8) + b / 8;
@@ -216,7 +171,7 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
 
   const int gx = x0 + tx, gy = y0 + ty;
   const bool active = gx < nx && gy < ny;
-  // The lane's own column (clamped so inactive lanes read valid memory).
+  // The lane's own column (clamped or wrapped so every lane reads valid memory).
   int ghx, ghy;
   const int sxc = resolve(gx, nx, G.bc[0], ghx);
   const int syc = resolve(gy, ny, G.bc[2], ghy);
@@ -226,22 +181,46 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
   Prim<T> wz0, wz1, wz2, sz0;
   Cons<T> fzlo;
   if constexpr (RANK == 3) {
-    const Prim<T> wm2 = column_w<T, FLOOR64>(Us, G, col, kz0 - 2, g);
-    const Prim<T> wm1 = column_w<T, FLOOR64>(Us, G, col, kz0 - 1, g);
-    wz0 = column_w<T, FLOOR64>(Us, G, col, kz0, g);
-    wz1 = column_w<T, FLOOR64>(Us, G, col, kz0 + 1, g);
-    wz2 = column_w<T, FLOOR64>(Us, G, col, kz0 + 2, g);
-    // slope of cell kz0-1 needs its own ghost rule at the lo boundary: if kz0-1
-    // is the ghost plane its slope is never used (the lo face is a boundary face).
+    const Cons<T> um2 = load_cons(Us, n, column_src(G, col, kz0 - 2));
+    const Cons<T> um1 = load_cons(Us, n, column_src(G, col, kz0 - 1));
+    const Cons<T> u0 = load_cons(Us, n, column_src(G, col, kz0));
+    const Cons<T> u1 = load_cons(Us, n, column_src(G, col, kz0 + 1));
+    const Cons<T> u2 = load_cons(Us, n, column_src(G, col, kz0 + 2));
+    const Prim<T> wm2 = column_w<T, FLOOR64>(um2, G, kz0 - 2, g);
+    const Prim<T> wm1 = column_w<T, FLOOR64>(um1, G, kz0 - 1, g);
+    wz0 = column_w<T, FLOOR64>(u0, G, kz0, g);
+    wz1 = column_w<T, FLOOR64>(u1, G, kz0 + 1, g);
+    wz2 = column_w<T, FLOOR64>(u2, G, kz0 + 2, g);
+    // The slope of the ghost plane kz0-1 is never used when kz0-1 is outside a
+    // non-periodic domain: that face is a boundary face (lo state = ghost).
     const Prim<T> sm1 = slope(wm2, wm1, wz0);
     sz0 = slope(wm1, wz0, wz1);
-    fzlo = zface(kz0 - 1, G, wm1, sm1, wz0, sz0, g);
+    const int bz = G.bc[4];
+    if (bz != BC_PERIODIC && kz0 == 0) {
+      const Prim<T> R = face_value(wz0, sz0, T(-0.5));
+      fzlo = hllc(ghost(R, bz, 3), R, g, igm1, 3);
+    } else {
+      fzlo = hllc(face_value(wm1, sm1, T(0.5)), face_value(wz0, sz0, T(-0.5)), g, igm1, 3);
+    }
   }
 
   double cfl_max = 0.0;
 
   for (int k = kz0; k < kz1; ++k) {
-    // ---- stage the plane's primitive state in LDS ----
+    const long long c = (long long)k * plane + (long long)gy * nx + gx;
+    // Early loads: the next z plane of the column and this cell's own state.
+    Cons<T> unext, uown, u0own;
+    if constexpr (RANK == 3) {
+      if (k + 1 < kz1) unext = load_cons(Us, n, column_src(G, col, k + 3));
+    }
+#if VH_EARLY_OWN
+    if (active) {
+      uown = load_cons(Us, n, c);
+      if constexpr (STAGE2) u0own = load_cons(U0, n, c);
+    }
+#endif
+
+    // ---- P0: stage the plane's primitive state in LDS ----
     Prim<T> wc;
     if constexpr (RANK == 3) {
       wc = wz0;
@@ -249,11 +228,11 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
       wc = cons2prim<T, FLOOR64>(load_cons(Us, n, col), g, G.gamma);
     }
     {
-      // Own slot: apply the x/y ghost when the lane sits past a non-periodic edge.
       Prim<T> w = wc;
       if (ghx) w = ghost(w, G.bc[ghx - 1], 1);
       if (ghy) w = ghost(w, G.bc[2 + ghy - 1], 2);
-      lds_put_w(S, ty + 2, tx + 2, w);
+      VH_PUT5(S.w, [ty + 2][tx + 2], w);
+      wc = w;
     }
     constexpr int NHX = 4 * TY;
     constexpr int NHY = (RANK > 1) ? 4 * TX : 0;
@@ -276,88 +255,142 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
                                         G.gamma);
       if (hgx) w = ghost(w, G.bc[hgx - 1], 1);
       if (hgy) w = ghost(w, G.bc[2 + hgy - 1], 2);
-      lds_put_w(S, ly + 2, lx + 2, w);
+      VH_PUT5(S.w, [ly + 2][lx + 2], w);
     }
     __syncthreads();
 
-    // ---- x- and y-face fluxes of the tile (each face once) ----
-    constexpr int NFX = (TX + 1) * TY;
-    constexpr int NFY = (RANK > 1) ? TX * (TY + 1) : 0;
-    for (int f = tid; f < NFX + NFY; f += NT) {
-      int ax, fi, fj;  // ax 0: x-face at local x = fi (between fi-1 and fi), row fj
-      if (f < NFX) {
-        ax = 0; fi = f % (TX + 1); fj = f / (TX + 1);
-      } else {
-        ax = 1; fi = (f - NFX) % TX; fj = (f - NFX) / TX;
+    // ---- PA: slopes once per cell; publish hi face states ----
+    Prim<T> xlo, ylo;
+    {
+      const Prim<T> sx = slope(lds_w(S, ty + 2, tx + 1), wc, lds_w(S, ty + 2, tx + 3));
+      xlo = face_value(wc, sx, T(-0.5));
+      const Prim<T> xhi = face_value(wc, sx, T(0.5));
+      VH_PUT5(S.xh, [ty][tx + 1], xhi);
+      if constexpr (RANK > 1) {
+        const Prim<T> sy = slope(lds_w(S, ty + 1, tx + 2), wc, lds_w(S, ty + 3, tx + 2));
+        ylo = face_value(wc, sy, T(-0.5));
+        const Prim<T> yhi = face_value(wc, sy, T(0.5));
+        VH_PUT5(S.yh, [ty + 1][tx], yhi);
       }
-      // Global index of the cell on the +side of the face along ax, and row.
-      const int gpos = ax == 0 ? x0 + fi : y0 + fj;
-      const int grow = ax == 0 ? y0 + fj : x0 + fi;
-      const int nax = ax == 0 ? nx : ny;
-      const int nrow = ax == 0 ? ny : nx;
-      if (grow >= nrow || gpos > nax) continue;
-      // LDS slots of the four cells gpos-2 .. gpos+1 along ax.
-      const int sy0 = ax == 0 ? fj + 2 : fj;        // slot of gpos-2 (y)
-      const int sx0 = ax == 0 ? fi : fi + 2;        // slot of gpos-2 (x)
-      const int dy = ax == 0 ? 0 : 1, dx = ax == 0 ? 1 : 0;
-      const Prim<T> wa2 = lds_get_w(S, sy0, sx0);
-      const Prim<T> wa = lds_get_w(S, sy0 + dy, sx0 + dx);
-      const Prim<T> wb = lds_get_w(S, sy0 + 2 * dy, sx0 + 2 * dx);
-      const Prim<T> wb2 = lds_get_w(S, sy0 + 3 * dy, sx0 + 3 * dx);
-      const bool per = G.bc[2 * ax] == BC_PERIODIC;
-      const int dim = ax + 1;
-      Prim<T> L, R;
-      if (!per && gpos == 0) {
-        R = face_value(wb, slope(wa, wb, wb2), T(-0.5));
-        L = ghost(R, G.bc[2 * ax], dim);
-      } else if (!per && gpos == nax) {
-        L = face_value(wa, slope(wa2, wa, wb), T(0.5));
-        R = ghost(L, G.bc[2 * ax + 1], dim);
+    }
+    // Halo rim: x hi state of cell -1 and lo state of cell TX per row; y hi
+    // state of cell -1 and lo state of cell TY per column.
+    constexpr int NRX = 2 * TY;
+    constexpr int NRY = (RANK > 1) ? 2 * TX : 0;
+    for (int h = tid; h < NRX + NRY; h += NT) {
+      if (h < NRX) {
+        const int r = h >> 1;
+        if ((h & 1) == 0) {   // cell -1: slots 0, 1, 2
+          const Prim<T> wm = lds_w(S, r + 2, 1);
+          const Prim<T> s = slope(lds_w(S, r + 2, 0), wm, lds_w(S, r + 2, 2));
+          const Prim<T> v = face_value(wm, s, T(0.5));
+          VH_PUT5(S.xh, [r][0], v);
+        } else {              // cell TX: slots TX+1, TX+2, TX+3
+          const Prim<T> wp = lds_w(S, r + 2, TX + 2);
+          const Prim<T> s = slope(lds_w(S, r + 2, TX + 1), wp, lds_w(S, r + 2, TX + 3));
+          const Prim<T> v = face_value(wp, s, T(-0.5));
+          VH_PUT5(S.xl, [r], v);
+        }
       } else {
-        L = face_value(wa, slope(wa2, wa, wb), T(0.5));
-        R = face_value(wb, slope(wa, wb, wb2), T(-0.5));
+        const int hh = h - NRX;
+        const int cc = hh >> 1;
+        if ((hh & 1) == 0) {
+          const Prim<T> wm = lds_w(S, 1, cc + 2);
+          const Prim<T> s = slope(lds_w(S, 0, cc + 2), wm, lds_w(S, 2, cc + 2));
+          const Prim<T> v = face_value(wm, s, T(0.5));
+          VH_PUT5(S.yh, [0][cc], v);
+        } else {
+          const Prim<T> wp = lds_w(S, TY + 2, cc + 2);
+          const Prim<T> s = slope(lds_w(S, TY + 1, cc + 2), wp, lds_w(S, TY + 3, cc + 2));
+          const Prim<T> v = face_value(wp, s, T(-0.5));
+          VH_PUT5(S.yl, [cc], v);
+        }
       }
-      const Cons<T> F = hllc(L, R, g, dim);
-      if (ax == 0) {
-        S.fx[0][fj][fi] = F.d; S.fx[1][fj][fi] = F.mx; S.fx[2][fj][fi] = F.my;
-        S.fx[3][fj][fi] = F.mz; S.fx[4][fj][fi] = F.e;
+    }
+    __syncthreads();
+
+    // ---- PB: face fluxes (each face once), written over the consumed state ----
+    const int bx = G.bc[0], by = G.bc[2];
+    if (gy < ny && gx <= nx) {
+      Prim<T> L;
+      VH_GET5(L, S.xh, [ty][tx]);
+      const Cons<T> F = axis_face(gx, nx, bx, G.bc[1], L, xlo, g, igm1, 1);
+      VH_PUTF(S.xh, [ty][tx], F);
+    }
+    if constexpr (RANK > 1) {
+      if (gx < nx && gy <= ny) {
+        Prim<T> L;
+        VH_GET5(L, S.yh, [ty][tx]);
+        const Cons<T> F = axis_face(gy, ny, by, G.bc[3], L, ylo, g, igm1, 2);
+        VH_PUTF(S.yh, [ty][tx], F);
+      }
+    }
+    // Tile hi-edge faces: x face TX of each row, y face TY of each column.
+    constexpr int NEX = TY;
+    constexpr int NEY = (RANK > 1) ? TX : 0;
+    for (int h = tid; h < NEX + NEY; h += NT) {
+      if (h < NEX) {
+        const int r = h;
+        const int gp = x0 + TX;
+        if (y0 + r < ny && gp <= nx) {
+          Prim<T> L, R;
+          VH_GET5(L, S.xh, [r][TX]);
+          VH_GET5(R, S.xl, [r]);
+          const Cons<T> F = axis_face(gp, nx, bx, G.bc[1], L, R, g, igm1, 1);
+          VH_PUTF(S.xh, [r][TX], F);
+        }
       } else {
-        S.fy[0][fj][fi] = F.d; S.fy[1][fj][fi] = F.mx; S.fy[2][fj][fi] = F.my;
-        S.fy[3][fj][fi] = F.mz; S.fy[4][fj][fi] = F.e;
+        const int cc = h - NEX;
+        const int gp = y0 + TY;
+        if (x0 + cc < nx && gp <= ny) {
+          Prim<T> L, R;
+          VH_GET5(L, S.yh, [TY][cc]);
+          VH_GET5(R, S.yl, [cc]);
+          const Cons<T> F = axis_face(gp, ny, by, G.bc[3], L, R, g, igm1, 2);
+          VH_PUTF(S.yh, [TY][cc], F);
+        }
       }
     }
 
-    // ---- z-face at k+½ for the lane's column ----
+    // z face at k+½ for the lane's column.
     Cons<T> fzhi;
     Prim<T> sz1;
     if constexpr (RANK == 3) {
       sz1 = slope(wz0, wz1, wz2);
-      fzhi = zface(k, G, wz0, sz0, wz1, sz1, g);
+      const int bz = G.bc[4];
+      const Prim<T> L = face_value(wz0, sz0, T(0.5));
+      if (bz != BC_PERIODIC && k == nz - 1)
+        fzhi = hllc(L, ghost(L, G.bc[5], 3), g, igm1, 3);
+      else
+        fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
     }
     __syncthreads();
 
-    // ---- gather the six face fluxes in the reference order and update ----
+    // ---- PC: gather the six face fluxes in the reference order and update ----
     if (active) {
-      const long long c = (long long)k * plane + (long long)gy * nx + gx;
+#if !VH_EARLY_OWN
+      uown = load_cons(Us, n, c);
+      if constexpr (STAGE2) u0own = load_cons(U0, n, c);
+#endif
       Cons<T> fxlo, fxhi, fylo, fyhi;
-      fxlo.d = S.fx[0][ty][tx]; fxlo.mx = S.fx[1][ty][tx]; fxlo.my = S.fx[2][ty][tx];
-      fxlo.mz = S.fx[3][ty][tx]; fxlo.e = S.fx[4][ty][tx];
-      fxhi.d = S.fx[0][ty][tx + 1]; fxhi.mx = S.fx[1][ty][tx + 1]; fxhi.my = S.fx[2][ty][tx + 1];
-      fxhi.mz = S.fx[3][ty][tx + 1]; fxhi.e = S.fx[4][ty][tx + 1];
+      VH_GETF(fxlo, S.xh, [ty][tx]);
+      VH_GETF(fxhi, S.xh, [ty][tx + 1]);
       if constexpr (RANK > 1) {
-        fylo.d = S.fy[0][ty][tx]; fylo.mx = S.fy[1][ty][tx]; fylo.my = S.fy[2][ty][tx];
-        fylo.mz = S.fy[3][ty][tx]; fylo.e = S.fy[4][ty][tx];
-        fyhi.d = S.fy[0][ty + 1][tx]; fyhi.mx = S.fy[1][ty + 1][tx]; fyhi.my = S.fy[2][ty + 1][tx];
-        fyhi.mz = S.fy[3][ty + 1][tx]; fyhi.e = S.fy[4][ty + 1][tx];
+        VH_GETF(fylo, S.yh, [ty][tx]);
+        VH_GETF(fyhi, S.yh, [ty + 1][tx]);
       }
       // Lo-face kinds: 0 interior (partner earlier), 1 domain boundary, 2 periodic wrap.
-      const int kx = gx > 0 ? 0 : (G.bc[0] == BC_PERIODIC ? 2 : 1);
-      const int ky = gy > 0 ? 0 : (G.bc[2] == BC_PERIODIC ? 2 : 1);
+      const int kx = gx > 0 ? 0 : (bx == BC_PERIODIC ? 2 : 1);
+      const int ky = gy > 0 ? 0 : (by == BC_PERIODIC ? 2 : 1);
       const int kz = k > 0 ? 0 : (G.bc[4] == BC_PERIODIC ? 2 : 1);
       Cons<T> acc;
       acc.d = acc.mx = acc.my = acc.mz = acc.e = T(0);
-      if (RANK > 2 && kz == 0) acc_add(acc, fzlo, -a2);
-      if (RANK > 1 && ky == 0) acc_add(acc, fylo, -a1);
+      if constexpr (RANK > 2) {
+        if (kz == 0) acc_add(acc, fzlo, -a2);
+      }
+      if constexpr (RANK > 1) {
+        if (ky == 0) acc_add(acc, fylo, -a1);
+      }
       if (kx == 0) acc_add(acc, fxlo, -a0);
       acc_add(acc, fxhi, a0);
       if (kx == 1) acc_add(acc, fxlo, -a0);
@@ -370,27 +403,29 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
         if (kz == 1) acc_add(acc, fzlo, -a2);
       }
       if (kx == 2) acc_add(acc, fxlo, -a0);
-      if (RANK > 1 && ky == 2) acc_add(acc, fylo, -a1);
-      if (RANK > 2 && kz == 2) acc_add(acc, fzlo, -a2);
+      if constexpr (RANK > 1) {
+        if (ky == 2) acc_add(acc, fylo, -a1);
+      }
+      if constexpr (RANK > 2) {
+        if (kz == 2) acc_add(acc, fzlo, -a2);
+      }
 
-      const Cons<T> us = load_cons(Us, n, c);
       Cons<T> o;
       if constexpr (!STAGE2) {
         // _euler_apply!: dst = base − invV·acc (src/driver.jl:297-307).
-        o.d = us.d - invV * acc.d;
-        o.mx = us.mx - invV * acc.mx;
-        o.my = us.my - invV * acc.my;
-        o.mz = us.mz - invV * acc.mz;
-        o.e = us.e - invV * acc.e;
+        o.d = uown.d - invV * acc.d;
+        o.mx = uown.mx - invV * acc.mx;
+        o.my = uown.my - invV * acc.my;
+        o.mz = uown.mz - invV * acc.mz;
+        o.e = uown.e - invV * acc.e;
       } else {
         // _rk2_combine!: ½·u0 + ½·(u1 − invV·acc) (src/driver.jl:310-321).
-        const Cons<T> u0 = load_cons(U0, n, c);
         const T h = T(0.5);
-        o.d = h * u0.d + h * (us.d - invV * acc.d);
-        o.mx = h * u0.mx + h * (us.mx - invV * acc.mx);
-        o.my = h * u0.my + h * (us.my - invV * acc.my);
-        o.mz = h * u0.mz + h * (us.mz - invV * acc.mz);
-        o.e = h * u0.e + h * (us.e - invV * acc.e);
+        o.d = h * u0own.d + h * (uown.d - invV * acc.d);
+        o.mx = h * u0own.mx + h * (uown.mx - invV * acc.mx);
+        o.my = h * u0own.my + h * (uown.my - invV * acc.my);
+        o.mz = h * u0own.mz + h * (uown.mz - invV * acc.mz);
+        o.e = h * u0own.e + h * (uown.e - invV * acc.e);
         cfl_max = vmax(cfl_max, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(o, g, G.gamma), g, G));
       }
       store_cons(Uout, n, c, o);
@@ -401,9 +436,8 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
       sz0 = sz1;
       wz0 = wz1;
       wz1 = wz2;
-      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64>(Us, G, col, k + 3, g);
+      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64>(unext, G, k + 3, g);
     }
-    __syncthreads();
   }
 
   if constexpr (STAGE2) {
@@ -412,4 +446,18 @@ __global__ void __launch_bounds__(TX* TY) rk_stage_kernel(const T* __restrict__ 
   }
 }
 
+// Per-workgroup max of the CFL term over a grid-stride range of cells.
+template <typename T, int RANK, bool FLOOR64>
+__global__ void __launch_bounds__(256) cfl_kernel(const T* __restrict__ U, GridDev G,
+                                                  double* __restrict__ partials) {
+  __shared__ double red[16];
+  const T g = T(G.gamma);
+  double m = 0.0;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < G.ncell; c += (long long)gridDim.x * 256)
+    m = vmax(m, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g, G));
+  m = block_max<256>(m, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+}  // namespace VH_PROFILE_NS
 }  // namespace vh

@@ -1,10 +1,11 @@
 // vespahip — C-ABI entry points of the gfx950 flux-divergence library.
 //
-// Kernels:
+// Kernels (per arithmetic profile, built from stages.hip):
 //   face_flux_kernel   one lane per face, HLLC (tier-1 drop-in for the
 //                      KernelAbstractions kernel, ext/VespaKAFluxExt.jl:23-35)
 //   rk_stage_kernel    fused RK stage over a uniform grid (stencil.h)
 //   cfl_kernel         per-workgroup max of the CFL term (src/driver.jl:178-185)
+// Kernels here (profile-independent):
 //   dt_finalize_kernel reduce the partial maxima into dt on the device
 //   totals_kernel      per-workgroup Σ U·V partial sums (src/diagnostics.jl:16-31)
 #include <hip/hip_runtime.h>
@@ -15,7 +16,7 @@
 #include <string>
 
 #include "../../include/vespahip.h"
-#include "stencil.h"
+#include "profiles.h"
 
 namespace vh {
 
@@ -38,40 +39,7 @@ static int fail(int code, const char* fmt, const char* a = "", long long b = 0) 
   } while (0)
 
 // ----------------------------------------------------------------------------
-// tier-1 face flux
-
-template <typename T>
-__global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL, const T* __restrict__ WR,
-                                                        const int32_t* __restrict__ axis, long long nf,
-                                                        T g, T* __restrict__ F) {
-  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  Prim<T> L, R;
-  const T* l = WL + 5 * f;
-  const T* r = WR + 5 * f;
-  L.r = l[0]; L.vx = l[1]; L.vy = l[2]; L.vz = l[3]; L.p = l[4];
-  R.r = r[0]; R.vx = r[1]; R.vy = r[2]; R.vz = r[3]; R.p = r[4];
-  int d = axis[f];
-  d = d < 1 ? 1 : (d > 3 ? 3 : d);
-  const Cons<T> o = hllc(L, R, g, d);
-  T* out = F + 5 * f;
-  out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
-}
-
-// ----------------------------------------------------------------------------
 // reductions
-
-template <typename T, int RANK, bool FLOOR64>
-__global__ void __launch_bounds__(256) cfl_kernel(const T* __restrict__ U, GridDev G,
-                                                  double* __restrict__ partials) {
-  __shared__ double red[16];
-  const T g = T(G.gamma);
-  double m = 0.0;
-  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < G.ncell; c += (long long)gridDim.x * 256)
-    m = vmax(m, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g, G));
-  m = block_max<256>(m, red);
-  if (threadIdx.x == 0) partials[blockIdx.x] = m;
-}
 
 // clock: [0] t, [1] dt, [2] invdt, [3] steps (as double)
 __global__ void __launch_bounds__(256) dt_finalize_kernel(const double* __restrict__ partials, int np,
@@ -112,13 +80,6 @@ __global__ void __launch_bounds__(256) totals_kernel(const T* __restrict__ U, lo
   }
 }
 
-// Zero the accumulator output used by the diagnostic accumulate path.
-template <typename T>
-__global__ void fill_kernel(T* p, long long n, T v) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 // acc = (U − U1)/invV reconstructed exactly is not possible; the diagnostic
 // accumulate path instead runs stage 1 with dt = −V (invV = −1 in Tf), so
 // U1 = U + acc·1, then subtracts U in a second kernel. With invV = −1 the
@@ -131,62 +92,12 @@ __global__ void sub_kernel(const T* __restrict__ a, const T* __restrict__ b, T* 
   if (i < n) out[i] = a[i] - b[i];
 }
 
-// ----------------------------------------------------------------------------
-// launch table
-
-struct TileShape {
-  int tx, ty;
-};
-
-static const TileShape kTiles3[] = {{32, 8}, {64, 4}, {16, 16}, {64, 8}};
-static const TileShape kTiles2[] = {{32, 8}, {64, 4}, {16, 16}};
-static const TileShape kTiles1[] = {{256, 1}, {128, 1}};
-
-template <typename T, int RANK, int TX, int TY, bool FLOOR64>
-static void launch_stage_t(bool stage2, int blocks, hipStream_t s, const T* Us, const T* U0, T* Uout,
-                           const double* dtp, double* partials, const GridDev& G) {
-  if (stage2)
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
-                       Us, U0, Uout, dtp, partials, G);
-  else
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64>), dim3(blocks), dim3(TX * TY), 0, s,
-                       Us, U0, Uout, dtp, partials, G);
-}
-
-template <typename T, bool FLOOR64>
-static void launch_stage(int rank, int variant, bool stage2, int blocks, hipStream_t s, const T* Us,
-                         const T* U0, T* Uout, const double* dtp, double* partials, const GridDev& G) {
-  if (rank == 3) {
-    switch (variant) {
-      case 0: launch_stage_t<T, 3, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      case 1: launch_stage_t<T, 3, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      case 2: launch_stage_t<T, 3, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      default: launch_stage_t<T, 3, 64, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-    }
-  } else if (rank == 2) {
-    switch (variant) {
-      case 0: launch_stage_t<T, 2, 32, 8, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      case 1: launch_stage_t<T, 2, 64, 4, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      default: launch_stage_t<T, 2, 16, 16, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-    }
-  } else {
-    switch (variant) {
-      case 0: launch_stage_t<T, 1, 256, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-      default: launch_stage_t<T, 1, 128, 1, FLOOR64>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G); break;
-    }
-  }
-}
-
-template <typename T, bool FLOOR64>
-static void launch_cfl(int rank, int blocks, hipStream_t s, const T* U, const GridDev& G, double* partials) {
-  if (rank == 3) hipLaunchKernelGGL((cfl_kernel<T, 3, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
-  else if (rank == 2) hipLaunchKernelGGL((cfl_kernel<T, 2, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
-  else hipLaunchKernelGGL((cfl_kernel<T, 1, FLOOR64>), dim3(blocks), dim3(256), 0, s, U, G, partials);
-}
-
 }  // namespace vh
 
 using namespace vh;
+
+// The C-ABI profile codes must survive every header of this translation unit.
+static_assert(VH_EXACT == 0 && VH_FAST == 1, "profile codes of vespahip.h were redefined");
 
 // ----------------------------------------------------------------------------
 // context
@@ -194,6 +105,7 @@ using namespace vh;
 struct vh_grid {
   int dtype;
   int device;
+  int mode;        // VH_EXACT or VH_FAST
   GridDev G;
   int variant;
   int tx, ty;
@@ -210,16 +122,6 @@ struct vh_grid {
 };
 
 static const int kMaxCflBlocks = 2048;
-
-static int tile_count_for_rank(int rank) {
-  return rank == 3 ? 4 : (rank == 2 ? 3 : 2);
-}
-
-static TileShape tile_for(int rank, int variant) {
-  if (rank == 3) return kTiles3[variant];
-  if (rank == 2) return kTiles2[variant];
-  return kTiles1[variant];
-}
 
 static int configure(vh_grid* g, int variant, int zchunk) {
   GridDev& G = g->G;
@@ -240,7 +142,7 @@ static int configure(vh_grid* g, int variant, int zchunk) {
   G.zchunk = zchunk;
   G.chunks_z = (G.n[2] + zchunk - 1) / zchunk;
   G.xcd_remap = 1;
-  long long blocks = (long long)G.tiles_x * G.tiles_y * G.chunks_z;
+  const long long blocks = (long long)G.tiles_x * G.tiles_y * G.chunks_z;
   if (blocks > (1LL << 30)) return fail(-1, "grid too large for one launch%s (%lld workgroups)", "", blocks);
   g->stage_blocks = (int)blocks;
   const int need = g->stage_blocks > kMaxCflBlocks ? g->stage_blocks : kMaxCflBlocks;
@@ -253,19 +155,71 @@ static int configure(vh_grid* g, int variant, int zchunk) {
   return 0;
 }
 
-template <typename T>
-static int face_flux_dev(const T* dWL, const T* dWR, const int32_t* dax, int64_t nf, double gamma, T* dF,
-                         void* stream) {
+static void enqueue_stage(vh_grid* g, bool stage2, const void* Us, const void* U0, void* Uout) {
+  const double* dtp = g->clock + 1;
+  if (g->mode == VH_FAST)
+    fast::launch_stage(g->dtype, g->G.rank, g->variant, stage2, g->stage_blocks, g->stream, Us, U0, Uout, dtp,
+                       g->partials, g->G);
+  else
+    exact::launch_stage(g->dtype, g->G.rank, g->variant, stage2, g->stage_blocks, g->stream, Us, U0, Uout, dtp,
+                        g->partials, g->G);
+}
+
+static void enqueue_cfl(vh_grid* g) {
+  if (g->mode == VH_FAST)
+    fast::launch_cfl(g->dtype, g->G.rank, g->cfl_blocks, g->stream, g->U, g->G, g->partials);
+  else
+    exact::launch_cfl(g->dtype, g->G.rank, g->cfl_blocks, g->stream, g->U, g->G, g->partials);
+}
+
+// One RK2 step reading dt from clock[1]; stage 2 leaves the partial CFL maxima
+// of the new state in g->partials (g->stage_blocks entries). Optional events
+// bracket the two stage kernels.
+static void enqueue_step(vh_grid* g, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, hipEvent_t e2 = nullptr) {
+  if (e0) (void)hipEventRecord(e0, g->stream);
+  enqueue_stage(g, false, g->U, g->U, g->U1);
+  if (e1) (void)hipEventRecord(e1, g->stream);
+  enqueue_stage(g, true, g->U1, g->U, g->U);
+  if (e2) (void)hipEventRecord(e2, g->stream);
+  hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
+}
+
+extern "C" {
+
+const char* vh_last_error(void) { return g_err.c_str(); }
+int vh_abi_version(void) { return 1; }
+
+int vh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+// ---- tier 1 ----
+
+int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf, double gamma,
+                         double* dF, void* stream) {
   if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
   if (nf == 0) return 0;
-  if (!dWL || !dWR || !dax || !dF) return fail(-1, "null pointer%s", "");
-  const long long blocks = (nf + 255) / 256;
-  if (blocks > 0x7fffffffLL) return fail(-1, "too many faces%s (%lld)", "", nf);
-  hipLaunchKernelGGL((face_flux_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, dWL,
-                     dWR, dax, (long long)nf, T(gamma), dF);
+  if (!dWL || !dWR || !daxis || !dF) return fail(-1, "null pointer%s", "");
+  if ((nf + 255) / 256 > 0x7fffffffLL) return fail(-1, "too many faces%s (%lld)", "", nf);
+  exact::launch_face_flux(0, nf, (hipStream_t)stream, dWL, dWR, daxis, gamma, dF);
   VH_HIP(hipGetLastError());
   return 0;
 }
+
+int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf, double gamma,
+                         float* dF, void* stream) {
+  if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
+  if (nf == 0) return 0;
+  if (!dWL || !dWR || !daxis || !dF) return fail(-1, "null pointer%s", "");
+  if ((nf + 255) / 256 > 0x7fffffffLL) return fail(-1, "too many faces%s (%lld)", "", nf);
+  exact::launch_face_flux(1, nf, (hipStream_t)stream, dWL, dWR, daxis, gamma, dF);
+  VH_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"
 
 // Host-pointer form: persistent device staging per thread (grown on demand).
 struct Staging {
@@ -273,9 +227,6 @@ struct Staging {
   size_t bytes = 0;
   int device = -1;
   hipStream_t stream = nullptr;
-  ~Staging() {
-    // Process teardown: the runtime may already be gone; do not call into it.
-  }
 };
 static thread_local Staging g_stage;
 
@@ -307,34 +258,14 @@ static int face_flux_host(const T* WL, const T* WR, const int32_t* ax, int64_t n
   VH_HIP(hipMemcpyAsync(dWL, WL, nv, hipMemcpyHostToDevice, s));
   VH_HIP(hipMemcpyAsync(dWR, WR, nv, hipMemcpyHostToDevice, s));
   VH_HIP(hipMemcpyAsync(dax, ax, (size_t)nf * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  int rc = face_flux_dev<T>(dWL, dWR, dax, nf, gamma, dF, s);
-  if (rc) return rc;
+  exact::launch_face_flux(sizeof(T) == 8 ? 0 : 1, nf, s, dWL, dWR, dax, gamma, dF);
+  VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(F, dF, nv, hipMemcpyDeviceToHost, s));
   VH_HIP(hipStreamSynchronize(s));
   return 0;
 }
 
 extern "C" {
-
-const char* vh_last_error(void) { return g_err.c_str(); }
-int vh_abi_version(void) { return 1; }
-
-int vh_device_count(void) {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-  return n;
-}
-
-// ---- tier 1 ----
-
-int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf, double gamma,
-                         double* dF, void* stream) {
-  return face_flux_dev<double>(dWL, dWR, daxis, nf, gamma, dF, stream);
-}
-int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf, double gamma,
-                         float* dF, void* stream) {
-  return face_flux_dev<float>(dWL, dWR, daxis, nf, gamma, dF, stream);
-}
 
 int vh_face_flux_f64(const double* WL, const double* WR, const int32_t* axis, int64_t nf, double gamma,
                      double* F) {
@@ -366,6 +297,11 @@ int vh_buffer_download(void* host, const void* dptr, int64_t bytes) {
   if (bytes) VH_HIP(hipMemcpy(host, dptr, (size_t)bytes, hipMemcpyDeviceToHost));
   return 0;
 }
+int vh_buffer_copy(void* ddst, const void* dsrc, int64_t bytes) {
+  if (bytes < 0 || (bytes > 0 && (!ddst || !dsrc))) return fail(-1, "bad argument%s", "");
+  if (bytes) VH_HIP(hipMemcpy(ddst, dsrc, (size_t)bytes, hipMemcpyDeviceToDevice));
+  return 0;
+}
 
 // ---- tier 2 ----
 
@@ -379,6 +315,7 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
   vh_grid* g = new vh_grid();
   memset(g, 0, sizeof(*g));
   g->dtype = dtype;
+  g->mode = VH_EXACT;
   g->esize = dtype == VH_F64 ? 8 : 4;
   GridDev& G = g->G;
   G.rank = rank;
@@ -411,6 +348,7 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
   }
   G.ncell = n;
   G.gamma = gamma;
+  G.igm1 = 1.0 / (gamma - 1.0);
   // lib/RefMesh/src/RefMesh.jl:127 and :135-142 (left folds in Float64).
   double V = G.w[0];
   for (int d = 1; d < rank; ++d) V = V * G.w[d];
@@ -422,7 +360,9 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
     G.area[a] = ar;
   }
   if (hipGetDevice(&g->device) != hipSuccess) { delete g; return fail(-2, "no HIP device%s", ""); }
-  int rc = configure(g, 0, 0);
+  // Default launch shape per rank (measured on MI355X at 256^3 / 512^2; the
+  // autotuner in the host package can override it per workload).
+  int rc = configure(g, rank == 3 ? 2 : 0, rank == 3 ? 16 : 0);
   if (rc) { vh_grid_destroy(g); return rc; }
   const size_t bytes = (size_t)5 * n * g->esize;
   if (hipMalloc(&g->U, bytes) != hipSuccess || hipMalloc(&g->U1, bytes) != hipSuccess ||
@@ -459,6 +399,16 @@ int vh_grid_destroy(vh_grid* g) {
 
 int64_t vh_grid_ncell(const vh_grid* g) { return g ? g->G.ncell : -1; }
 
+int vh_grid_set_mode(vh_grid* g, int32_t mode) {
+  if (!g) return fail(-1, "null argument%s", "");
+  if (mode != VH_EXACT && mode != VH_FAST) return fail(-1, "mode must be VH_EXACT or VH_FAST%s (got %lld)", "", mode);
+  VH_HIP(hipStreamSynchronize(g->stream));
+  g->mode = mode;
+  return 0;
+}
+
+int vh_grid_get_mode(const vh_grid* g) { return g ? g->mode : -1; }
+
 int vh_grid_upload(vh_grid* g, const void* U) {
   if (!g || !U) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(g->device));
@@ -481,37 +431,6 @@ int vh_grid_state_device_ptr(vh_grid* g, void** dptr) {
   return 0;
 }
 
-static void enqueue_cfl(vh_grid* g) {
-  if (g->dtype == VH_F64)
-    launch_cfl<double, false>(g->G.rank, g->cfl_blocks, g->stream, (const double*)g->U, g->G, g->partials);
-  else
-    launch_cfl<float, true>(g->G.rank, g->cfl_blocks, g->stream, (const float*)g->U, g->G, g->partials);
-}
-
-// Enqueue one RK2 step reading dt from clock[1]; stage 2 leaves the partial CFL
-// maxima of the new state in g->partials (g->stage_blocks entries).
-static int enqueue_step(vh_grid* g) {
-  const double* dtp = g->clock + 1;
-  if (g->dtype == VH_F64) {
-    double* U = (double*)g->U;
-    double* U1 = (double*)g->U1;
-    launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, U1, dtp,
-                                g->partials, g->G);
-    launch_stage<double, false>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1, U, U, dtp,
-                                g->partials, g->G);
-  } else {
-    float* U = (float*)g->U;
-    float* U1 = (float*)g->U1;
-    launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, U1, dtp,
-                              g->partials, g->G);
-    launch_stage<float, true>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1, U, U, dtp,
-                              g->partials, g->G);
-  }
-  hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
-  VH_HIP(hipGetLastError());
-  return 0;
-}
-
 int vh_grid_compute_dt(vh_grid* g, double cfl, double* dt) {
   if (!g || !dt) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(g->device));
@@ -530,19 +449,20 @@ int vh_grid_step(vh_grid* g, double dt) {
   if (!g) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(g->device));
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, dt);
-  return enqueue_step(g);
+  enqueue_step(g);
+  VH_HIP(hipGetLastError());
+  return 0;
 }
 
 int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double* t_io, int64_t* steps) {
   if (!g || !t_io) return fail(-1, "null argument%s", "");
+  const bool clip = tfinal > 0.0;
+  if (!clip && max_steps <= 0) return fail(-1, "need tfinal > 0 or max_steps > 0%s", "");
   VH_HIP(hipSetDevice(g->device));
   double clk[4] = {*t_io, 0.0, 0.0, 0.0};
   VH_HIP(hipMemcpyAsync(g->clock, clk, sizeof clk, hipMemcpyHostToDevice, g->stream));
   int64_t k = 0;
-  const bool clip = tfinal > 0.0;
-  if (!clip && max_steps <= 0) return fail(-1, "need tfinal > 0 or max_steps > 0%s", "");
-  // Initial CFL partials from the current state.
-  enqueue_cfl(g);
+  enqueue_cfl(g);   // CFL partials of the current state
   int np = g->cfl_blocks;
   double t = *t_io;
   while (true) {
@@ -550,8 +470,8 @@ int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double
     if (clip && !(t < tfinal * (1 - 1e-12))) break;
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, tfinal,
                        clip ? 1 : 0, g->clock);
-    int rc = enqueue_step(g);
-    if (rc) return rc;
+    enqueue_step(g);
+    VH_HIP(hipGetLastError());
     np = g->stage_blocks;
     ++k;
     if (clip) {
@@ -595,20 +515,15 @@ int vh_grid_accumulate_flux(vh_grid* g, void* acc_host) {
   // Stage 1 with dt = −V gives invV = Tf(−V)/(Tf(1)·Tf(V)) = −1 exactly, so
   // U1 = U + acc; acc is recovered as U1 − U (rounded; inspection only).
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, -g->G.V);
-  const double* dtp = g->clock + 1;
   const long long n5 = 5 * g->G.ncell;
   const unsigned blocks = (unsigned)((n5 + 255) / 256);
-  if (g->dtype == VH_F64) {
-    launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (const double*)g->U,
-                                (const double*)g->U, (double*)g->U1, dtp, g->partials, g->G);
+  enqueue_stage(g, false, g->U, g->U, g->U1);
+  if (g->dtype == VH_F64)
     hipLaunchKernelGGL(sub_kernel<double>, dim3(blocks), dim3(256), 0, g->stream, (const double*)g->U1,
                        (const double*)g->U, (double*)g->U1, n5);
-  } else {
-    launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (const float*)g->U,
-                              (const float*)g->U, (float*)g->U1, dtp, g->partials, g->G);
+  else
     hipLaunchKernelGGL(sub_kernel<float>, dim3(blocks), dim3(256), 0, g->stream, (const float*)g->U1,
                        (const float*)g->U, (float*)g->U1, n5);
-  }
   VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(acc_host, g->U1, (size_t)n5 * g->esize, hipMemcpyDeviceToHost, g->stream));
   VH_HIP(hipStreamSynchronize(g->stream));
@@ -652,26 +567,10 @@ int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1
   if (!rc) {
     enqueue_cfl(g);
     int np = g->cfl_blocks;
-    const double* dtp = g->clock + 1;
     for (int64_t k = 0; k < nsteps; ++k) {
       hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, 0.0, 0,
                          g->clock);
-      (void)hipEventRecord(ev[3 * k], g->stream);
-      if (g->dtype == VH_F64) {
-        launch_stage<double, false>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (double*)g->U,
-                                    (double*)g->U, (double*)g->U1, dtp, g->partials, g->G);
-        (void)hipEventRecord(ev[3 * k + 1], g->stream);
-        launch_stage<double, false>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, (double*)g->U1,
-                                    (double*)g->U, (double*)g->U, dtp, g->partials, g->G);
-      } else {
-        launch_stage<float, true>(g->G.rank, g->variant, false, g->stage_blocks, g->stream, (float*)g->U,
-                                  (float*)g->U, (float*)g->U1, dtp, g->partials, g->G);
-        (void)hipEventRecord(ev[3 * k + 1], g->stream);
-        launch_stage<float, true>(g->G.rank, g->variant, true, g->stage_blocks, g->stream, (float*)g->U1,
-                                  (float*)g->U, (float*)g->U, dtp, g->partials, g->G);
-      }
-      (void)hipEventRecord(ev[3 * k + 2], g->stream);
-      hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
+      enqueue_step(g, ev[3 * k], ev[3 * k + 1], ev[3 * k + 2]);
       np = g->stage_blocks;
     }
     hipError_t e = hipGetLastError();

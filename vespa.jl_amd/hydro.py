@@ -305,6 +305,15 @@ class Simulation:
         check(self._lib.vh_grid_state_device_ptr(self._h, C.byref(p)))
         return int(p.value)
 
+    def set_mode(self, mode: str):
+        """'exact' (bit-identical to the reference arithmetic) or 'fast'."""
+        code = {"exact": _lib.VH_EXACT, "fast": _lib.VH_FAST}[mode]
+        check(self._lib.vh_grid_set_mode(self._h, code))
+
+    @property
+    def mode(self) -> str:
+        return {_lib.VH_EXACT: "exact", _lib.VH_FAST: "fast"}[self._lib.vh_grid_get_mode(self._h)]
+
     def set_launch(self, tile_variant: int = 0, zchunk: int = 0):
         check(self._lib.vh_grid_set_launch(self._h, int(tile_variant), int(zchunk)))
 
