@@ -43,10 +43,11 @@ def load_package():
     return mod
 
 
-def make_sim(V, n, dtype, U, tile, zchunk):
+def make_sim(V, n, dtype, U, tile, zchunk, mode="fast"):
     prob = V.Problem("turbulence256", (n, n, n), ((0.0, 1.0),) * 3, 5.0 / 3.0, "periodic",
                      lambda x, y, z: (1.0, 0.0, 0.0, 0.0, 1.0), 1.0, 0.3)
     sim = V.Simulation(V.UniformMesh((n, n, n), prob.domain), prob, eltype=dtype)
+    sim.set_mode(mode)
     if tile >= 0 or zchunk > 0:
         sim.set_launch(max(tile, 0), zchunk)
     sim.upload(U)
@@ -92,7 +93,12 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--tile", type=int, default=-1, help="tile variant (-1: library default)")
     ap.add_argument("--zchunk", type=int, default=0)
+    ap.add_argument("--mode", choices=("fast", "exact"), default="fast",
+                    help="arithmetic profile of the headline FP64 run (fast: within 1e-12 of the "
+                         "reference arithmetic; exact: bit-identical)")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-other-mode", action="store_true",
+                    help="skip the FP64 run in the other arithmetic profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also time 3 device-to-device copies of 1 GiB (counter calibration)")
@@ -130,8 +136,8 @@ def main():
         b.free()
         log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
 
-    log("[bench] FP64 run")
-    sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk)
+    log(f"[bench] FP64 run ({args.mode} profile)")
+    sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, args.mode)
     launch = sim.launch()
     wall, ev = timed_run(V, sim, args.steps, args.warmup)
     value = ncell * args.steps / wall
@@ -141,7 +147,7 @@ def main():
     ach = stage2_bytes / (ev["stage2_ms"] * 1e-3) / 1e9
     log(f"[bench] FP64 {value / 1e9:.3f} G zone-updates/s, {ms_per_step:.3f} ms/step, "
         f"stage1 {ev['stage1_ms']:.3f} ms, stage2 {ev['stage2_ms']:.3f} ms, stage2 {ach:.0f} GB/s")
-    U64_end = sim.state() if not args.no_fp32 else None
+    U64_end = sim.state() if not (args.no_fp32 and args.no_other_mode) else None
     sim.close()
 
     traffic = None
@@ -164,6 +170,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
+        "arithmetic_profile": args.mode,
         "data": "synthetic",
         "config": {"workload": f"3D {n}^3 periodic smooth turbulence-like state (rho=1+0.1*delta, "
                                f"|k|<=8 random-phase modes, v_rms=0.1 c_s, p=1), gamma=5/3, "
@@ -183,17 +190,34 @@ def main():
     if calib is not None:
         out["calibration"] = calib
 
+    if not args.no_other_mode:
+        other = "exact" if args.mode == "fast" else "fast"
+        log(f"[bench] FP64 run ({other} profile)")
+        so = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, other)
+        wo, evo = timed_run(V, so, args.steps, args.warmup)
+        Uo_end = so.state()
+        so.close()
+        vo = ncell * args.steps / wo
+        rel = (np.abs(Uo_end - U64_end).reshape(5, -1).max(axis=1) /
+               np.abs(U64_end).reshape(5, -1).max(axis=1)) if U64_end is not None else None
+        out["fp64_" + other] = {"value": vo, "unit": "zone-updates/s", "ms_per_step": 1e3 * wo / args.steps,
+                                "stage1_ms": evo["stage1_ms"], "stage2_ms": evo["stage2_ms"],
+                                "stage2_GBps": stage2_bytes / (evo["stage2_ms"] * 1e-3) / 1e9,
+                                "max_rel_diff_vs_headline_per_field":
+                                    rel.tolist() if rel is not None else None}
+        log(f"[bench] FP64 {other}: {vo / 1e9:.3f} G zone-updates/s")
+
     if not args.no_fp32:
         log("[bench] FP32 run")
         U32 = U64.astype(np.float32)
-        s32 = make_sim(V, n, np.float32, U32, args.tile, args.zchunk)
+        s32 = make_sim(V, n, np.float32, U32, args.tile, args.zchunk, args.mode)
         wall32, ev32 = timed_run(V, s32, args.steps, args.warmup)
         v32 = ncell * args.steps / wall32
         U32_end = s32.state().astype(np.float64)
         s32.close()
         den = np.abs(U64_end).mean(axis=(1, 2, 3))
         l1 = (np.abs(U32_end - U64_end).mean(axis=(1, 2, 3)) / den).tolist()
-        out["fp32"] = {"value": v32, "unit": "zone-updates/s", "ratio_vs_fp64": v32 / value,
+        out["fp32"] = {"arithmetic_profile": args.mode, "value": v32, "unit": "zone-updates/s", "ratio_vs_fp64": v32 / value,
                        "ms_per_step": 1e3 * wall32 / args.steps, "stage1_ms": ev32["stage1_ms"],
                        "stage2_ms": ev32["stage2_ms"],
                        "stage2_GBps": 15 * 4 * ncell / (ev32["stage2_ms"] * 1e-3) / 1e9,

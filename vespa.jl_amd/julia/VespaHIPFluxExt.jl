@@ -1,0 +1,150 @@
+# VespaHIPFluxExt — Julia binding of libvespahip (gfx950) behind Vespa.jl's flux seam.
+#
+# Drop-in for the KernelAbstractions device backend (ext/VespaKAFluxExt.jl):
+#   * tier 1: `HIPFlux <: Vespa.AbstractFluxBackend` with a
+#     `Vespa.compute_face_fluxes!` method (contract src/ka_flux.jl:40-53), so
+#     `sim.flux = HIPFlux()` routes `accumulate_flux!` (src/driver.jl:210-211)
+#     through the batched gather → GPU HLLC → scatter path unchanged;
+#   * tier 2: `HIPGrid`, a device-resident uniform grid whose `hip_step!` /
+#     `hip_evolve!` replace `step!` / `evolve!` (src/driver.jl:333-359, 483-508)
+#     for RefMesh `UniformMesh` simulations, and `hip_set_mode!` selecting the
+#     exact (bit-identical) or fast arithmetic profile.
+#
+# Load with `include` after `using Vespa`; set ENV["VESPAHIP_LIB"] or keep
+# libvespahip.so next to the package's vespa.jl_amd/ directory.
+module VespaHIPFluxExt
+
+using Vespa
+using Vespa: AbstractFluxBackend, adiabatic_index
+using MeshInterface: Outflow, Periodic, Reflecting, bc_on
+
+const libvespahip = get(ENV, "VESPAHIP_LIB", joinpath(@__DIR__, "..", "libvespahip.so"))
+
+const VH_F64 = Int32(0)
+const VH_F32 = Int32(1)
+const VH_EXACT = Int32(0)
+const VH_FAST = Int32(1)
+
+_check(rc) = rc == 0 || error("vespahip: ", unsafe_string(ccall((:vh_last_error, libvespahip), Cstring, ())))
+
+# ------------------------------------------------------------------ tier 1
+
+"GPU flux backend: the per-face HLLC of `compute_face_fluxes!` on gfx950."
+struct HIPFlux <: AbstractFluxBackend end
+
+function Vespa.compute_face_fluxes!(::HIPFlux, model, F::Vector{NTuple{5,T}}, WL::Vector{NTuple{5,T}},
+                                    WR::Vector{NTuple{5,T}}, axisv::Vector{Int}) where {T<:Union{Float64,Float32}}
+    nf = length(F)
+    nf == 0 && return F
+    ax = convert(Vector{Int32}, axisv)
+    γ = Float64(adiabatic_index(model))
+    fn = T === Float64 ? :vh_face_flux_f64 : :vh_face_flux_f32
+    GC.@preserve F WL WR ax begin
+        pF = Ptr{T}(pointer(F)); pL = Ptr{T}(pointer(WL)); pR = Ptr{T}(pointer(WR))
+        rc = T === Float64 ?
+            ccall((:vh_face_flux_f64, libvespahip), Cint,
+                  (Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Int64, Float64, Ptr{Float64}),
+                  pL, pR, ax, nf, γ, pF) :
+            ccall((:vh_face_flux_f32, libvespahip), Cint,
+                  (Ptr{Float32}, Ptr{Float32}, Ptr{Int32}, Int64, Float64, Ptr{Float32}),
+                  pL, pR, ax, nf, γ, pF)
+        _check(rc)
+    end
+    return F
+end
+
+# ------------------------------------------------------------------ tier 2
+
+_bc_code(::Periodic) = Int32(0)
+_bc_code(::Outflow) = Int32(1)
+_bc_code(::Reflecting) = Int32(2)
+_bc_code(bc) = error("vespahip: boundary condition $(typeof(bc)) is not supported on the device path")
+
+"Device-resident state of a `Simulation` on a RefMesh `UniformMesh`."
+mutable struct HIPGrid
+    ptr::Ptr{Cvoid}
+    ncell::Int
+    eltype::DataType
+end
+
+function HIPGrid(sim::Vespa.Simulation)
+    b = sim.backend
+    N = Vespa.rank(b)
+    dims = Int64[i <= N ? b.dims[i] : 1 for i in 1:3]
+    lo = Float64[i <= N ? b.lo[i] : 0.0 for i in 1:3]
+    hi = Float64[i <= N ? b.hi[i] : 1.0 for i in 1:3]
+    bc = Int32[]
+    for d in 1:3
+        if d <= N
+            push!(bc, _bc_code(bc_on(sim.bcs, d, :lo)), _bc_code(bc_on(sim.bcs, d, :hi)))
+        else
+            push!(bc, Int32(1), Int32(1))
+        end
+    end
+    T = eltype(sim.sv[1])
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:vh_grid_create, libvespahip), Cint,
+                 (Ref{Ptr{Cvoid}}, Int32, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Float64, Int32),
+                 h, Int32(N), dims, lo, hi, bc, Float64(sim.γ), T === Float64 ? VH_F64 : VH_F32))
+    g = HIPGrid(h[], prod(dims), T)
+    finalizer(g) do x
+        x.ptr == C_NULL || ccall((:vh_grid_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    hip_upload!(g, sim)
+    return g
+end
+
+"Copy the simulation's SoA conserved fields (one dense array per field) to the device."
+function hip_upload!(g::HIPGrid, sim)
+    buf = Vector{g.eltype}(undef, 5 * g.ncell)
+    for f in 1:5
+        copyto!(buf, (f - 1) * g.ncell + 1, vec(sim.sv[f]), 1, g.ncell)
+    end
+    GC.@preserve buf _check(ccall((:vh_grid_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, buf))
+    return g
+end
+
+"Copy the device state back into the simulation's field arrays."
+function hip_download!(sim, g::HIPGrid)
+    buf = Vector{g.eltype}(undef, 5 * g.ncell)
+    GC.@preserve buf _check(ccall((:vh_grid_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, buf))
+    for f in 1:5
+        copyto!(vec(sim.sv[f]), 1, buf, (f - 1) * g.ncell + 1, g.ncell)
+    end
+    return sim
+end
+
+hip_set_mode!(g::HIPGrid, fast::Bool) =
+    (_check(ccall((:vh_grid_set_mode, libvespahip), Cint, (Ptr{Cvoid}, Int32), g.ptr, fast ? VH_FAST : VH_EXACT)); g)
+
+"step!(sim, dt) on the device (src/driver.jl:333-359)."
+function hip_step!(sim, g::HIPGrid, dt::Real)
+    _check(ccall((:vh_grid_step, libvespahip), Cint, (Ptr{Cvoid}, Float64), g.ptr, Float64(dt)))
+    sim.t += dt
+    sim.step += 1
+    return sim
+end
+
+"evolve!(sim) on the device (src/driver.jl:483-508): CFL dt reduced on the GPU each step."
+function hip_evolve!(sim, g::HIPGrid; max_steps::Integer = 0)
+    t = Ref(Float64(sim.t))
+    k = Ref{Int64}(0)
+    _check(ccall((:vh_grid_run, libvespahip), Cint, (Ptr{Cvoid}, Float64, Float64, Int64, Ref{Float64}, Ref{Int64}),
+                 g.ptr, Float64(sim.problem.cfl), Float64(sim.problem.tfinal), Int64(max_steps), t, k))
+    sim.t = t[]
+    sim.step += k[]
+    hip_download!(sim, g)
+    return sim
+end
+
+"conserved_totals(sim) (src/diagnostics.jl:16-31) from the device state."
+function hip_conserved_totals(g::HIPGrid)
+    out = zeros(Float64, 5)
+    _check(ccall((:vh_grid_totals, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Float64}), g.ptr, out))
+    return (mass = out[1], momentum_x = out[2], momentum_y = out[3], momentum_z = out[4], energy = out[5])
+end
+
+export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals
+
+end # module
