@@ -279,3 +279,83 @@ def exact_riemann_sample(WL, WR, gamma, s):
     out = np.zeros(3)
     lib().vo_exact_riemann_sample(_ptr(a), _ptr(b), gamma, float(s), _ptr(out))
     return out
+
+
+# ---------------- block-structured AMR (blocks_oracle.c) ----------------
+
+class BlockOracle:
+    """Block-mesh restatement: `mesh` is a vespa.jl_amd.blockmesh.BlockMesh (or
+    anything with the same fields)."""
+
+    def __init__(self, mesh):
+        L = lib()
+        if not getattr(L, "_bo_typed", False):
+            vp = C.c_void_p
+            L.bo_create.restype = vp
+            L.bo_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.c_double,
+                                    vp, vp, vp, vp]
+            L.bo_destroy.argtypes = [vp]
+            L.bo_ncell.argtypes = [vp]
+            L.bo_ncell.restype = C.c_int64
+            L.bo_validate.argtypes = [vp]
+            L.bo_validate.restype = C.c_int
+            for suf in ("f64", "f32"):
+                getattr(L, f"bo_accumulate_flux_{suf}").argtypes = [vp, vp, vp, C.c_int]
+                getattr(L, f"bo_step_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, vp, C.c_int]
+                getattr(L, f"bo_compute_dt_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, C.c_int]
+                getattr(L, f"bo_compute_dt_{suf}").restype = C.c_double
+                getattr(L, f"bo_totals_{suf}").argtypes = [vp, vp, vp]
+                getattr(L, f"bo_evolve_{suf}").argtypes = [vp, vp, C.c_double, C.c_double, C.c_int64,
+                                                           C.c_int, C.c_int, C.POINTER(C.c_double)]
+                getattr(L, f"bo_evolve_{suf}").restype = C.c_int64
+            L._bo_typed = True
+        self.L = L
+        self.mesh = mesh
+        self._keep = [np.ascontiguousarray(mesh.level, dtype=np.int32),
+                      np.ascontiguousarray(mesh.dims, dtype=np.int32),
+                      np.ascontiguousarray(mesh.origin, dtype=np.int64),
+                      np.ascontiguousarray(mesh.bc, dtype=np.int32)]
+        h0 = (C.c_double * 3)(*mesh.h0[:3])
+        self.h = L.bo_create(mesh.rank, len(mesh.level), mesh.maxlevel, h0, mesh.gamma,
+                             *[a.ctypes.data for a in self._keep])
+        if not self.h:
+            raise ValueError("invalid block mesh")
+        self.ncell = int(L.bo_ncell(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.bo_destroy(self.h)
+            self.h = None
+
+    def validate(self) -> int:
+        return int(self.L.bo_validate(self.h))
+
+    def accumulate_flux(self, U, floor64=False):
+        U = np.ascontiguousarray(U)
+        acc = np.zeros_like(U)
+        getattr(self.L, f"bo_accumulate_flux_{_suf(U.dtype)}")(self.h, U.ctypes.data, acc.ctypes.data, int(floor64))
+        return acc
+
+    def step(self, U, dt, level=-1, floor64=False):
+        assert U.flags.c_contiguous
+        work = np.empty(10 * self.ncell, dtype=U.dtype)
+        getattr(self.L, f"bo_step_{_suf(U.dtype)}")(self.h, U.ctypes.data, float(dt), int(level), work.ctypes.data,
+                                                   int(floor64))
+
+    def compute_dt(self, U, cfl, level=-1, floor64=False):
+        U = np.ascontiguousarray(U)
+        return getattr(self.L, f"bo_compute_dt_{_suf(U.dtype)}")(self.h, U.ctypes.data, cfl, int(level),
+                                                                int(floor64))
+
+    def totals(self, U):
+        U = np.ascontiguousarray(U)
+        out = np.zeros(5)
+        getattr(self.L, f"bo_totals_{_suf(U.dtype)}")(self.h, U.ctypes.data, out.ctypes.data)
+        return out
+
+    def evolve(self, U, tfinal, cfl, max_steps=0, subcycle=False, floor64=False, t0=0.0):
+        assert U.flags.c_contiguous
+        t = C.c_double(t0)
+        k = getattr(self.L, f"bo_evolve_{_suf(U.dtype)}")(self.h, U.ctypes.data, tfinal, cfl, int(max_steps),
+                                                         int(subcycle), int(floor64), C.byref(t))
+        return t.value, int(k)

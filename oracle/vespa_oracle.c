@@ -249,3 +249,6 @@ void vo_exact_riemann_sample(const double* WL, const double* WR, double g, doubl
 }
 
 int vo_abi_version(void) { return 1; }
+
+/* Block-structured AMR restatement (shares the scalar kernels above). */
+#include "blocks_oracle.c"

@@ -1,0 +1,212 @@
+"""Block-structured AMR mesh description (host data) for the batched block path.
+
+A mesh is a list of blocks. Block b has a refinement level L_b (0 = base), a
+cell count per axis ``dims[b]`` and an integer ``origin[b]`` on the lattice of the
+finest cells (one lattice unit = one cell of level ``maxlevel``), so a level-L
+cell spans 2^(maxlevel−L) lattice units and ``h0/2^L`` physical units. Leaves
+(cells) are ordered block by block and column-major (x fastest) inside a block;
+field f of cell c of block b lives at ``U[f*ncell + offset[b] + c]``. Blocks touch
+only geometrically; ``bc[b]`` gives the boundary condition of each block face
+where no other block touches it (periodic wraps the block onto itself).
+
+Constraints (checked by ``validate``): adjacent leaves differ by at most one
+level (2:1 balance, as HGBackend keeps it, lib/HGBackend/src/HGBackend.jl:26-29)
+and a fine block's origin and extent are aligned to the cells of any coarser
+block it touches, so each coarse face sees the 2^(rank−1) fine cells of one block.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PER, OUT, REF = 0, 1, 2
+
+
+@dataclass
+class BlockMesh:
+    rank: int
+    maxlevel: int
+    h0: tuple                      # level-0 cell width per axis
+    gamma: float
+    level: np.ndarray              # (nb,) int32
+    dims: np.ndarray               # (nb, 3) int32 (axes ≥ rank hold 1)
+    origin: np.ndarray             # (nb, 3) int64 lattice units
+    bc: np.ndarray                 # (nb, 6) int32, [2*axis + side]
+    x0: tuple = (0.0, 0.0, 0.0)    # physical coordinate of lattice point 0
+
+    def __post_init__(self):
+        self.level = np.ascontiguousarray(self.level, dtype=np.int32)
+        self.dims = np.ascontiguousarray(self.dims, dtype=np.int32).reshape(-1, 3)
+        self.origin = np.ascontiguousarray(self.origin, dtype=np.int64).reshape(-1, 3)
+        self.bc = np.ascontiguousarray(self.bc, dtype=np.int32).reshape(-1, 6)
+        self.h0 = tuple(float(h) for h in self.h0) + (1.0,) * (3 - len(self.h0))
+        self.x0 = tuple(float(x) for x in self.x0) + (0.0,) * (3 - len(self.x0))
+
+    @property
+    def nblocks(self) -> int:
+        return len(self.level)
+
+    @property
+    def counts(self) -> np.ndarray:
+        return np.prod(self.dims.astype(np.int64), axis=1)
+
+    @property
+    def offsets(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+
+    @property
+    def ncell(self) -> int:
+        return int(self.counts.sum())
+
+    def span(self, b: int) -> int:
+        return 1 << (self.maxlevel - int(self.level[b]))
+
+    def width(self, b: int):
+        return tuple(self.h0[d] / (1 << int(self.level[b])) for d in range(3))
+
+    def cell_centers(self, b: int):
+        """Physical centres of block b's cells as (z, y, x)-shaped arrays per axis."""
+        s = self.span(b)
+        u = self.h0[0] / (1 << self.maxlevel)
+        out = []
+        for d in range(3):
+            n = int(self.dims[b, d])
+            ud = self.h0[d] / (1 << self.maxlevel)
+            lat = self.origin[b, d] + (np.arange(n) + 0.5) * s
+            out.append(self.x0[d] + lat * ud)
+        Z, Y, X = np.meshgrid(out[2], out[1], out[0], indexing="ij")
+        return X, Y, Z
+
+    def validate(self) -> None:
+        """Raise ValueError when two blocks overlap, a level jump exceeds one or a
+        fine block is not aligned to the coarse cells it touches."""
+        nb, r = self.nblocks, self.rank
+        lo = self.origin[:, :r]
+        hi = lo + self.dims[:, :r].astype(np.int64) * np.array([self.span(b) for b in range(nb)])[:, None]
+        for b in range(nb):
+            if (self.dims[b, :r] < 1).any() or (self.dims[b, r:] != 1).any():
+                raise ValueError(f"block {b}: bad dims {self.dims[b]}")
+            if (self.origin[b, :r] % self.span(b)).any():
+                raise ValueError(f"block {b}: origin not aligned to its own cells")
+            for d in range(r):
+                if (self.bc[b, 2 * d] == PER) != (self.bc[b, 2 * d + 1] == PER):
+                    raise ValueError(f"block {b}: periodic must be set on both sides of axis {d}")
+        for a in range(nb):
+            for b in range(a + 1, nb):
+                ov = np.minimum(hi[a], hi[b]) - np.maximum(lo[a], lo[b])
+                if (ov > 0).all():
+                    raise ValueError(f"blocks {a} and {b} overlap")
+                touch = (ov >= 0).all() and (ov == 0).sum() == 1
+                if not touch:
+                    continue
+                la, lb = int(self.level[a]), int(self.level[b])
+                if abs(la - lb) > 1:
+                    raise ValueError(f"blocks {a} and {b} touch across a level jump of {abs(la - lb)}")
+                d = int(np.nonzero(ov == 0)[0][0])
+                if self.bc[a, 2 * d] == PER or self.bc[b, 2 * d] == PER:
+                    raise ValueError(f"blocks {a} and {b} touch along a periodic axis")
+                if la != lb:
+                    f, c = (a, b) if la > lb else (b, a)
+                    sc = self.span(c)
+                    for e in range(r):
+                        if e == d:
+                            continue
+                        # fine extent must start and end on coarse cell boundaries inside the overlap
+                        olo = max(lo[f][e], lo[c][e])
+                        ohi = min(hi[f][e], hi[c][e])
+                        if (olo - lo[c][e]) % sc or (ohi - lo[c][e]) % sc:
+                            raise ValueError(f"fine block {f} is not aligned to the cells of block {c}")
+
+
+def uniform_as_blocks(dims, domain, bcs, gamma) -> BlockMesh:
+    """One block covering a UniformMesh (the RefMesh case)."""
+    rank = len(dims)
+    h0 = tuple((hi - lo) / n for (lo, hi), n in zip(domain, dims))
+    d3 = list(dims) + [1] * (3 - rank)
+    bc = []
+    for d in range(3):
+        if d < rank:
+            lo, hi = bcs[d] if isinstance(bcs, (list, tuple)) else (bcs, bcs)
+            bc += [lo, hi]
+        else:
+            bc += [OUT, OUT]
+    return BlockMesh(rank, 0, h0, gamma, [0], [d3], [[0, 0, 0]], [bc],
+                     x0=tuple(lo for lo, _ in domain))
+
+
+def random_block_pairs(nblocks: int, seed: int, rank: int = 3, edges=(8, 16, 24, 32), p_link: float = 0.5,
+                       p_fine: float = 0.5, gamma: float = 5.0 / 3.0) -> BlockMesh:
+    """The batched-blocks workload: `nblocks` blocks with edges drawn uniformly from
+    `edges`, laid out far apart as independent units, except that a random subset
+    is glued face to face in pairs: conforming pairs (same level, same face extent)
+    and 2:1 pairs (a level-1 block against a level-0 block, aligned to its cells).
+    Faces that touch nothing get outflow or reflecting boundaries at random."""
+    rng = np.random.default_rng(seed)
+    maxlevel = 1
+    level, dims, origin, bc = [], [], [], []
+    slot = 0
+    pitch = 4 * max(edges) * 2            # lattice units between independent units
+
+    def place(lv, e, org):
+        level.append(lv)
+        d3 = [e if d < rank else 1 for d in range(3)]
+        dims.append(d3)
+        origin.append(org)
+        bc.append([int(rng.integers(1, 3)) if d < rank else OUT for d in range(3) for _ in (0, 1)])
+
+    b = 0
+    while b < nblocks:
+        base = [slot * pitch if d == 0 else 0 for d in range(3)]
+        slot += 1
+        e = int(rng.choice(edges))
+        if b + 1 < nblocks and rng.random() < p_link:
+            ax = int(rng.integers(0, rank))
+            if rng.random() < p_fine:
+                # coarse block (level 0, span 2) and a fine block (level 1, span 1)
+                ef = int(rng.choice([x for x in edges if x // 2 <= e]))
+                place(0, e, base)
+                org = list(base)
+                org[ax] = base[ax] + 2 * e            # touches the coarse hi face
+                for d in range(rank):
+                    if d != ax:
+                        room = 2 * e - ef             # lattice units free along d
+                        org[d] = base[d] + 2 * int(rng.integers(0, room // 2 + 1)) if room > 0 else base[d]
+                place(1, ef, org)
+            else:
+                place(0, e, base)
+                org = list(base)
+                org[ax] = base[ax] + 2 * e
+                place(0, e, org)
+            b += 2
+        else:
+            place(0, e, base)
+            b += 1
+    h0 = tuple([1.0 / 32] * rank)
+    return BlockMesh(rank, maxlevel, h0, gamma, level, dims, origin, bc)
+
+
+def smooth_block_state(mesh: BlockMesh, seed: int, dtype=np.float64):
+    """A smooth seeded state per block: ρ, p ∈ [0.5, 2], |v| ≤ 0.3 (conserved, SoA)."""
+    rng = np.random.default_rng(seed)
+    U = np.empty((5, mesh.ncell), dtype=dtype)
+    off = mesh.offsets
+    g = mesh.gamma
+    for b in range(mesh.nblocks):
+        X, Y, Z = mesh.cell_centers(b)
+        k = rng.uniform(1.0, 4.0, (5, 3))
+        ph = rng.uniform(0, 2 * np.pi, 5)
+        f = [np.sin(2 * np.pi * (k[i, 0] * X + k[i, 1] * Y + k[i, 2] * Z) + ph[i]) for i in range(5)]
+        rho = 1.25 + 0.75 * f[0]
+        vx, vy, vz = 0.17 * f[1], 0.17 * f[2], 0.17 * f[3]
+        if mesh.rank < 3:
+            vz = 0.0 * vz
+        if mesh.rank < 2:
+            vy = 0.0 * vy
+        p = 1.25 + 0.75 * f[4]
+        W = np.stack([rho, vx, vy, vz, p]).reshape(5, -1).astype(dtype)
+        T = np.dtype(dtype).type
+        r, a, bb, c, pp = W
+        E = pp / (T(g) - T(1)) + (T(0.5) * r) * ((a * a + bb * bb) + c * c)
+        U[:, off[b]:off[b + 1]] = np.stack([r, r * a, r * bb, r * c, E])
+    return U
