@@ -157,6 +157,79 @@ int vh_grid_get_launch(const vh_grid* g, int32_t* tile_x, int32_t* tile_y, int32
 int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1, double* ms_stage2,
                        double* ms_step);
 
+/* ---------------------------------------------------------------------------
+ * Tier 2 — batched blocks (block-structured AMR with hanging nodes).
+ * A mesh is a list of blocks: block b has a refinement level level[b] (0 =
+ * base, at most maxlevel), cells per axis dims[3*b + d] and an integer origin
+ * origin[3*b + d] on the lattice of the finest cells (one unit = one cell of
+ * level maxlevel; a level-L cell spans 2^(maxlevel-L) units and h0[d]/2^L
+ * physical units). Leaves are ordered block by block, column-major (x fastest)
+ * inside a block; the state is SoA over that order, U[f*ncell + leaf]. bc[6*b +
+ * 2*d + side] applies where no other block touches the face; periodic wraps
+ * the block onto itself. Blocks that touch must differ by at most one level,
+ * and each coarse face cell must see the 2^(rank-1) finer cells of one block.
+ * Faces follow HGBackend (lib/HGBackend/src/HGBackend.jl:202-276):
+ * conforming faces once from the hi side; coarse-fine faces once from the fine
+ * side with the fine area; reconstruction across a block face uses one
+ * representative neighbour (lib/HGBackend/src/HGBackend.jl:178-200): the
+ * neighbour cell, the coarse cell containing the position, or the finer cell at
+ * the lo corner of the coarse face. One persistent launch per RK stage covers
+ * every block.
+ */
+typedef struct vh_blocks vh_blocks;
+
+/* Validate a block mesh and count its derived tables without a device:
+ * counts6 = {cells, tiles, link faces, link-cell records, flux-register cells,
+ * face slots}. zchunk: z planes per tile (0 = whole block). */
+int vh_blocks_describe(int32_t rank, int32_t nblocks, int32_t maxlevel, const double* h0,
+                       const int32_t* level, const int32_t* dims, const int64_t* origin,
+                       const int32_t* bc, int32_t zchunk, int64_t* counts6);
+
+/* Create a block-mesh context (the HGBackend Simulation, src/driver.jl:27-79
+ * with lib/HGBackend/src/HGBackend.jl:300-340) on the current device. */
+int vh_blocks_create(vh_blocks** out, int32_t rank, int32_t nblocks, int32_t maxlevel,
+                     const double* h0, double gamma, const int32_t* level, const int32_t* dims,
+                     const int64_t* origin, const int32_t* bc, int32_t dtype);
+int vh_blocks_destroy(vh_blocks* m);
+int64_t vh_blocks_ncell(const vh_blocks* m);
+int vh_blocks_set_mode(vh_blocks* m, int32_t mode);
+int vh_blocks_upload(vh_blocks* m, const void* U);
+int vh_blocks_download(vh_blocks* m, void* U);
+
+/* compute_dt(sim; level) (src/driver.jl:169-199); level < 0: every leaf. */
+int vh_blocks_compute_dt(vh_blocks* m, double cfl, int32_t level, double* dt);
+
+/* step!(sim, dt; level) (src/driver.jl:333-378): level < 0 updates every leaf;
+ * level >= 0 updates that level's leaves only (others are read). No flux
+ * registers. Stream-ordered. */
+int vh_blocks_step(vh_blocks* m, double dt, int32_t level);
+
+/* evolve! (src/driver.jl:483-508): subcycle = 0, a global dt each step from the
+ * device CFL reduction (tfinal <= 0: exactly max_steps steps, no host
+ * synchronisation inside the loop); subcycle = 1, evolve_level! recursion with
+ * per-level dt and flux registers (src/driver.jl:553-605, src/reflux.jl:51-103),
+ * which needs tfinal > 0. *t_io in/out, *steps = root steps taken. */
+int vh_blocks_run(vh_blocks* m, double cfl, double tfinal, int64_t max_steps, int32_t subcycle,
+                  double* t_io, int64_t* steps);
+
+/* conserved_totals(sim) (src/diagnostics.jl:16-31): sum of U*V in Float64. */
+int vh_blocks_totals(vh_blocks* m, double* out5);
+
+/* Launch shape: z planes per tile (0 = whole block), cap on the persistent
+ * workgroups (0 = 3 per compute unit), and whether the single-rate step is
+ * replayed from a captured HIP graph (1, the default) or enqueued eagerly. */
+int vh_blocks_set_launch(vh_blocks* m, int32_t zchunk, int32_t max_workgroups, int32_t use_graph);
+int vh_blocks_info(const vh_blocks* m, int64_t* counts6, int32_t* zchunk, int32_t* workgroups);
+int vh_blocks_sync(vh_blocks* m);
+
+/* Time nsteps single-rate steps (device dt) with HIP events after two untimed
+ * steps. method 0: one persistent launch per stage, eager; 1: one launch per
+ * block per stage (the comparison loop); 2: persistent launches replayed from a
+ * HIP graph. ms_stage2 (optional): mean duration of the persistent stage-2
+ * launch over up to 20 further eager steps. */
+int vh_blocks_time_steps(vh_blocks* m, double cfl, int64_t nsteps, int32_t method, double* ms_step,
+                         double* ms_stage2);
+
 #ifdef __cplusplus
 }
 #endif

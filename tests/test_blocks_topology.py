@@ -1,0 +1,119 @@
+"""Host topology of the batched block path (no device): the library's table
+builder against an independent count of contact faces, and its mesh checks.
+
+A link face is a face between two different blocks; there is one per finer-side
+cell on every contact (HGBackend emits coarse-fine faces once per fine leaf,
+lib/HGBackend/src/HGBackend.jl:202-276). Flux-register cells are the coarse
+cells with finer neighbours (src/reflux.jl:51-103).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import load_vespa
+
+V = load_vespa()
+from vespa_jl_amd import blockmesh as BM  # noqa: E402
+from vespa_jl_amd._lib import VespaHIPError  # noqa: E402
+
+
+def contact_sets(m):
+    """Brute force over block pairs: (#link faces, cells touching a link face,
+    coarse cells with a finer neighbour)."""
+    r = m.rank
+    span = np.array([m.span(b) for b in range(m.nblocks)])
+    lo = m.origin[:, :r]
+    hi = lo + m.dims[:, :r].astype(np.int64) * span[:, None]
+    faces = 0
+    touch, reg = set(), set()
+
+    def face_cells(b, d, side, olo, ohi):
+        s = span[b]
+        ranges = []
+        for e in range(r):
+            if e == d:
+                ranges.append([m.dims[b, d] - 1 if side else 0])
+            else:
+                ranges.append(range((olo[e] - lo[b][e]) // s, (ohi[e] - lo[b][e]) // s))
+        off = m.offsets[b]
+        nx, ny = m.dims[b, 0], m.dims[b, 1]
+        for ijk in itertools.product(*ranges):
+            ijk = list(ijk) + [0] * (3 - r)
+            yield int(off + (ijk[2] * ny + ijk[1]) * nx + ijk[0])
+
+    for a in range(m.nblocks):
+        for b in range(a + 1, m.nblocks):
+            ov = np.minimum(hi[a], hi[b]) - np.maximum(lo[a], lo[b])
+            if not ((ov >= 0).all() and (ov == 0).sum() == 1):
+                continue
+            d = int(np.nonzero(ov == 0)[0][0])
+            olo = np.maximum(lo[a], lo[b])
+            ohi = np.minimum(hi[a], hi[b])
+            fine = a if m.level[a] >= m.level[b] else b
+            nf = 1
+            for e in range(r):
+                if e != d:
+                    nf *= (ohi[e] - olo[e]) // span[fine]
+            faces += int(nf)
+            a_hi = hi[a][d] == lo[b][d]   # a on the lo side
+            touch.update(face_cells(a, d, 1 if a_hi else 0, olo, ohi))
+            touch.update(face_cells(b, d, 0 if a_hi else 1, olo, ohi))
+            if m.level[a] != m.level[b]:
+                c = b if fine == a else a
+                c_side = (1 if a_hi else 0) if c == a else (0 if a_hi else 1)
+                reg.update(face_cells(c, d, c_side, olo, ohi))
+    return faces, touch, reg
+
+
+MESHES = {
+    "2d_two_level": lambda: BM.BlockMesh(2, 1, (1 / 16, 1 / 16), 1.4, [0, 1, 1], [[8, 8, 1], [8, 8, 1], [8, 4, 1]],
+                                         [[0, 0, 0], [16, 4, 0], [16, 0, 0]], [[2] * 4 + [1, 1]] * 3),
+    "3d_two_level": lambda: BM.BlockMesh(3, 1, (1 / 8,) * 3, 1.4, [0, 1, 0], [[4, 4, 4], [8, 8, 8], [4, 4, 4]],
+                                         [[0, 0, 0], [8, 0, 0], [16, 0, 0]], [[2] * 6] * 3),
+    "1d_sod": lambda: BM.BlockMesh(1, 1, (1 / 64,), 1.4, [0, 1, 0], [[16, 1, 1], [64, 1, 1], [16, 1, 1]],
+                                   [[0, 0, 0], [32, 0, 0], [96, 0, 0]], [[1] * 6] * 3),
+    "random_40": lambda: BM.random_block_pairs(40, seed=7, rank=3, edges=(8, 16)),
+    "random_2d": lambda: BM.random_block_pairs(30, seed=11, rank=2, edges=(8, 16, 24, 32)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_describe_matches_brute_force_contacts(name):
+    m = MESHES[name]()
+    m.validate()
+    c = BM.describe(m)
+    faces, touch, reg = contact_sets(m)
+    assert c["cells"] == m.ncell
+    assert c["link_faces"] == faces
+    assert c["link_cells"] == len(touch)
+    assert c["register_cells"] == len(reg)
+    assert c["slot_faces"] == 2 * faces          # each face is listed by both of its cells
+    assert c["tiles"] >= m.nblocks
+
+
+def test_describe_tiles_follow_zchunk():
+    m = BM.uniform_as_blocks((32, 24, 40), [(0, 1)] * 3, [(0, 0)] * 3, 1.4)
+    whole = BM.describe(m)["tiles"]
+    # x: 32 (one tile), y: 24 in tiles of 8 -> 3, z whole
+    assert whole == 3
+    assert BM.describe(m, zchunk=8)["tiles"] == 3 * 5
+    assert BM.describe(m, zchunk=16)["tiles"] == 3 * 3    # 40 planes in 3 balanced chunks
+
+
+@pytest.mark.parametrize("bad,match", [
+    ("jump", "more than one level"),
+    ("periodic_contact", "periodic"),
+    ("misaligned", "aligned"),
+])
+def test_describe_rejects_bad_meshes(bad, match):
+    if bad == "jump":
+        m = BM.BlockMesh(2, 2, (0.1, 0.1), 1.4, [0, 2], [[4, 4, 1]] * 2, [[0, 0, 0], [16, 0, 0]], [[1] * 6] * 2)
+    elif bad == "periodic_contact":
+        m = BM.BlockMesh(2, 0, (0.1, 0.1), 1.4, [0, 0], [[4, 4, 1]] * 2, [[0, 0, 0], [4, 0, 0]],
+                         [[0, 0, 1, 1, 1, 1]] * 2)
+    else:
+        m = BM.BlockMesh(2, 1, (0.1, 0.1), 1.4, [0, 1], [[4, 4, 1], [4, 4, 1]], [[0, 0, 0], [8, 1, 0]],
+                         [[1] * 6] * 2)
+    with pytest.raises(VespaHIPError, match=match):
+        BM.describe(m)

@@ -26,6 +26,10 @@ EXPORTED = (
     "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
     "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
     "vh_grid_tile_count", "vh_grid_get_launch", "vh_grid_time_steps",
+    "vh_blocks_describe", "vh_blocks_create", "vh_blocks_destroy", "vh_blocks_ncell",
+    "vh_blocks_set_mode", "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt",
+    "vh_blocks_step", "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch",
+    "vh_blocks_info", "vh_blocks_sync", "vh_blocks_time_steps",
 )
 
 
@@ -92,6 +96,29 @@ def load_library(path: str | None = None):
               "vh_grid_state_device_ptr", "vh_grid_compute_dt", "vh_grid_step", "vh_grid_run",
               "vh_grid_totals", "vh_grid_accumulate_flux", "vh_grid_sync", "vh_grid_set_launch",
               "vh_grid_tile_count", "vh_grid_get_launch", "vh_grid_time_steps"):
+        getattr(L, n).restype = C.c_int
+    L.vh_blocks_describe.argtypes = [C.c_int32, C.c_int32, C.c_int32, dp, i32p, i32p, i64p, i32p, C.c_int32,
+                                     i64p]
+    L.vh_blocks_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_int32, C.c_int32, dp, C.c_double, i32p, i32p,
+                                   i64p, i32p, C.c_int32]
+    L.vh_blocks_destroy.argtypes = [vp]
+    L.vh_blocks_ncell.argtypes = [vp]
+    L.vh_blocks_ncell.restype = C.c_int64
+    L.vh_blocks_set_mode.argtypes = [vp, C.c_int32]
+    L.vh_blocks_upload.argtypes = [vp, vp]
+    L.vh_blocks_download.argtypes = [vp, vp]
+    L.vh_blocks_compute_dt.argtypes = [vp, C.c_double, C.c_int32, dp]
+    L.vh_blocks_step.argtypes = [vp, C.c_double, C.c_int32]
+    L.vh_blocks_run.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_int32, dp, i64p]
+    L.vh_blocks_totals.argtypes = [vp, dp]
+    L.vh_blocks_set_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.vh_blocks_info.argtypes = [vp, i64p, i32p, i32p]
+    L.vh_blocks_sync.argtypes = [vp]
+    L.vh_blocks_time_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, dp, dp]
+    for n in ("vh_blocks_describe", "vh_blocks_create", "vh_blocks_destroy", "vh_blocks_set_mode",
+              "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt", "vh_blocks_step",
+              "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch", "vh_blocks_info",
+              "vh_blocks_sync", "vh_blocks_time_steps"):
         getattr(L, n).restype = C.c_int
     if path is None:
         _lib = L

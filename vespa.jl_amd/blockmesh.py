@@ -210,3 +210,136 @@ def smooth_block_state(mesh: BlockMesh, seed: int, dtype=np.float64):
         E = pp / (T(g) - T(1)) + (T(0.5) * r) * ((a * a + bb * bb) + c * c)
         U[:, off[b]:off[b + 1]] = np.stack([r, r * a, r * bb, r * c, E])
     return U
+
+
+# ---------------------------------------------------------------- device context
+
+def _mesh_args(mesh: BlockMesh):
+    import ctypes as C
+    h0 = (C.c_double * 3)(*mesh.h0)
+    lv = np.ascontiguousarray(mesh.level, dtype=np.int32)
+    dm = np.ascontiguousarray(mesh.dims, dtype=np.int32)
+    og = np.ascontiguousarray(mesh.origin, dtype=np.int64)
+    bc = np.ascontiguousarray(mesh.bc, dtype=np.int32)
+    i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    return (h0, lv.ctypes.data_as(i32p), dm.ctypes.data_as(i32p), og.ctypes.data_as(i64p), bc.ctypes.data_as(i32p),
+            (lv, dm, og, bc))
+
+
+COUNT_NAMES = ("cells", "tiles", "link_faces", "link_cells", "register_cells", "slot_faces")
+
+
+def describe(mesh: BlockMesh, zchunk: int = 0) -> dict:
+    """Validate a mesh with the library's topology builder (no device needed)
+    and return the sizes of its derived tables."""
+    import ctypes as C
+    from ._lib import check, load_library
+    L = load_library()
+    h0, lv, dm, og, bc, keep = _mesh_args(mesh)
+    out = (C.c_int64 * 6)()
+    check(L.vh_blocks_describe(mesh.rank, mesh.nblocks, mesh.maxlevel, h0, lv, dm, og, bc, int(zchunk), out))
+    return dict(zip(COUNT_NAMES, [int(v) for v in out]))
+
+
+class BlockSimulation:
+    """Device-resident simulation on a block mesh (the HGBackend Simulation,
+    src/driver.jl:27-79): one persistent launch per RK stage over every block,
+    link faces between blocks, flux registers for subcycling."""
+
+    def __init__(self, mesh: BlockMesh, eltype=np.float64, cfl: float = 0.4):
+        import ctypes as C
+        from . import _lib
+        from ._lib import check, load_library
+        self._C, self._check = C, check
+        self.mesh = mesh
+        self.eltype = np.dtype(eltype)
+        if self.eltype not in (np.float64, np.float32):
+            raise ValueError(f"eltype must be float64 or float32, got {self.eltype}")
+        self.cfl = float(cfl)
+        self.t = 0.0
+        self.step_count = 0
+        self._lib = load_library()
+        h0, lv, dm, og, bc, keep = _mesh_args(mesh)
+        h = C.c_void_p()
+        check(self._lib.vh_blocks_create(C.byref(h), mesh.rank, mesh.nblocks, mesh.maxlevel, h0,
+                                         float(mesh.gamma), lv, dm, og, bc,
+                                         _lib.VH_F64 if self.eltype == np.float64 else _lib.VH_F32))
+        self._h = h
+        self.ncell = int(self._lib.vh_blocks_ncell(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vh_blocks_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=self.eltype)
+        if U.size != 5 * self.ncell:
+            raise ValueError(f"state must hold 5*{self.ncell} values")
+        self._check(self._lib.vh_blocks_upload(self._h, U.ctypes.data_as(self._C.c_void_p)))
+
+    def state(self) -> np.ndarray:
+        out = np.empty((5, self.ncell), dtype=self.eltype)
+        self._check(self._lib.vh_blocks_download(self._h, out.ctypes.data_as(self._C.c_void_p)))
+        return out
+
+    def set_mode(self, mode: str):
+        from . import _lib
+        self._check(self._lib.vh_blocks_set_mode(self._h, {"exact": _lib.VH_EXACT, "fast": _lib.VH_FAST}[mode]))
+
+    def set_launch(self, zchunk: int = 16, max_workgroups: int = 0, use_graph: bool = True):
+        self._check(self._lib.vh_blocks_set_launch(self._h, int(zchunk), int(max_workgroups), int(bool(use_graph))))
+
+    def info(self) -> dict:
+        C = self._C
+        out = (C.c_int64 * 6)()
+        zc, wg = C.c_int32(), C.c_int32()
+        self._check(self._lib.vh_blocks_info(self._h, out, C.byref(zc), C.byref(wg)))
+        d = dict(zip(COUNT_NAMES, [int(v) for v in out]))
+        d.update(zchunk=zc.value, workgroups=wg.value)
+        return d
+
+    def compute_dt(self, level: int = -1) -> float:
+        dt = self._C.c_double()
+        self._check(self._lib.vh_blocks_compute_dt(self._h, self.cfl, int(level), self._C.byref(dt)))
+        return dt.value
+
+    def step(self, dt: float, level: int = -1):
+        self._check(self._lib.vh_blocks_step(self._h, float(dt), int(level)))
+        if level < 0:
+            self.t += dt
+            self.step_count += 1
+
+    def evolve(self, tfinal: float = 0.0, max_steps: int = 0, subcycle: bool = False):
+        C = self._C
+        t = C.c_double(self.t)
+        k = C.c_int64()
+        self._check(self._lib.vh_blocks_run(self._h, self.cfl, float(tfinal), int(max_steps), int(bool(subcycle)),
+                                            C.byref(t), C.byref(k)))
+        self.t = t.value
+        self.step_count += k.value
+        return self.t, k.value
+
+    def totals(self) -> np.ndarray:
+        out = (self._C.c_double * 5)()
+        self._check(self._lib.vh_blocks_totals(self._h, out))
+        return np.array(list(out))
+
+    def sync(self):
+        self._check(self._lib.vh_blocks_sync(self._h))
+
+    def time_steps(self, nsteps: int, method: str = "graph", stage2: bool = True) -> dict:
+        """Time nsteps single-rate steps: method 'persistent' (eager), 'per_block'
+        (one launch per block per stage) or 'graph' (persistent, HIP graph)."""
+        C = self._C
+        code = {"persistent": 0, "per_block": 1, "graph": 2}[method]
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.vh_blocks_time_steps(self._h, self.cfl, int(nsteps), code, C.byref(a),
+                                                   C.byref(b) if stage2 else None))
+        return {"step_ms": a.value, "stage2_ms": b.value if stage2 else None}

@@ -48,6 +48,73 @@ struct GridDev {
   double area[3];  // face areas
 };
 
+// ---- batched blocks (block-structured AMR) ----
+
+// One block of the mesh: cells per axis, boundary conditions where no other
+// block touches a face, offset of its first cell in the SoA arrays, geometry.
+// Face slot s = 2*axis + side (side 0 = lo, 1 = hi).
+struct BlockDev {
+  int n[3];
+  int bc[6];
+  int level;
+  int haslinks;    // 1 if any cell of the block touches a link face
+  int repoff[6];   // offset of the slot's representative map in repmap, -1 if no block touches it
+  int lcoff[6];    // offset of the slot's link-cell map in lcmap, -1 if no link face is on it
+  long long off;   // first cell of the block in the leaf order
+  double w[3];
+  double area[3];
+  double V;
+};
+
+// A tile of work: block, x/y origin and shape of the tile, z range.
+struct TileDev {
+  int block, x0, y0, z0, z1, tx, ty, pad;
+};
+
+// A face between two different blocks (emitted once: same level from the hi
+// side, coarse-fine from the fine side, with the emitting cell's area). nL/nR:
+// lo and hi representative neighbours of iL and iR along the axis (>= 0: cell;
+// < 0: ghost of the cell itself with boundary code -1-value).
+struct LinkFaceDev {
+  int iL, iR;
+  int nL[2], nR[2];
+  int axis;
+  int pad;
+  double area;
+};
+
+// Gather record of a cell that touches link faces.
+// type[s]: 0 flux computed by the tile (or none), 1 one link face emitted by
+// this cell at its own walk position, 2 a group of link faces emitted by cells
+// of another block. start/count: the slot's faces in slot_faces (walk order).
+// grp[0..ngrp): the type-2 slots in increasing emitting-block order; the first
+// npre come from blocks that precede this one. finer: bit s set when slot s is
+// a group of finer cells (the coarse side of a flux register).
+struct LinkCellDev {
+  int type[6];
+  int start[6], count[6];
+  int grp[6];
+  int npre, ngrp;
+  int finer;
+  int pad;
+};
+
+// Everything a block kernel needs, passed by value.
+struct BlocksDev {
+  const BlockDev* blocks;
+  const TileDev* tiles;
+  const int* repmap;
+  const int* lcmap;
+  const LinkCellDev* linkcells;
+  const int* slot_faces;
+  const LinkFaceDev* faces;
+  const void* fluxbuf;     // 5 values per link face (AoS)
+  long long ncell;
+  double gamma, igm1;
+  int ntiles;
+  int nfaces;
+};
+
 __device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
 __device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }

@@ -35,6 +35,14 @@ inline TileShape tile_for(int rank, int variant) {
                   double* partials);                                                                \
   void launch_face_flux(int dtype, long long nf, hipStream_t s, const void* WL, const void* WR,     \
                         const int32_t* axis, double gamma, void* F);                                \
+  void launch_link_flux(int dtype, int nfaces, hipStream_t s, const void* U, const BlocksDev& B,    \
+                        void* F);                                                                   \
+  void launch_block_stage(int dtype, int rank, bool stage2, int wgs, hipStream_t s, const void* Us, \
+                          const void* U0, void* Uout, const double* dtp, double* partials,          \
+                          const BlocksDev& B);                                                      \
+  void launch_block_cfl(int dtype, int rank, int wgs, hipStream_t s, const void* U,                 \
+                        const BlocksDev& B, const int* blist, const long long* prefix, int nb,      \
+                        double* partials);                                                          \
   }
 
 VH_DECLARE_PROFILE(exact)

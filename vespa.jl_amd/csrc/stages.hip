@@ -3,6 +3,7 @@
 // evaluation order) and VH_FAST_PROFILE=1 with FMA contraction (fast).
 #include "profiles.h"
 #include "stencil.h"
+#include "blocks.h"
 
 namespace vh {
 namespace VH_PROFILE_NS {
@@ -19,9 +20,9 @@ __global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL
   const T* r = WR + 5 * f;
   L.r = l[0]; L.vx = l[1]; L.vy = l[2]; L.vz = l[3]; L.p = l[4];
   R.r = r[0]; R.vx = r[1]; R.vy = r[2]; R.vz = r[3]; R.p = r[4];
-  int d = axis[f];
-  d = d < 1 ? 1 : (d > 3 ? 3 : d);
-  const Cons<T> o = hllc(L, R, g, igm1, d);
+  const int d = axis[f];
+  // constant axis per branch keeps the normal-component selection in registers
+  const Cons<T> o = d <= 1 ? hllc(L, R, g, igm1, 1) : (d == 2 ? hllc(L, R, g, igm1, 2) : hllc(L, R, g, igm1, 3));
   T* out = F + 5 * f;
   out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
 }
@@ -92,6 +93,65 @@ void launch_face_flux(int dtype, long long nf, hipStream_t s, const void* WL, co
   else
     hipLaunchKernelGGL((face_flux_kernel<float>), dim3(blocks), dim3(256), 0, s, (const float*)WL,
                        (const float*)WR, axis, nf, float(gamma), float(igm1), (float*)F);
+}
+
+// ---- batched blocks ----
+
+void launch_link_flux(int dtype, int nfaces, hipStream_t s, const void* U, const BlocksDev& B, void* F) {
+  if (nfaces <= 0) return;
+  const unsigned blocks = (unsigned)((nfaces + 255) / 256);
+  if (dtype == 0)
+    hipLaunchKernelGGL((link_flux_kernel<double, false>), dim3(blocks), dim3(256), 0, s, (const double*)U, B,
+                       (double*)F);
+  else
+    hipLaunchKernelGGL((link_flux_kernel<float, true>), dim3(blocks), dim3(256), 0, s, (const float*)U, B,
+                       (float*)F);
+}
+
+template <typename T, int RANK, bool FLOOR64>
+static void block_stage_t(bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
+                          const double* dtp, double* partials, const BlocksDev& B) {
+  if (stage2)
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, true, FLOOR64>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
+                       (const T*)U0, (T*)Uout, dtp, partials, B);
+  else
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, false, FLOOR64>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
+                       (const T*)U0, (T*)Uout, dtp, partials, B);
+}
+
+void launch_block_stage(int dtype, int rank, bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0,
+                        void* Uout, const double* dtp, double* partials, const BlocksDev& B) {
+  if (wgs <= 0) return;
+  if (dtype == 0) {
+    if (rank == 3) block_stage_t<double, 3, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else if (rank == 2) block_stage_t<double, 2, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else block_stage_t<double, 1, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+  } else {
+    if (rank == 3) block_stage_t<float, 3, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else if (rank == 2) block_stage_t<float, 2, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else block_stage_t<float, 1, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+  }
+}
+
+template <typename T, bool FLOOR64>
+static void block_cfl_t(int rank, int wgs, hipStream_t s, const void* U, const BlocksDev& B, const int* blist,
+                        const long long* prefix, int nb, double* partials) {
+  if (rank == 3)
+    hipLaunchKernelGGL((block_cfl_kernel<T, 3, FLOOR64>), dim3(wgs), dim3(256), 0, s, (const T*)U, B, blist, prefix,
+                       nb, partials);
+  else if (rank == 2)
+    hipLaunchKernelGGL((block_cfl_kernel<T, 2, FLOOR64>), dim3(wgs), dim3(256), 0, s, (const T*)U, B, blist, prefix,
+                       nb, partials);
+  else
+    hipLaunchKernelGGL((block_cfl_kernel<T, 1, FLOOR64>), dim3(wgs), dim3(256), 0, s, (const T*)U, B, blist, prefix,
+                       nb, partials);
+}
+
+void launch_block_cfl(int dtype, int rank, int wgs, hipStream_t s, const void* U, const BlocksDev& B,
+                      const int* blist, const long long* prefix, int nb, double* partials) {
+  if (wgs <= 0 || nb <= 0) return;
+  if (dtype == 0) block_cfl_t<double, false>(rank, wgs, s, U, B, blist, prefix, nb, partials);
+  else block_cfl_t<float, true>(rank, wgs, s, U, B, blist, prefix, nb, partials);
 }
 
 }  // namespace VH_PROFILE_NS

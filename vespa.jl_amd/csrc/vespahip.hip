@@ -16,6 +16,7 @@
 #include <string>
 
 #include "../../include/vespahip.h"
+#include "host.h"
 #include "profiles.h"
 
 namespace vh {
@@ -24,19 +25,12 @@ namespace vh {
 // error plumbing
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
+int fail(int code, const char* fmt, const char* a, long long b) {
   char buf[512];
   snprintf(buf, sizeof buf, fmt, a, b);
   g_err = buf;
   return code;
 }
-
-#define VH_HIP(call)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (call);                                                              \
-    if (e_ != hipSuccess) return fail(-2, "HIP error %s (code %lld) at " #call,          \
-                                      hipGetErrorString(e_), (long long)e_);             \
-  } while (0)
 
 // ----------------------------------------------------------------------------
 // reductions
@@ -90,6 +84,17 @@ __global__ void sub_kernel(const T* __restrict__ a, const T* __restrict__ b, T* 
                            long long n) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] - b[i];
+}
+
+void enqueue_dt_finalize(hipStream_t s, const double* partials, int np, double cfl, double tfinal, int clip,
+                         double* clock) {
+  hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, s, partials, np, cfl, tfinal, clip, clock);
+}
+void enqueue_set_dt(hipStream_t s, double* clock, double dt) {
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, s, clock, dt);
+}
+void enqueue_advance_clock(hipStream_t s, double* clock) {
+  hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, s, clock);
 }
 
 }  // namespace vh
