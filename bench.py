@@ -85,6 +85,65 @@ def cpu_baseline(U, n, steps=2):
                       f"gather form with OpenMP on {threads} host threads, {el:.2f} s wall"}
 
 
+def blocks_leg(V, args):
+    """Batched uneven AMR blocks: `args.blocks` blocks with edges drawn from
+    {8, 16, 24, 32}, glued in random conforming and 2:1 pairs, smooth seeded
+    state, single-rate SSP-RK2 with the device CFL dt. One persistent launch per
+    RK stage replayed from a HIP graph (headline of the leg), the same launches
+    enqueued eagerly, and one launch per block per stage (the comparison loop)."""
+    import numpy as np
+    from vespa_jl_amd import blockmesh as BM
+    m = BM.random_block_pairs(args.blocks, seed=2024, rank=3, edges=(8, 16, 24, 32))
+    log(f"[bench] blocks: {m.nblocks} blocks, {m.ncell} cells; building the state")
+    U = BM.smooth_block_state(m, seed=1)
+    res = {"workload": f"{m.nblocks} blocks (edges from {{8,16,24,32}}, seed 2024), {int((m.level == 1).sum())} "
+                       f"level-1 blocks in 2:1 pairs, conforming pairs, smooth state rho,p in [0.5,2], "
+                       f"|v|<=0.3, gamma=5/3, CFL 0.3 device dt, fast profile",
+           "cells": m.ncell}
+    for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
+        s = BM.BlockSimulation(m, dtype, cfl=0.3)
+        s.set_mode(args.mode)
+        s.upload(U.astype(dtype))
+        info = s.info()
+        esz = np.dtype(dtype).itemsize
+        r = {"tables": {k: info[k] for k in ("tiles", "link_faces", "register_cells")},
+             "zchunk": info["zchunk"], "workgroups": info["workgroups"]}
+        for method, nst in (("graph", args.block_steps), ("persistent", args.block_steps),
+                            ("per_block", max(2, args.block_steps // 4))):
+            if key == "f32" and method != "graph":
+                continue
+            t = s.time_steps(nst, method=method, stage2=(method == "graph"))
+            r[method] = {"ms_per_step": t["step_ms"], "zone_updates_per_s": m.ncell / (t["step_ms"] * 1e-3),
+                         "steps": nst}
+            if method == "graph":
+                r["stage2_ms"] = t["stage2_ms"]
+                r["stage2_GBps"] = 15 * esz * m.ncell / (t["stage2_ms"] * 1e-3) / 1e9
+            log(f"[bench] blocks {key} {method}: {t['step_ms']:.3f} ms/step, "
+                f"{m.ncell / (t['step_ms'] * 1e-3) / 1e9:.2f} G zone-updates/s")
+        if key == "f64":
+            r["speedup_graph_vs_per_block"] = r["per_block"]["ms_per_step"] / r["graph"]["ms_per_step"]
+        s.close()
+        res[key] = r
+    if not args.no_cpu_baseline:
+        if ROOT not in sys.path:
+            sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        comps = BM.components(m)
+        sel = [b for g in comps[:300] for b in g]
+        sub, cells = BM.submesh(m, sel)
+        Us = np.ascontiguousarray(U[:, cells])
+        bo = O.BlockOracle(sub)
+        t0 = time.perf_counter()
+        bo.step(Us, bo.compute_dt(Us, 0.3))
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": sub.ncell / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"1 SSP-RK2 step (CFL dt) of the first 300 independent units "
+                                         f"({len(sel)} blocks, {sub.ncell} cells) of the same block table, "
+                                         f"serial block oracle, {el:.2f} s"}
+        log(f"[bench] blocks CPU {sub.ncell / el / 1e6:.2f} M zone-updates/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +159,9 @@ def main():
     ap.add_argument("--no-other-mode", action="store_true",
                     help="skip the FP64 run in the other arithmetic profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-blocks", action="store_true", help="skip the batched-blocks leg")
+    ap.add_argument("--blocks", type=int, default=3000, help="blocks of the batched-blocks leg")
+    ap.add_argument("--block-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also time 3 device-to-device copies of 1 GiB (counter calibration)")
     args = ap.parse_args()
@@ -232,6 +294,9 @@ def main():
         log(f"[bench] CPU {out['cpu_baseline']['value'] / 1e6:.2f} M zone-updates/s")
     else:
         out["cpu_baseline"] = None
+
+    if not args.no_blocks:
+        out["batched_blocks"] = blocks_leg(V, args)
 
     print(json.dumps(out), flush=True)
 

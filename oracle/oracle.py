@@ -82,6 +82,12 @@ def lib():
         L.vo_bface_count.argtypes = [gp]
         L.vo_bface_count.restype = C.c_int64
         L.vo_star_state.argtypes = [dp, dp, C.c_double, dp, dp]
+        L.vo_accumulate_vjp_f64.argtypes = [gp, dp, dp, dp]
+        L.vo_accumulate_vjp_f64.restype = C.c_int
+        L.vo_step_vjp_f64.argtypes = [gp, dp, C.c_double, dp, dp]
+        L.vo_step_vjp_f64.restype = C.c_int
+        L.vo_c2p_vjp_f64.argtypes = [dp, C.c_double, dp, dp]
+        L.vo_hllc_vjp_f64.argtypes = [dp, dp, C.c_double, C.c_int, dp, dp, dp]
         L.vo_exact_riemann_sample.argtypes = [dp, dp, C.c_double, C.c_double, dp]
         _lib = L
     return _lib
@@ -195,6 +201,43 @@ def step(g: Grid, U: np.ndarray, dt: float, floor64=False, bflux=None) -> None:
                                                    int(floor64), bp)
     if rc != 0:
         raise ValueError("invalid grid")
+
+
+# ---------------- reverse mode (Float64, dt fixed) ----------------
+
+def accumulate_vjp(g: Grid, U: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """A'(U)^T lam for A = accumulate_flux."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    out = np.zeros_like(U)
+    if lib().vo_accumulate_vjp_f64(C.byref(g), _ptr(U), _ptr(lam), _ptr(out)) != 0:
+        raise ValueError("invalid grid")
+    return out
+
+
+def step_vjp(g: Grid, U: np.ndarray, dt: float, wbar: np.ndarray) -> np.ndarray:
+    """dL/dU for L depending on step(U, dt) through wbar = dL/d(step(U, dt))."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    wbar = np.ascontiguousarray(wbar, dtype=np.float64)
+    out = np.zeros_like(U)
+    if lib().vo_step_vjp_f64(C.byref(g), _ptr(U), float(dt), _ptr(wbar), _ptr(out)) != 0:
+        raise ValueError("invalid grid")
+    return out
+
+
+def cons2prim_vjp(U, gamma, Wbar):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Wbar = np.ascontiguousarray(Wbar, dtype=np.float64)
+    out = np.zeros(5)
+    lib().vo_c2p_vjp_f64(_ptr(U), float(gamma), _ptr(Wbar), _ptr(out))
+    return out
+
+
+def hllc_vjp(WL, WR, gamma, axis, Fbar):
+    WL, WR, Fbar = (np.ascontiguousarray(a, dtype=np.float64) for a in (WL, WR, Fbar))
+    a, b = np.zeros(5), np.zeros(5)
+    lib().vo_hllc_vjp_f64(_ptr(WL), _ptr(WR), float(gamma), int(axis), _ptr(Fbar), _ptr(a), _ptr(b))
+    return a, b
 
 
 def accumulate_flux_gather(g: Grid, U: np.ndarray, floor64=False) -> np.ndarray:

@@ -239,3 +239,39 @@ def test_bad_mesh_raises_before_touching_the_device():
     m = BM.BlockMesh(2, 2, (0.1, 0.1), 1.4, [0, 2], [[4, 4, 1]] * 2, [[0, 0, 0], [16, 0, 0]], [[1] * 6] * 2)
     with pytest.raises(V.VespaHIPError, match="more than one level"):
         BM.BlockSimulation(m)
+
+
+def test_c3_full_size_sampled_units_bitwise(O):
+    """The 3,000-block workload at full size (37 M cells): the device CFL dt of
+    the whole table equals the oracle's, and after two steps every sampled
+    independent unit (blocks connected by faces) equals the oracle run on that
+    unit alone with the same dt, bit for bit."""
+    m = BM.random_block_pairs(3000, seed=2024, rank=3, edges=(8, 16, 24, 32))
+    print(f"C3 full size: {m.ncell} cells", flush=True)
+    U0 = BM.smooth_block_state(m, seed=1)
+    s = _sim(m, np.float64)
+    s.upload(U0)
+    dt0 = s.compute_dt()
+    assert dt0 == O.BlockOracle(m).compute_dt(U0, 0.3)
+    dts = []
+    for _ in range(2):
+        dt = s.compute_dt()
+        s.step(dt)
+        dts.append(dt)
+    G = s.state()
+    s.close()
+    comps = BM.components(m)
+    rng = np.random.default_rng(0)
+    pairs = [g for g in comps if len(g) == 2 and m.level[g].max() == 1]
+    conf = [g for g in comps if len(g) == 2 and m.level[g].max() == 0]
+    single = [g for g in comps if len(g) == 1]
+    picks = [pairs[i] for i in rng.choice(len(pairs), 12, replace=False)]
+    picks += [conf[i] for i in rng.choice(len(conf), 6, replace=False)]
+    picks += [single[i] for i in rng.choice(len(single), 4, replace=False)]
+    for g in picks:
+        sub, cells = BM.submesh(m, g)
+        bo = O.BlockOracle(sub)
+        Ug = np.ascontiguousarray(U0[:, cells])
+        for dt in dts:
+            bo.step(Ug, dt)
+        np.testing.assert_array_equal(G[:, cells], Ug, err_msg=f"unit {g}")

@@ -119,6 +119,47 @@ class BlockMesh:
                             raise ValueError(f"fine block {f} is not aligned to the cells of block {c}")
 
 
+def components(mesh: BlockMesh) -> list:
+    """Groups of blocks connected by face contact (each sorted by block index).
+    Blocks of different groups never exchange fluxes, so a group evolves the same
+    inside the whole mesh and on its own."""
+    nb, r = mesh.nblocks, mesh.rank
+    span = np.array([mesh.span(b) for b in range(nb)], dtype=np.int64)
+    lo = mesh.origin[:, :r]
+    hi = lo + mesh.dims[:, :r].astype(np.int64) * span[:, None]
+    parent = list(range(nb))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for a in range(nb):
+        ov = np.minimum(hi[a], hi) - np.maximum(lo[a], lo)          # (nb, r)
+        touch = (ov >= 0).all(axis=1) & ((ov == 0).sum(axis=1) == 1)
+        touch[a] = False
+        for b in np.nonzero(touch)[0]:
+            ra, rb = find(a), find(int(b))
+            if ra != rb:
+                parent[max(ra, rb)] = min(ra, rb)
+    groups = {}
+    for b in range(nb):
+        groups.setdefault(find(b), []).append(b)
+    return [sorted(g) for g in groups.values()]
+
+
+def submesh(mesh: BlockMesh, blocks) -> tuple:
+    """The mesh made of the listed blocks (in the given order) and, for each of
+    its cells, the index of the same cell in the parent's leaf order."""
+    blocks = list(blocks)
+    sub = BlockMesh(mesh.rank, mesh.maxlevel, mesh.h0[:mesh.rank], mesh.gamma, mesh.level[blocks],
+                    mesh.dims[blocks], mesh.origin[blocks], mesh.bc[blocks], mesh.x0)
+    off = mesh.offsets
+    cells = np.concatenate([np.arange(off[b], off[b + 1]) for b in blocks])
+    return sub, cells
+
+
 def uniform_as_blocks(dims, domain, bcs, gamma) -> BlockMesh:
     """One block covering a UniformMesh (the RefMesh case)."""
     rank = len(dims)
