@@ -158,6 +158,38 @@ int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1
                        double* ms_step);
 
 /* ---------------------------------------------------------------------------
+ * Reverse mode (Float64 uniform grids). The reference has no derivative of its
+ * step (no AD anywhere in the tree); these entry points are what a
+ * ChainRulesCore rrule over step!(sim, dt) (src/driver.jl:333-359) would call.
+ * dt is held fixed; non-smooth points take the active branch (minmod's
+ * selected argument or zero, the active Davis min/max argument, the upwind
+ * HLLC branch, the pressure floor when it is the larger argument).
+ */
+
+/* dUbar = (∂ step(U, dt)/∂U)ᵀ dWbar for device arrays of 5*ncell doubles (the
+ * grid's layout). dUbar may alias dWbar. Uses the grid's arithmetic profile for
+ * the recomputed stage 1. Synchronises. */
+int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, void* dUbar);
+
+/* Checkpointed rollout of nsteps steps with the device CFL dt: the forward
+ * keeps the state every `every` steps and each step's dt; the backward
+ * recomputes each segment from its checkpoint and applies the step VJPs in
+ * reverse. */
+typedef struct vh_adjoint vh_adjoint;
+int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t every);
+int vh_adjoint_destroy(vh_adjoint* a);
+/* nsteps steps from the grid's current state (left at the final state). */
+int vh_adjoint_forward(vh_adjoint* a, double cfl);
+/* dUbar (device) = gradient w.r.t. the initial state of a loss whose gradient
+ * w.r.t. the final state is dWbar (device). */
+int vh_adjoint_backward(vh_adjoint* a, const void* dWbar, void* dUbar);
+/* Event times of the last forward/backward (ms), device bytes held by the
+ * rollout and the grid's reverse-mode work buffers, and (optional) the dt of
+ * every step (nsteps doubles). */
+int vh_adjoint_info(const vh_adjoint* a, double* ms_forward, double* ms_backward, int64_t* device_bytes,
+                    double* dts_host);
+
+/* ---------------------------------------------------------------------------
  * Tier 2 — batched blocks (block-structured AMR with hanging nodes).
  * A mesh is a list of blocks: block b has a refinement level level[b] (0 =
  * base, at most maxlevel), cells per axis dims[3*b + d] and an integer origin

@@ -1,0 +1,71 @@
+"""Host logic of the dynamic block AMR driver (no device): tagging, refinement
+with 2:1 balance, conservative remap, and the block mesh it hands to the
+library's topology builder."""
+import numpy as np
+
+from conftest import load_vespa
+
+V = load_vespa()
+from vespa_jl_amd import amr, blockmesh as BM  # noqa: E402
+
+
+def _bump(qb, width):
+    x, y = qb.cell_centers()
+    U = amr.sedov_quadblocks_state(qb)
+    U[0] = 1.0 + 0.5 * np.exp(-(x * x + y * y) / width)
+    return U
+
+
+def test_refine_balance_and_conservative_remap():
+    qb = amr.QuadBlocks(6, 8, 3)
+    U = _bump(qb, 0.01)
+    tot0 = (U * qb.volumes()).sum(axis=1)
+    for _ in range(4):
+        old = list(qb.blocks)
+        qb.refine(qb.tag(U[0], 0.02))
+        U = qb.remap(old, U)
+        # injection keeps every conserved total (to round-off of the sums)
+        np.testing.assert_allclose((U * qb.volumes()).sum(axis=1), tot0, rtol=1e-14, atol=1e-16)
+        m = qb.mesh()
+        m.validate()                       # overlap / 2:1 / alignment
+        c = BM.describe(m)
+        assert c["cells"] == len(qb.blocks) * 64
+    levels = sorted({b[0] for b in qb.blocks})
+    assert levels == [0, 1, 2, 3]
+    # the blocks tile the square exactly
+    area = sum(qb.h(l) ** 2 * 64 for (l, _, _) in qb.blocks)
+    assert abs(area - 1.0) < 1e-12
+
+
+def test_mesh_is_four_fold_symmetric():
+    qb = amr.QuadBlocks(6, 8, 3)
+    U = _bump(qb, 0.02)
+    for _ in range(3):
+        old = list(qb.blocks)
+        qb.refine(qb.tag(U[0], 0.02))
+        U = qb.remap(old, U)
+    n = {}
+    for (l, ix, iy) in qb.blocks:
+        nl = 6 << l
+        n[(l, ix, iy)] = 1
+        for img in ((l, nl - 1 - ix, iy), (l, ix, nl - 1 - iy), (l, iy, ix)):
+            assert img in set(qb.blocks), img
+
+
+def test_sedov_initial_state_deposits_the_energy():
+    qb = amr.QuadBlocks(6, 8, 0)
+    U = amr.sedov_quadblocks_state(qb)
+    vol = qb.volumes()
+    x, y = qb.cell_centers()
+    hot = np.hypot(x, y) < 0.02
+    assert hot.sum() == 4                                    # base 48^2: the four central cells
+    E = (U[4] * vol).sum()
+    assert abs(E - (1.0 + 1e-5 / 0.4 * (1.0 - vol[hot].sum()))) < 1e-12
+
+
+def test_shock_radius_of_a_ring():
+    N = 384
+    c = -0.5 + (np.arange(N) + 0.5) / N
+    X, Y = np.meshgrid(c, c)
+    rho = 1.0 + np.exp(-((np.hypot(X, Y) - 0.2) / 0.01) ** 2)
+    assert abs(amr.shock_radius(rho) - 0.2) < 2.0 / N

@@ -30,6 +30,8 @@ EXPORTED = (
     "vh_blocks_set_mode", "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt",
     "vh_blocks_step", "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch",
     "vh_blocks_info", "vh_blocks_sync", "vh_blocks_time_steps",
+    "vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
+    "vh_adjoint_backward", "vh_adjoint_info",
 )
 
 
@@ -119,6 +121,15 @@ def load_library(path: str | None = None):
               "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt", "vh_blocks_step",
               "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch", "vh_blocks_info",
               "vh_blocks_sync", "vh_blocks_time_steps"):
+        getattr(L, n).restype = C.c_int
+    L.vh_grid_step_vjp.argtypes = [vp, vp, C.c_double, vp, vp]
+    L.vh_adjoint_create.argtypes = [C.POINTER(vp), vp, C.c_int64, C.c_int32]
+    L.vh_adjoint_destroy.argtypes = [vp]
+    L.vh_adjoint_forward.argtypes = [vp, C.c_double]
+    L.vh_adjoint_backward.argtypes = [vp, vp, vp]
+    L.vh_adjoint_info.argtypes = [vp, dp, dp, i64p, dp]
+    for n in ("vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
+              "vh_adjoint_backward", "vh_adjoint_info"):
         getattr(L, n).restype = C.c_int
     if path is None:
         _lib = L

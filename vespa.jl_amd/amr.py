@@ -1,0 +1,279 @@
+"""Dynamic block AMR driver on top of the batched block path (2D quadtree of
+fixed-size blocks).
+
+The reference refines leaves with a density-gradient indicator every `every`
+steps and once before the first step, refine-only, with a conservative remap
+(src/driver.jl:392-462; HGBackend refine + remap, lib/HGBackend/src/HGBackend.jl;
+injection / volume mean, lib/RefMesh/src/RefMesh.jl:250-302). Here the unit of
+refinement is a block of B×B cells: a block is refined (split into four
+children of B×B cells at the next level) when any of its cells has
+``max |Δρ| / max(ρ_c, ρ_n)`` over its face neighbours above `refine_above`
+(density_gradient_indicator, src/driver.jl:400-413), then 2:1 balance is
+restored across block faces. The remap injects each parent cell's conserved
+values into its 2×2 children, so Σ U·V is unchanged. Between regrids the leaves
+are advanced on the GPU by the batched kernel (one persistent launch per RK
+stage).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .blockmesh import BlockMesh, BlockSimulation, REF
+
+
+class QuadBlocks:
+    """Blocks of B×B cells on the square [x0, x0 + L]², level 0 = nroot×nroot
+    root blocks; a level-l block (l ≤ maxlevel) has index (ix, iy) in units of its
+    own size."""
+
+    def __init__(self, nroot: int, B: int, maxlevel: int, x0: float = -0.5, L: float = 1.0, bc: int = REF,
+                 gamma: float = 1.4):
+        self.nroot, self.B, self.maxlevel = int(nroot), int(B), int(maxlevel)
+        self.x0, self.L, self.bc, self.gamma = float(x0), float(L), int(bc), float(gamma)
+        self.blocks = [(0, ix, iy) for iy in range(nroot) for ix in range(nroot)]
+
+    # -- geometry
+    def h(self, level: int) -> float:
+        return self.L / (self.nroot * self.B * (1 << level))
+
+    def mesh(self) -> BlockMesh:
+        ml = self.maxlevel
+        lv = [b[0] for b in self.blocks]
+        dims = [[self.B, self.B, 1]] * len(self.blocks)
+        org = []
+        bc = []
+        for (l, ix, iy) in self.blocks:
+            span = self.B * (1 << (ml - l))          # lattice units per block edge
+            org.append([ix * span, iy * span, 0])
+            n_l = self.nroot << l                     # blocks per axis at this level
+            bc.append([self.bc, self.bc, self.bc, self.bc, 1, 1])
+            assert 0 <= ix < n_l and 0 <= iy < n_l
+        h0 = self.L / (self.nroot * self.B)
+        return BlockMesh(2, ml, (h0, h0), self.gamma, lv, dims, org, bc, x0=(self.x0, self.x0))
+
+    def cell_centers(self):
+        """(x, y) of every cell in leaf order (block by block, x fastest)."""
+        xs, ys = [], []
+        for (l, ix, iy) in self.blocks:
+            h = self.h(l)
+            i = np.arange(self.B)
+            X, Y = np.meshgrid(self.x0 + (ix * self.B + i + 0.5) * h, self.x0 + (iy * self.B + i + 0.5) * h)
+            xs.append(X.ravel())
+            ys.append(Y.ravel())
+        return np.concatenate(xs), np.concatenate(ys)
+
+    def volumes(self):
+        return np.concatenate([np.full(self.B * self.B, self.h(l) ** 2) for (l, _, _) in self.blocks])
+
+    # -- rasterisation on the finest uniform grid (injection)
+    def raster(self, Uf: np.ndarray) -> np.ndarray:
+        """Fields (nf, ncell) → (nf, N, N) on the finest grid, N = nroot·B·2^maxlevel."""
+        N = self.nroot * self.B << self.maxlevel
+        out = np.empty((Uf.shape[0], N, N), dtype=Uf.dtype)
+        B2 = self.B * self.B
+        for q, (l, ix, iy) in enumerate(self.blocks):
+            r = 1 << (self.maxlevel - l)
+            blk = Uf[:, q * B2:(q + 1) * B2].reshape(-1, self.B, self.B)
+            fine = np.repeat(np.repeat(blk, r, axis=1), r, axis=2)
+            y0, x0 = iy * self.B * r, ix * self.B * r
+            out[:, y0:y0 + self.B * r, x0:x0 + self.B * r] = fine
+        return out
+
+    # -- indicator / tagging
+    def tag(self, rho: np.ndarray, refine_above: float) -> set:
+        """Blocks below maxlevel with a cell whose density jump to a face
+        neighbour exceeds refine_above (jumps measured on the finest raster)."""
+        R = self.raster(rho[None])[0]
+        g = np.zeros_like(R)
+        for ax in (0, 1):
+            for sh in (1, -1):
+                Rn = np.roll(R, sh, axis=ax)
+                valid = np.ones_like(R, dtype=bool)     # no wrap across the domain edge
+                if ax == 0:
+                    if sh == 1:
+                        valid[0, :] = False
+                    else:
+                        valid[-1, :] = False
+                else:
+                    if sh == 1:
+                        valid[:, 0] = False
+                    else:
+                        valid[:, -1] = False
+                jump = np.abs(Rn - R) / np.maximum(np.maximum(R, Rn), np.finfo(float).eps)
+                g = np.maximum(g, np.where(valid, jump, 0.0))
+        tags = set()
+        for (l, ix, iy) in self.blocks:
+            if l >= self.maxlevel:
+                continue
+            r = 1 << (self.maxlevel - l)
+            y0, x0 = iy * self.B * r, ix * self.B * r
+            if g[y0:y0 + self.B * r, x0:x0 + self.B * r].max() > refine_above:
+                tags.add((l, ix, iy))
+        return tags
+
+    def _leaf_at(self, leafset: dict, level: int, fx: int, fy: int):
+        """Leaf block containing finest-lattice block position (fx, fy) (in units
+        of finest blocks), searched from `level` down to 0."""
+        for l in range(self.maxlevel, -1, -1):
+            s = 1 << (self.maxlevel - l)
+            key = (l, fx // s, fy // s)
+            if key in leafset:
+                return key
+        return None
+
+    def refine(self, tags: set) -> int:
+        """Refine the tagged blocks, then restore 2:1 balance across faces.
+        Returns the number of blocks split."""
+        leaves = {b: True for b in self.blocks}
+        todo = set(tags)
+        nsplit = 0
+        while todo:
+            nxt = set()
+            for b in sorted(todo):
+                if b not in leaves:
+                    continue
+                l, ix, iy = b
+                del leaves[b]
+                nsplit += 1
+                for cy in (0, 1):
+                    for cx in (0, 1):
+                        leaves[(l + 1, 2 * ix + cx, 2 * iy + cy)] = True
+            # 2:1 balance: a leaf whose face neighbour is more than one level finer must refine
+            for b in list(leaves):
+                l, ix, iy = b
+                if l >= self.maxlevel:
+                    continue
+                s = 1 << (self.maxlevel - l)                # finest blocks per edge of b
+                fx0, fy0 = ix * s, iy * s
+                nmax = self.nroot << self.maxlevel
+                need = False
+                for side in range(4):
+                    for t in range(s):
+                        if side == 0:
+                            px, py = fx0 - 1, fy0 + t
+                        elif side == 1:
+                            px, py = fx0 + s, fy0 + t
+                        elif side == 2:
+                            px, py = fx0 + t, fy0 - 1
+                        else:
+                            px, py = fx0 + t, fy0 + s
+                        if not (0 <= px < nmax and 0 <= py < nmax):
+                            continue
+                        nb = self._leaf_at(leaves, l, px, py)
+                        if nb is not None and nb[0] > l + 1:
+                            need = True
+                            break
+                    if need:
+                        break
+                if need:
+                    nxt.add(b)
+            todo = nxt
+        # order: by level, then row-major, for a deterministic layout
+        self.blocks = sorted(leaves, key=lambda b: (b[0], b[2], b[1]))
+        return nsplit
+
+    def remap(self, old_blocks: list, U_old: np.ndarray) -> np.ndarray:
+        """Conservative remap (injection) of the fields from old_blocks to the
+        current blocks: unchanged blocks are copied, a refined block's cells
+        inject their values into their 2×2 (or 4×4, ...) descendants."""
+        B2 = self.B * self.B
+        where = {b: q for q, b in enumerate(old_blocks)}
+        out = np.empty((U_old.shape[0], len(self.blocks) * B2), dtype=U_old.dtype)
+        for q, b in enumerate(self.blocks):
+            l, ix, iy = b
+            if b in where:
+                p = where[b]
+                out[:, q * B2:(q + 1) * B2] = U_old[:, p * B2:(p + 1) * B2]
+                continue
+            # find the old ancestor
+            for up in range(1, l + 1):
+                a = (l - up, ix >> up, iy >> up)
+                if a in where:
+                    break
+            else:
+                raise ValueError(f"block {b} has no ancestor in the old mesh")
+            p = where[a]
+            r = 1 << up
+            parent = U_old[:, p * B2:(p + 1) * B2].reshape(-1, self.B, self.B)
+            fine = np.repeat(np.repeat(parent, r, axis=1), r, axis=2)       # (nf, B·r, B·r)
+            oy = (iy - (a[2] << up)) * self.B
+            ox = (ix - (a[1] << up)) * self.B
+            out[:, q * B2:(q + 1) * B2] = fine[:, oy:oy + self.B, ox:ox + self.B].reshape(-1, B2)
+        return out
+
+
+def sedov_quadblocks_state(qb: QuadBlocks, E0: float = 1.0, rho0: float = 1.0, p0: float = 1e-5,
+                           r_hot: float = 0.02, dtype=np.float64) -> np.ndarray:
+    """Sedov–Taylor initial state on the current blocks: ρ = rho0, v = 0,
+    p = p0 outside, and the energy E0 deposited uniformly (as pressure) in the
+    cells whose centre lies within r_hot of the origin."""
+    x, y = qb.cell_centers()
+    vol = qb.volumes()
+    hot = np.hypot(x, y) < r_hot
+    if not hot.any():
+        raise ValueError("no cell centre within r_hot")
+    e_hot = E0 / vol[hot].sum()                      # energy density in the hot cells
+    g = qb.gamma
+    E = np.where(hot, e_hot, p0 / (g - 1.0))
+    n = x.size
+    U = np.stack([np.full(n, rho0), np.zeros(n), np.zeros(n), np.zeros(n), E])
+    return U.astype(dtype)
+
+
+def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.05, cfl: float = 0.4,
+              refine_above: float = 0.1, every: int = 8, dtype=np.float64, mode: str = "exact",
+              snapshots=(), max_steps: int = 100000):
+    """Sedov blast on [-0.5, 0.5]² with base (nroot·B)² cells and up to maxlevel
+    levels of refinement, reflecting walls. Regrid before the first step and
+    every `every` steps. Returns a dict with the final blocks/state, step count,
+    leaf counts and the raster density at the requested snapshot times."""
+    qb = QuadBlocks(nroot, B, maxlevel)
+    U = sedov_quadblocks_state(qb, dtype=np.float64).astype(dtype)
+    vol0 = qb.volumes()
+    totals0 = (U.astype(np.float64) * vol0).sum(axis=1)
+    t, steps = 0.0, 0
+    snaps = {}
+    pending = sorted(snapshots)
+    leaves_hist = []
+    while t < tfinal * (1 - 1e-12) and steps < max_steps:
+        old = list(qb.blocks)
+        tags = qb.tag(U[0].astype(np.float64), refine_above)
+        if tags:
+            qb.refine(tags)
+            U = qb.remap(old, U)
+        sim = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
+        sim.set_mode(mode)
+        sim.upload(U)
+        sim.t = t
+        stop = tfinal
+        if pending and pending[0] < tfinal:
+            stop = pending[0]
+        tt, k = sim.evolve(tfinal=stop, max_steps=every)
+        t = tt
+        steps += k
+        U = sim.state()
+        sim.close()
+        leaves_hist.append(len(qb.blocks) * B * B)
+        while pending and t >= pending[0] * (1 - 1e-12):
+            snaps[pending.pop(0)] = (t, qb.raster(U[:1].astype(np.float64))[0])
+    vol = qb.volumes()
+    totals = (U.astype(np.float64) * vol).sum(axis=1)
+    return {"blocks": list(qb.blocks), "qb": qb, "U": U, "t": t, "steps": steps, "leaves": len(qb.blocks) * B * B,
+            "max_leaves": max(leaves_hist) if leaves_hist else 0, "totals0": totals0, "totals": totals,
+            "snapshots": snaps}
+
+
+def shock_radius(rho_raster: np.ndarray, x0: float = -0.5, L: float = 1.0) -> float:
+    """Radius of the density maximum, averaged over angle (radial bins of one
+    raster cell)."""
+    N = rho_raster.shape[0]
+    h = L / N
+    c = x0 + (np.arange(N) + 0.5) * h
+    X, Y = np.meshgrid(c, c)
+    r = np.hypot(X, Y)
+    bins = (r / h).astype(int)
+    sums = np.bincount(bins.ravel(), weights=rho_raster.ravel())
+    cnt = np.bincount(bins.ravel())
+    prof = sums / np.maximum(cnt, 1)
+    k = int(np.argmax(prof[: N // 2]))
+    return (k + 0.5) * h

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "common.h"
+
 namespace vh {
 
 // Records a formatted message for vh_last_error() and returns code.
@@ -25,5 +27,19 @@ void enqueue_dt_finalize(hipStream_t s, const double* partials, int np, double c
                          double* clock);
 void enqueue_set_dt(hipStream_t s, double* clock, double dt);
 void enqueue_advance_clock(hipStream_t s, double* clock);
+
+}  // namespace vh
+
+namespace vh {
+
+// Reverse-mode launchers (adjoint.hip), Float64 uniform grids. Face buffers
+// gL/gR per axis hold 5 SoA fields over adj_face_slots(G, axis) face slots.
+long long adj_face_slots(const GridDev& G, int axis);
+void adj_launch_prim(hipStream_t s, const double* U, double* W, const GridDev& G);
+void adj_launch_faces(hipStream_t s, const double* W, const double* lam, const GridDev& G, double* const gL[3],
+                      double* const gR[3]);
+void adj_launch_cells(hipStream_t s, const double* U, const double* W, double* const gL[3], double* const gR[3],
+                      const double* base1, double a1, const double* base2, double a2, double beta,
+                      const double* dtp, const GridDev& G, double* out);
 
 }  // namespace vh

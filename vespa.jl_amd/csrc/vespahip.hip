@@ -10,6 +10,7 @@
 //   totals_kernel      per-workgroup Σ U·V partial sums (src/diagnostics.jl:16-31)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -124,6 +125,7 @@ struct vh_grid {
   double* clock;
   double* totals_partials;
   hipStream_t stream;
+  struct AdjWork* adj;   // reverse-mode work buffers (allocated on first use)
 };
 
 static const int kMaxCflBlocks = 2048;
@@ -389,9 +391,12 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
   return 0;
 }
 
+static void free_adj_work(vh_grid* g);
+
 int vh_grid_destroy(vh_grid* g) {
   if (!g) return 0;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  free_adj_work(g);
   if (g->U) (void)hipFree(g->U);
   if (g->U1) (void)hipFree(g->U1);
   if (g->partials) (void)hipFree(g->partials);
@@ -603,6 +608,252 @@ int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1
   // First stage-1 start to last stage-2 end, per step (includes the dt reduction
   // and clock kernels between steps).
   if (ms_step) *ms_step = (double)tot / nsteps;
+  return 0;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------
+// reverse mode (Float64): one step's VJP and a checkpointed rollout
+
+namespace vh {
+__global__ void set_value_kernel(double* p, double v) { *p = v; }
+__global__ void copy_value_kernel(const double* src, double* dst) { *dst = *src; }
+}  // namespace vh
+
+struct AdjWork {
+  double* W = nullptr;
+  double* gL[3] = {nullptr, nullptr, nullptr};
+  double* gR[3] = {nullptr, nullptr, nullptr};
+  double* lam1 = nullptr;
+  double* U1 = nullptr;
+  double* dt = nullptr;
+  size_t bytes = 0;
+};
+
+static void free_adj_work(vh_grid* g) {
+  if (!g->adj) return;
+  AdjWork* w = g->adj;
+  (void)hipFree(w->W);
+  for (int d = 0; d < 3; ++d) {
+    (void)hipFree(w->gL[d]);
+    (void)hipFree(w->gR[d]);
+  }
+  (void)hipFree(w->lam1);
+  (void)hipFree(w->U1);
+  (void)hipFree(w->dt);
+  delete w;
+  g->adj = nullptr;
+}
+
+static int ensure_adj_work(vh_grid* g) {
+  if (g->adj) return 0;
+  if (g->dtype != VH_F64) return fail(-1, "the reverse mode is Float64 only%s", "");
+  AdjWork* w = new AdjWork();
+  g->adj = w;
+  const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
+  VH_HIP(hipMalloc(&w->W, nb));
+  VH_HIP(hipMalloc(&w->lam1, nb));
+  VH_HIP(hipMalloc(&w->U1, nb));
+  VH_HIP(hipMalloc(&w->dt, sizeof(double)));
+  w->bytes = 3 * nb + sizeof(double);
+  for (int d = 0; d < g->G.rank; ++d) {
+    const size_t fb = sizeof(double) * 5 * (size_t)adj_face_slots(g->G, d);
+    VH_HIP(hipMalloc(&w->gL[d], fb));
+    VH_HIP(hipMalloc(&w->gR[d], fb));
+    w->bytes += 2 * fb;
+  }
+  return 0;
+}
+
+// λ ← a1·base1 + a2·base2 + beta·(dt/V)·A'(U)ᵀλ_in, all device arrays.
+static void enqueue_jat(vh_grid* g, const double* U, const double* lam_in, const double* base1, double a1,
+                        const double* base2, double a2, double beta, const double* dtp, double* out) {
+  AdjWork* w = g->adj;
+  adj_launch_prim(g->stream, U, w->W, g->G);
+  adj_launch_faces(g->stream, w->W, lam_in, g->G, w->gL, w->gR);
+  adj_launch_cells(g->stream, U, w->W, w->gL, w->gR, base1, a1, base2, a2, beta, dtp, g->G, out);
+}
+
+// VJP of one step from state U (stage-1 state U1 given, or recomputed into the
+// work buffer when null): wbar → out (out may alias wbar).
+static void enqueue_step_vjp(vh_grid* g, const double* U, const double* U1, const double* dtp, const double* wbar,
+                             double* out) {
+  AdjWork* w = g->adj;
+  if (!U1) {
+    if (g->mode == VH_FAST)
+      fast::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, w->U1, dtp,
+                         g->partials, g->G);
+    else
+      exact::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, U, U, w->U1, dtp,
+                          g->partials, g->G);
+    U1 = w->U1;
+  }
+  // λ1 = ½w − ½·invV·A'(U1)ᵀw
+  enqueue_jat(g, U1, wbar, wbar, 0.5, nullptr, 0.0, -0.5, dtp, w->lam1);
+  // ∂L/∂U = ½w + λ1 − invV·A'(U)ᵀλ1
+  enqueue_jat(g, U, w->lam1, wbar, 0.5, w->lam1, 1.0, -1.0, dtp, out);
+}
+
+struct vh_adjoint {
+  vh_grid* g = nullptr;
+  int64_t nsteps = 0;
+  int every = 20;
+  int nseg = 0;
+  double* dts = nullptr;
+  double* ckpt = nullptr;
+  double* seg = nullptr;
+  double* seg1 = nullptr;
+  double* w = nullptr;
+  size_t bytes = 0;
+  float fwd_ms = 0, bwd_ms = 0;
+  bool have_forward = false;
+};
+
+extern "C" {
+
+int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, void* dUbar) {
+  if (!g || !dU || !dWbar || !dUbar) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  int rc = ensure_adj_work(g);
+  if (rc) return rc;
+  hipLaunchKernelGGL(set_value_kernel, dim3(1), dim3(1), 0, g->stream, g->adj->dt, dt);
+  enqueue_step_vjp(g, (const double*)dU, nullptr, g->adj->dt, (const double*)dWbar, (double*)dUbar);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_adjoint_destroy(vh_adjoint* a) {
+  if (!a) return 0;
+  if (a->g && a->g->stream) (void)hipStreamSynchronize(a->g->stream);
+  (void)hipFree(a->dts);
+  (void)hipFree(a->ckpt);
+  (void)hipFree(a->seg);
+  (void)hipFree(a->seg1);
+  (void)hipFree(a->w);
+  delete a;
+  return 0;
+}
+
+int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t every) {
+  if (!out || !g) return fail(-1, "null argument%s", "");
+  *out = nullptr;
+  if (nsteps < 1 || every < 1) return fail(-1, "need nsteps >= 1 and every >= 1%s", "");
+  if (g->dtype != VH_F64) return fail(-1, "the reverse mode is Float64 only%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  int rc = ensure_adj_work(g);
+  if (rc) return rc;
+  vh_adjoint* a = new vh_adjoint();
+  a->g = g;
+  a->nsteps = nsteps;
+  a->every = every > nsteps ? (int)nsteps : every;
+  a->nseg = (int)((nsteps + a->every - 1) / a->every);
+  const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
+  hipError_t e = hipMalloc(&a->dts, sizeof(double) * nsteps);
+  if (e == hipSuccess) e = hipMalloc(&a->ckpt, nb * a->nseg);
+  if (e == hipSuccess) e = hipMalloc(&a->seg, nb * a->every);
+  if (e == hipSuccess) e = hipMalloc(&a->seg1, nb * a->every);
+  if (e == hipSuccess) e = hipMalloc(&a->w, nb);
+  if (e != hipSuccess) {
+    vh_adjoint_destroy(a);
+    return fail(-2, "device allocation for the rollout failed: %s", hipGetErrorString(e));
+  }
+  a->bytes = sizeof(double) * nsteps + nb * (a->nseg + 2 * (size_t)a->every + 1) + g->adj->bytes;
+  *out = a;
+  return 0;
+}
+
+int vh_adjoint_forward(vh_adjoint* a, double cfl) {
+  if (!a) return fail(-1, "null argument%s", "");
+  vh_grid* g = a->g;
+  VH_HIP(hipSetDevice(g->device));
+  const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
+  hipEvent_t e0, e1;
+  VH_HIP(hipEventCreate(&e0));
+  VH_HIP(hipEventCreate(&e1));
+  VH_HIP(hipEventRecord(e0, g->stream));
+  VH_HIP(hipMemcpyAsync(a->ckpt, g->U, nb, hipMemcpyDeviceToDevice, g->stream));
+  enqueue_cfl(g);
+  int np = g->cfl_blocks;
+  for (int64_t k = 0; k < a->nsteps; ++k) {
+    hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, 0.0, 0, g->clock);
+    hipLaunchKernelGGL(copy_value_kernel, dim3(1), dim3(1), 0, g->stream, g->clock + 1, a->dts + k);
+    enqueue_step(g);
+    np = g->stage_blocks;
+    if ((k + 1) % a->every == 0 && k + 1 < a->nsteps)
+      VH_HIP(hipMemcpyAsync((char*)a->ckpt + nb * ((k + 1) / a->every), g->U, nb, hipMemcpyDeviceToDevice, g->stream));
+  }
+  VH_HIP(hipEventRecord(e1, g->stream));
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipEventSynchronize(e1));
+  VH_HIP(hipEventElapsedTime(&a->fwd_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  a->have_forward = true;
+  return 0;
+}
+
+int vh_adjoint_backward(vh_adjoint* a, const void* dWbar, void* dUbar) {
+  if (!a || !dWbar || !dUbar) return fail(-1, "null argument%s", "");
+  if (!a->have_forward) return fail(-1, "run vh_adjoint_forward first%s", "");
+  vh_grid* g = a->g;
+  VH_HIP(hipSetDevice(g->device));
+  const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
+  const size_t n5 = 5 * (size_t)g->G.ncell;
+  hipEvent_t e0, e1;
+  VH_HIP(hipEventCreate(&e0));
+  VH_HIP(hipEventCreate(&e1));
+  VH_HIP(hipEventRecord(e0, g->stream));
+  VH_HIP(hipMemcpyAsync(a->w, dWbar, nb, hipMemcpyDeviceToDevice, g->stream));
+  for (int s = a->nseg - 1; s >= 0; --s) {
+    const int64_t k0 = (int64_t)s * a->every;
+    const int64_t k1 = std::min<int64_t>(a->nsteps, k0 + a->every);
+    VH_HIP(hipMemcpyAsync(a->seg, (char*)a->ckpt + nb * s, nb, hipMemcpyDeviceToDevice, g->stream));
+    // recompute the segment's states U_k and stage-1 states U1_k
+    for (int64_t k = k0; k < k1; ++k) {
+      const double* Uk = a->seg + n5 * (k - k0);
+      double* U1k = a->seg1 + n5 * (k - k0);
+      const double* dtp = a->dts + k;
+      if (g->mode == VH_FAST)
+        fast::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, Uk, Uk, U1k, dtp,
+                           g->partials, g->G);
+      else
+        exact::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, Uk, Uk, U1k, dtp,
+                            g->partials, g->G);
+      if (k + 1 < k1) {
+        double* Un = a->seg + n5 * (k + 1 - k0);
+        if (g->mode == VH_FAST)
+          fast::launch_stage(VH_F64, g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1k, Uk, Un, dtp,
+                             g->partials, g->G);
+        else
+          exact::launch_stage(VH_F64, g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1k, Uk, Un, dtp,
+                              g->partials, g->G);
+      }
+    }
+    for (int64_t k = k1 - 1; k >= k0; --k)
+      enqueue_step_vjp(g, a->seg + n5 * (k - k0), a->seg1 + n5 * (k - k0), a->dts + k, a->w, a->w);
+  }
+  VH_HIP(hipMemcpyAsync(dUbar, a->w, nb, hipMemcpyDeviceToDevice, g->stream));
+  VH_HIP(hipEventRecord(e1, g->stream));
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipEventSynchronize(e1));
+  VH_HIP(hipEventElapsedTime(&a->bwd_ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int vh_adjoint_info(const vh_adjoint* a, double* ms_forward, double* ms_backward, int64_t* device_bytes,
+                    double* dts_host) {
+  if (!a) return fail(-1, "null argument%s", "");
+  if (ms_forward) *ms_forward = a->fwd_ms;
+  if (ms_backward) *ms_backward = a->bwd_ms;
+  if (device_bytes) *device_bytes = (int64_t)a->bytes;
+  if (dts_host) {
+    if (!a->have_forward) return fail(-1, "run vh_adjoint_forward first%s", "");
+    VH_HIP(hipMemcpy(dts_host, a->dts, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost));
+  }
   return 0;
 }
 
