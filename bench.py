@@ -144,6 +144,139 @@ def blocks_leg(V, args):
     return res
 
 
+def adjoint_leg(V, args):
+    """Kelvin-Helmholtz 512^2 periodic FP64: 200 steps (device CFL dt), loss
+    L = sum over cells of (rho_final - rho_target)^2 with the target the final
+    density of the same run from a second seeded state; gradient w.r.t. the
+    initial conserved fields by the hand-written adjoint with checkpoints every
+    20 steps. Reports the backward/forward time ratio, device memory, central
+    differences on random directions (eps = 1e-6) and the GPU gradient against
+    the oracle's chained VJPs on a 128^2 version of the same rollout."""
+    import numpy as np
+    from vespa_jl_amd import adjoint as AD
+    from vespa_jl_amd import problems as P
+    n, nst = args.adjoint_n, args.adjoint_steps
+
+    def sim_for(U):
+        prob = V.Problem("kh", (n, n), ((0.0, 1.0),) * 2, 1.4, "periodic",
+                         lambda x, y, z: (1.0, 0.0, 0.0, 0.0, 1.0), 1.0, 0.4)
+        s = V.Simulation(V.UniformMesh((n, n), prob.domain), prob)
+        s.upload(U)
+        return s
+
+    U0 = np.ascontiguousarray(P.kh_state(n, seed=7))
+    st = sim_for(np.ascontiguousarray(P.kh_state(n, seed=8)))
+    V.run_steps(st, nst)
+    target = st.state()[0]
+    st.close()
+    sim = sim_for(U0)
+    # plain forward timing (same steps, device dt, no checkpoints)
+    sim.sync()
+    t0 = time.perf_counter()
+    V.run_steps(sim, nst)
+    sim.sync()
+    plain_ms = 1e3 * (time.perf_counter() - t0)
+    sim.upload(U0)
+    sim.t = 0.0
+    ro = AD.Rollout(sim, nst, every=20)
+    ro.forward()                      # warm
+    sim.upload(U0)
+    UF = ro.forward()
+    w = np.zeros_like(UF)
+    w[0] = 2.0 * (UF[0] - target)
+    ro.backward(w)                    # warm
+    G = ro.backward(w)
+    info = ro.info()
+    dts = info["dts"]
+    loss0 = float(np.sum((UF[0] - target) ** 2))
+
+    def loss_of(U):
+        sim.upload(U)
+        for dt in dts:
+            V.step(sim, float(dt))
+        return float(np.sum((sim.state()[0] - target) ** 2))
+
+    rng = np.random.default_rng(2024)
+    fd = []
+    for _ in range(args.fd_directions):
+        v = rng.normal(size=U0.shape)
+        v /= np.linalg.norm(v)
+        ad = float(np.sum(G * v))
+        f = (loss_of(U0 + 1e-6 * v) - loss_of(U0 - 1e-6 * v)) / 2e-6
+        fd.append({"adjoint": ad, "fd": f, "rel": abs(ad - f) / max(abs(f), 1e-300)})
+    ro.close()
+    sim.close()
+    res = {"workload": f"KH {n}^2 periodic, gamma=1.4, {nst} steps CFL 0.4, checkpoints every 20, FP64",
+           "forward_ms": info["forward_ms"], "forward_plain_ms": plain_ms, "backward_ms": info["backward_ms"],
+           "ratio_backward_over_forward": info["backward_ms"] / info["forward_ms"],
+           "device_bytes_rollout": info["device_bytes"], "loss": loss0,
+           "fd_eps": 1e-6, "fd_max_rel": max(d["rel"] for d in fd), "fd_median_rel": float(np.median([d["rel"] for d in fd])),
+           "fd": fd}
+    log(f"[bench] adjoint: fwd {info['forward_ms']:.2f} ms, bwd {info['backward_ms']:.2f} ms, "
+        f"ratio {res['ratio_backward_over_forward']:.2f}, FD max rel {res['fd_max_rel']:.2e}")
+    # GPU gradient vs the oracle's chained VJPs (same schedule) at 128^2
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    m = 128
+    Us = np.ascontiguousarray(P.kh_state(m, seed=7))
+    prob = V.Problem("kh", (m, m), ((0.0, 1.0),) * 2, 1.4, "periodic",
+                     lambda x, y, z: (1.0, 0.0, 0.0, 0.0, 1.0), 1.0, 0.4)
+    s2 = V.Simulation(V.UniformMesh((m, m), prob.domain), prob)
+    s2.upload(Us)
+    r2 = AD.Rollout(s2, nst, every=20)
+    UF2 = r2.forward()
+    w2 = np.zeros_like(UF2)
+    w2[0] = 2.0 * (UF2[0] - np.ascontiguousarray(P.kh_state(m, seed=8))[0])
+    G2 = r2.backward(w2)
+    d2 = r2.info()["dts"]
+    r2.close()
+    s2.close()
+    g = O.make_grid((m, m), [(0.0, 1.0)] * 2, "periodic", 1.4)
+    states, U = [], Us.copy()
+    t0 = time.perf_counter()
+    for dt in d2:
+        states.append(U.copy())
+        O.step(g, U, float(dt))
+    wr = w2.copy()
+    for k in reversed(range(nst)):
+        wr = O.step_vjp(g, states[k], float(d2[k]), wr)
+    el = time.perf_counter() - t0
+    res["vs_oracle_gradient"] = {"grid": [m, m], "steps": nst,
+                                 "forward_bitwise": bool(np.array_equal(U, UF2)),
+                                 "max_rel_diff": float(np.abs(G2 - wr).max() / np.abs(wr).max()),
+                                 "oracle_seconds": el}
+    log(f"[bench] adjoint vs oracle at {m}^2: {res['vs_oracle_gradient']['max_rel_diff']:.2e}")
+    return res
+
+
+def sedov_leg(V, args):
+    """2D Sedov blast with dynamic block AMR (base 48^2 on [-0.5,0.5]^2, 8x8-cell
+    blocks, up to 3 levels, p_amb 1e-5, E=1 in r<0.02), to t=0.05, FP64 and FP32."""
+    import numpy as np
+    from vespa_jl_amd import amr
+    out = {}
+    for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
+        t0 = time.perf_counter()
+        r = amr.run_sedov(tfinal=0.05, dtype=dtype, snapshots=(0.0125, 0.025, 0.05), mode=args.mode)
+        el = time.perf_counter() - t0
+        rho = r["snapshots"][0.05][1]
+        sc = rho.max()
+        radii = {str(t): amr.shock_radius(v[1]) for t, v in r["snapshots"].items()}
+        ts = np.array(sorted(r["snapshots"]))
+        Rs = np.array([amr.shock_radius(r["snapshots"][t][1]) for t in ts])
+        expo = float(np.polyfit(np.log(ts), np.log(Rs), 1)[0])
+        out[key] = {"steps": r["steps"], "wall_s": el, "final_leaves": r["leaves"], "max_leaves": r["max_leaves"],
+                    "mass_rel_change": float(abs(r["totals"][0] / r["totals0"][0] - 1)),
+                    "energy_rel_change": float(abs(r["totals"][4] / r["totals0"][4] - 1)),
+                    "symmetry_max_rel": float(max(np.abs(rho - rho[::-1, :]).max(), np.abs(rho - rho[:, ::-1]).max(),
+                                                  np.abs(rho - rho.T).max()) / sc),
+                    "shock_radius": radii, "radius_exponent": expo}
+        log(f"[bench] sedov {key}: {r['steps']} steps, leaves {r['leaves']}, R ~ t^{expo:.3f}, "
+            f"mass {out[key]['mass_rel_change']:.1e}, symmetry {out[key]['symmetry_max_rel']:.1e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +293,11 @@ def main():
                     help="skip the FP64 run in the other arithmetic profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-blocks", action="store_true", help="skip the batched-blocks leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint rollout leg")
+    ap.add_argument("--no-sedov", action="store_true", help="skip the dynamic-AMR Sedov leg")
+    ap.add_argument("--adjoint-n", type=int, default=512)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--blocks", type=int, default=3000, help="blocks of the batched-blocks leg")
     ap.add_argument("--block-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
@@ -297,6 +435,10 @@ def main():
 
     if not args.no_blocks:
         out["batched_blocks"] = blocks_leg(V, args)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(V, args)
+    if not args.no_sedov:
+        out["sedov_amr"] = sedov_leg(V, args)
 
     print(json.dumps(out), flush=True)
 

@@ -80,9 +80,12 @@ class QuadBlocks:
         return out
 
     # -- indicator / tagging
-    def tag(self, rho: np.ndarray, refine_above: float) -> set:
+    def tag(self, rho: np.ndarray, refine_above: float, symmetric: bool = False) -> set:
         """Blocks below maxlevel with a cell whose density jump to a face
-        neighbour exceeds refine_above (jumps measured on the finest raster)."""
+        neighbour exceeds refine_above (jumps measured on the finest raster).
+        symmetric: also tag the mirror images of every tagged block (x, y and
+        diagonal mirrors of the square), so that round-off differences between
+        symmetric cells cannot make the mesh asymmetric."""
         R = self.raster(rho[None])[0]
         g = np.zeros_like(R)
         for ax in (0, 1):
@@ -109,6 +112,14 @@ class QuadBlocks:
             y0, x0 = iy * self.B * r, ix * self.B * r
             if g[y0:y0 + self.B * r, x0:x0 + self.B * r].max() > refine_above:
                 tags.add((l, ix, iy))
+        if symmetric:
+            leaves = set(self.blocks)
+            for (l, ix, iy) in list(tags):
+                nl = self.nroot << l
+                for img in ((l, nl - 1 - ix, iy), (l, ix, nl - 1 - iy), (l, nl - 1 - ix, nl - 1 - iy),
+                            (l, iy, ix), (l, nl - 1 - iy, ix), (l, iy, nl - 1 - ix), (l, nl - 1 - iy, nl - 1 - ix)):
+                    if img in leaves:
+                        tags.add(img)
         return tags
 
     def _leaf_at(self, leafset: dict, level: int, fx: int, fy: int):
@@ -222,7 +233,7 @@ def sedov_quadblocks_state(qb: QuadBlocks, E0: float = 1.0, rho0: float = 1.0, p
 
 def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.05, cfl: float = 0.4,
               refine_above: float = 0.1, every: int = 8, dtype=np.float64, mode: str = "exact",
-              snapshots=(), max_steps: int = 100000):
+              snapshots=(), max_steps: int = 100000, symmetric: bool = True):
     """Sedov blast on [-0.5, 0.5]² with base (nroot·B)² cells and up to maxlevel
     levels of refinement, reflecting walls. Regrid before the first step and
     every `every` steps. Returns a dict with the final blocks/state, step count,
@@ -237,7 +248,7 @@ def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.0
     leaves_hist = []
     while t < tfinal * (1 - 1e-12) and steps < max_steps:
         old = list(qb.blocks)
-        tags = qb.tag(U[0].astype(np.float64), refine_above)
+        tags = qb.tag(U[0].astype(np.float64), refine_above, symmetric=symmetric)
         if tags:
             qb.refine(tags)
             U = qb.remap(old, U)
@@ -277,3 +288,21 @@ def shock_radius(rho_raster: np.ndarray, x0: float = -0.5, L: float = 1.0) -> fl
     prof = sums / np.maximum(cnt, 1)
     k = int(np.argmax(prof[: N // 2]))
     return (k + 0.5) * h
+
+
+def shock_radius_mean(qb: QuadBlocks, rho: np.ndarray) -> float:
+    """Area-weighted mean radius of the cells whose density lies above the
+    midpoint between the background (1) and the peak (the estimator of the
+    reference's Sedov test, test/test_sedov.jl:11-27)."""
+    x, y = qb.cell_centers()
+    w = qb.volumes()
+    thresh = 0.5 * (1.0 + rho.max())
+    sel = rho >= thresh
+    return float((np.hypot(x, y)[sel] * w[sel]).sum() / w[sel].sum()) if sel.any() else 0.0
+
+
+def quadrant_masses(qb: QuadBlocks, rho: np.ndarray) -> np.ndarray:
+    """Mass in each quadrant about the centre (test/test_sedov.jl:58-69)."""
+    x, y = qb.cell_centers()
+    q = (x >= 0).astype(int) + 2 * (y >= 0).astype(int)
+    return np.bincount(q, weights=rho * qb.volumes(), minlength=4)
