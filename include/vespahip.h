@@ -171,6 +171,12 @@ int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1
  * the recomputed stage 1. Synchronises. */
 int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, void* dUbar);
 
+/* Kernel form of A'(U)ᵀλ inside the reverse mode: fused = 1 (default) uses one
+ * tiled kernel per evaluation on 2D grids periodic on both axes with sizes that
+ * are multiples of 32×8, fused = 0 always uses the three-pass form (any rank and
+ * boundary). Both give the same values to round-off. */
+int vh_grid_set_adjoint_form(vh_grid* g, int32_t fused);
+
 /* Checkpointed rollout of nsteps steps with the device CFL dt: the forward
  * keeps the state every `every` steps and each step's dt; the backward
  * recomputes each segment from its checkpoint and applies the step VJPs in
@@ -251,6 +257,9 @@ int vh_blocks_totals(vh_blocks* m, double* out5);
  * workgroups (0 = 3 per compute unit), and whether the single-rate step is
  * replayed from a captured HIP graph (1, the default) or enqueued eagerly. */
 int vh_blocks_set_launch(vh_blocks* m, int32_t zchunk, int32_t max_workgroups, int32_t use_graph);
+/* Widest tile along x (4..32, default 32); tiles take min(nx, max_tile_x)
+ * columns and as many rows as fit 256 lanes. */
+int vh_blocks_set_tiling(vh_blocks* m, int32_t max_tile_x);
 int vh_blocks_info(const vh_blocks* m, int64_t* counts6, int32_t* zchunk, int32_t* workgroups);
 int vh_blocks_sync(vh_blocks* m);
 

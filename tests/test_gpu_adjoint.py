@@ -136,3 +136,24 @@ def test_float32_grid_is_rejected():
     with pytest.raises(ValueError, match="Float64"):
         AD.Rollout(sim, 4)
     sim.close()
+
+
+@pytest.mark.parametrize("dims", [(64, 32), (32, 16)])
+def test_fused_tile_form_matches_three_pass_form_and_oracle(O, dims):
+    """The single-kernel 2D periodic form against the three-pass form and the
+    oracle (both forms differ from the oracle's scatter only in summation order)."""
+    print(f"fused jat {dims}", flush=True)
+    shape = (5,) + tuple(reversed(dims))
+    U = random_state(shape[1:], seed=11, vmax=0.4)
+    g = O.make_grid(dims, [(0.0, 1.0)] * 2, "periodic", 1.4)
+    dt = 0.5 * O.compute_dt(g, U, 0.4)
+    w = np.random.default_rng(12).normal(size=shape)
+    ref = O.step_vjp(g, U, dt, w)
+    sim = _sim(dims, "periodic", U)
+    AD.set_form(sim, True)
+    fused = AD.step_vjp(sim, U, dt, w)
+    AD.set_form(sim, False)
+    three = AD.step_vjp(sim, U, dt, w)
+    assert _rel(fused, ref) < 1e-12
+    assert _rel(three, ref) < 1e-12
+    sim.close()

@@ -33,6 +33,14 @@ def _require_f64(sim):
         raise ValueError("the reverse mode is Float64 only")
 
 
+def set_form(sim, fused: bool) -> None:
+    """Select the kernel form of the reverse mode for this simulation: the
+    single tiled kernel (2D, periodic, sizes multiple of 32×8) or the
+    three-pass form."""
+    _require_f64(sim)
+    check(load_library().vh_grid_set_adjoint_form(sim._h, int(bool(fused))))
+
+
 def step_vjp(sim, U: np.ndarray, dt: float, wbar: np.ndarray) -> np.ndarray:
     """(∂ step(U, dt)/∂U)ᵀ wbar on the GPU for host arrays shaped like sim.state()."""
     _require_f64(sim)

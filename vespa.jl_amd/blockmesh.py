@@ -337,6 +337,9 @@ class BlockSimulation:
     def set_launch(self, zchunk: int = 16, max_workgroups: int = 0, use_graph: bool = True):
         self._check(self._lib.vh_blocks_set_launch(self._h, int(zchunk), int(max_workgroups), int(bool(use_graph))))
 
+    def set_tiling(self, max_tile_x: int = 32):
+        self._check(self._lib.vh_blocks_set_tiling(self._h, int(max_tile_x)))
+
     def info(self) -> dict:
         C = self._C
         out = (C.c_int64 * 6)()

@@ -26,6 +26,18 @@ namespace VH_PROFILE_NS {
 
 using D5 = Prim<double>;
 
+// Division and the sound speed: correctly rounded in the exact profile, refined
+// hardware reciprocal / rsqrt in the fast profile (the reverse mode needs no bit
+// parity; it is tested to 1e-12 against a scatter-form CPU restatement).
+__device__ __forceinline__ double adiv(double a, double b) {
+#if VH_FAST_PROFILE
+  return fast_div(a, b);
+#else
+  return a / b;
+#endif
+}
+__device__ __forceinline__ double asound(double g, double p, double r) { return sound(g, p, r); }
+
 template <int A>
 __device__ __forceinline__ double& ncomp(D5& w) {
   if constexpr (A == 0) return w.vx;
@@ -59,7 +71,7 @@ __device__ __forceinline__ D5 ghostT(const D5& b, int bc) {
 // cons2prim adjoint at U: Wb → Ub (src/eos.jl:8-25).
 __device__ __forceinline__ Cons<double> c2p_vjp(const Cons<double>& U, double g, const D5& Wb) {
   const double rho = U.d;
-  const double inv = 1.0 / rho;
+  const double inv = recip(rho);
   const double vx = U.mx * inv, vy = U.my * inv, vz = U.mz * inv;
   const double v2 = (vx * vx + vy * vy) + vz * vz;
   const double kin = (0.5 * rho) * v2;
@@ -85,7 +97,7 @@ __device__ __forceinline__ Cons<double> c2p_vjp(const Cons<double>& U, double g,
 }
 
 __device__ __forceinline__ double energy64(const D5& W, double g) {
-  return W.p / (g - 1.0) + (0.5 * W.r) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
+  return adiv(W.p, g - 1.0) + (0.5 * W.r) * ((W.vx * W.vx + W.vy * W.vy) + W.vz * W.vz);
 }
 
 __device__ __forceinline__ void energy_vjp(const D5& W, double g, double Eb, D5& Wb) {
@@ -93,7 +105,7 @@ __device__ __forceinline__ void energy_vjp(const D5& W, double g, double Eb, D5&
   Wb.vx += Eb * W.r * W.vx;
   Wb.vy += Eb * W.r * W.vy;
   Wb.vz += Eb * W.r * W.vz;
-  Wb.p += Eb / (g - 1.0);
+  Wb.p += adiv(Eb, g - 1.0);
 }
 
 // euler_flux adjoint (src/riemann.jl:9-20) along axis A (0-based).
@@ -123,8 +135,8 @@ template <int A>
 __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double>& ob, D5& WLb, D5& WRb) {
   const double rL = WL.r, rR = WR.r, pL = WL.p, pR = WR.p;
   const double unL = ncomp<A>(WL), unR = ncomp<A>(WR);
-  const double xL = g * pL / rL, xR = g * pR / rR;
-  const double cL = vsqrt(xL), cR = vsqrt(xR);
+  const double cL = asound(g, pL, rL), cR = asound(g, pR, rR);
+  const double xL = cL * cL, xR = cR * cR;
   const double aL = unL - cL, aR = unR - cR, bL = unL + cL, bR = unR + cR;
   const double SL = vmin(aL, aR), SR = vmax(bL, bR);
   if (SL >= 0.0) {
@@ -138,7 +150,7 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   const double tL = rL * unL, tR = rR * unR;
   const double num = pR - pL + tL * (SL - unL) - tR * (SR - unR);
   const double den = rL * (SL - unL) - rR * (SR - unR);
-  const double Ss = num / den;
+  const double Ss = adiv(num, den);
   const bool useL = Ss >= 0.0;
   const D5 W = useL ? WL : WR;
   D5 Wb = d5_zero();
@@ -147,10 +159,10 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   const double E = energy64(W, g);
   const double q = r * (SK - un);
   const double d3 = SK - Ss;
-  const double factor = q / d3;
+  const double factor = adiv(q, d3);
   const double vs0 = A == 0 ? Ss : W.vx, vs1 = A == 1 ? Ss : W.vy, vs2 = A == 2 ? Ss : W.vz;
-  const double Aa = Ss - un, Bb = Ss + p / q;
-  const double Es = E / r + Aa * Bb;
+  const double Aa = Ss - un, Bb = Ss + adiv(p, q);
+  const double Es = adiv(E, r) + Aa * Bb;
   // out = F(W) + SK·(Us − Uk), Us = factor·(1, vs, Es), Uk = (r, r·v, E)
   const double SKb0 = ob.d * (factor - r) + ob.mx * (factor * vs0 - r * W.vx) + ob.my * (factor * vs1 - r * W.vy) +
                       ob.mz * (factor * vs2 - r * W.vz) + ob.e * (factor * Es - E);
@@ -169,20 +181,23 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   Wb.vz -= u3 * r;
   Eb -= u4;
   // Es = E/r + Aa·Bb
-  Eb += Esb / r;
-  rb -= Esb * E / (r * r);
+  const double rinv = recip(r);
+  Eb += Esb * rinv;
+  rb -= Esb * E * rinv * rinv;
   const double Ab = Esb * Bb, Bbb = Esb * Aa;
   Ssb += Ab;
   unb -= Ab;
   Ssb += Bbb;
-  pb += Bbb / q;
-  qb -= Bbb * p / (q * q);
+  const double qinv = recip(q);
+  pb += Bbb * qinv;
+  qb -= Bbb * p * qinv * qinv;
   if (A == 0) { Ssb += vsb0; } else { Wb.vx += vsb0; }
   if (A == 1) { Ssb += vsb1; } else { Wb.vy += vsb1; }
   if (A == 2) { Ssb += vsb2; } else { Wb.vz += vsb2; }
   // factor = q/(SK − Ss)
-  qb += factorb / d3;
-  const double d3b = -factorb * factor / d3;
+  const double d3inv = recip(d3);
+  qb += factorb * d3inv;
+  const double d3b = -factorb * factor * d3inv;
   SKb += d3b;
   Ssb -= d3b;
   // q = r·(SK − un)
@@ -197,7 +212,8 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   else d5_add(WRb, Wb);
   double SLb = useL ? SKb : 0.0, SRb = useL ? 0.0 : SKb;
   // Ss = num/den
-  const double numb = Ssb / den, denb = -Ssb * Ss / den;
+  const double deninv = recip(den);
+  const double numb = Ssb * deninv, denb = -Ssb * Ss * deninv;
   double rLb = 0, rRb = 0, pLb = 0, pRb = 0, unLb = 0, unRb = 0, cLb = 0, cRb = 0;
   pRb += numb;
   pLb -= numb;
@@ -219,11 +235,12 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   unRb += denb * rR;
   if (aL <= aR) { unLb += SLb; cLb -= SLb; } else { unRb += SLb; cRb -= SLb; }
   if (bL >= bR) { unLb += SRb; cLb += SRb; } else { unRb += SRb; cRb += SRb; }
-  const double xLb = cLb / (2.0 * cL), xRb = cRb / (2.0 * cR);
-  pLb += xLb * g / rL;
-  rLb -= xLb * xL / rL;
-  pRb += xRb * g / rR;
-  rRb -= xRb * xR / rR;
+  const double xLb = adiv(cLb, 2.0 * cL), xRb = adiv(cRb, 2.0 * cR);
+  const double rLi = recip(rL), rRi = recip(rR);
+  pLb += xLb * g * rLi;
+  rLb -= xLb * xL * rLi;
+  pRb += xRb * g * rRi;
+  rRb -= xRb * xR * rRi;
   WLb.r += rLb;
   WLb.p += pLb;
   ncomp<A>(WLb) += unLb;
@@ -480,6 +497,215 @@ __global__ void __launch_bounds__(256) adj_cell_kernel(const double* __restrict_
     double v = a1 * base1[f * n + c];
     if (base2) v += a2 * base2[f * n + c];
     out[f * n + c] = v + s * ub[f];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused A'(U)ᵀλ for 2D grids periodic on both axes: one workgroup per 32×8 tile
+// computes W for the tile and a cross-shaped halo of three cells, every face
+// adjoint its cells need (the tile's faces plus one more on each side), and the
+// gather, in LDS; one launch instead of four, no face buffers in HBM.
+constexpr int JTX = 32, JTY = 8;
+
+struct JatLDS {
+  double Wx[5][JTY][JTX + 6];   // W, rows y0..y0+7, cols x0-3..x0+34
+  double Wy[5][6][JTX];         // W, rows y0-3..y0-1 and y0+8..y0+10, cols x0..x0+31
+  double Lx[5][JTY][JTX + 4];   // λ, rows y0..y0+7, cols x0-2..x0+33
+  double Ly[5][4][JTX];         // λ, rows y0-2, y0-1, y0+8, y0+9
+  double G[10][(JTY + 3) * JTX];  // face-state adjoints (left 0..4, right 5..9) of the current axis
+};
+
+__device__ __forceinline__ D5 jat_get(const double (*a)[JTY][JTX + 6], int r, int c) {
+  D5 w;
+  w.r = a[0][r][c]; w.vx = a[1][r][c]; w.vy = a[2][r][c]; w.vz = a[3][r][c]; w.p = a[4][r][c];
+  return w;
+}
+// W at tile row lr ∈ [−3, JTY+2] of tile column c ∈ [0, JTX).
+__device__ __forceinline__ D5 jat_wy(const JatLDS& S, int lr, int c) {
+  if (lr >= 0 && lr < JTY) return jat_get(S.Wx, lr, c + 3);
+  const int i = lr < 0 ? lr + 3 : 3 + lr - JTY;
+  D5 w;
+  w.r = S.Wy[0][i][c]; w.vx = S.Wy[1][i][c]; w.vy = S.Wy[2][i][c]; w.vz = S.Wy[3][i][c]; w.p = S.Wy[4][i][c];
+  return w;
+}
+__device__ __forceinline__ Cons<double> jat_lx(const JatLDS& S, int r, int lc) {
+  Cons<double> l;
+  l.d = S.Lx[0][r][lc]; l.mx = S.Lx[1][r][lc]; l.my = S.Lx[2][r][lc]; l.mz = S.Lx[3][r][lc]; l.e = S.Lx[4][r][lc];
+  return l;
+}
+// λ at tile row lr ∈ [−2, JTY+1] of tile column c.
+__device__ __forceinline__ Cons<double> jat_ly(const JatLDS& S, int lr, int c) {
+  if (lr >= 0 && lr < JTY) return jat_lx(S, lr, c + 2);
+  const int i = lr < 0 ? lr + 2 : 2 + lr - JTY;
+  Cons<double> l;
+  l.d = S.Ly[0][i][c]; l.mx = S.Ly[1][i][c]; l.my = S.Ly[2][i][c]; l.mz = S.Ly[3][i][c]; l.e = S.Ly[4][i][c];
+  return l;
+}
+__device__ __forceinline__ D5 jat_g(const JatLDS& S, int side, int i) {
+  const int o = 5 * side;
+  D5 w;
+  w.r = S.G[o][i]; w.vx = S.G[o + 1][i]; w.vy = S.G[o + 2][i]; w.vz = S.G[o + 3][i]; w.p = S.G[o + 4][i];
+  return w;
+}
+__device__ __forceinline__ void jat_put_g(JatLDS& S, int i, const D5& l, const D5& r) {
+  S.G[0][i] = l.r; S.G[1][i] = l.vx; S.G[2][i] = l.vy; S.G[3][i] = l.vz; S.G[4][i] = l.p;
+  S.G[5][i] = r.r; S.G[6][i] = r.vx; S.G[7][i] = r.vy; S.G[8][i] = r.vz; S.G[9][i] = r.p;
+}
+__device__ __forceinline__ D5 half_diff(const D5& a, const D5& b) {
+  D5 s;
+  s.r = 0.5 * a.r - 0.5 * b.r;
+  s.vx = 0.5 * a.vx - 0.5 * b.vx;
+  s.vy = 0.5 * a.vy - 0.5 * b.vy;
+  s.vz = 0.5 * a.vz - 0.5 * b.vz;
+  s.p = 0.5 * a.p - 0.5 * b.p;
+  return s;
+}
+__device__ __forceinline__ Cons<double> mu_of(double a, const Cons<double>& l, const Cons<double>& r) {
+  Cons<double> m;
+  m.d = a * l.d - a * r.d;
+  m.mx = a * l.mx - a * r.mx;
+  m.my = a * l.my - a * r.my;
+  m.mz = a * l.mz - a * r.mz;
+  m.e = a * l.e - a * r.e;
+  return m;
+}
+
+// Slope adjoint terms gathered by a cell from its own slope and its two
+// neighbours' slopes along one axis. w[0..4] = W at offsets −2..+2; gl[k], gr[k]
+// = left/right face-state adjoints of the faces at offsets −3/2.. +3/2 (k = 0..3).
+__device__ __forceinline__ void gather_line(const D5 w[5], const D5 gl[4], const D5 gr[4], D5& Wb) {
+  // own faces: hi face (k = 2) left state, lo face (k = 1) right state
+  d5_add(Wb, gl[2]);
+  d5_add(Wb, gr[1]);
+  D5 lb = d5_zero(), cb = d5_zero(), rb = d5_zero();
+  slope_vjp(w[1], w[2], w[3], half_diff(gl[2], gr[1]), lb, cb, rb);
+  d5_add(Wb, cb);
+  D5 lb2 = d5_zero(), cb2 = d5_zero(), rb2 = d5_zero();
+  slope_vjp(w[2], w[3], w[4], half_diff(gl[3], gr[2]), lb2, cb2, rb2);   // hi neighbour: this cell is its lo
+  d5_add(Wb, lb2);
+  D5 lb3 = d5_zero(), cb3 = d5_zero(), rb3 = d5_zero();
+  slope_vjp(w[0], w[1], w[2], half_diff(gl[1], gr[0]), lb3, cb3, rb3);   // lo neighbour: this cell is its hi
+  d5_add(Wb, rb3);
+}
+
+__global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double* __restrict__ U,
+                                                                 const double* __restrict__ lam,
+                                                                 const double* base1, double a1,
+                                                                 const double* base2, double a2, double beta,
+                                                                 const double* __restrict__ dtp, GridDev G,
+                                                                 double* out) {
+  __shared__ JatLDS S;
+  const int nx = G.n[0], ny = G.n[1];
+  const long long n = G.ncell;
+  const int tid = threadIdx.x;
+  const int tx = tid % JTX, ty = tid / JTX;
+  const int x0 = blockIdx.x * JTX, y0 = blockIdx.y * JTY;
+  const double g = G.gamma;
+  auto wrapx = [&](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
+  auto wrapy = [&](int y) { return y < 0 ? y + ny : (y >= ny ? y - ny : y); };
+  auto cidx = [&](int x, int y) { return (long long)wrapy(y) * nx + wrapx(x); };
+  auto put5 = [](double* a0, int stride, int i, const D5& w) {
+    a0[i] = w.r; a0[stride + i] = w.vx; a0[2 * stride + i] = w.vy; a0[3 * stride + i] = w.vz; a0[4 * stride + i] = w.p;
+  };
+  (void)put5;
+  // ---- load: own cell, then the cross-shaped halos of W (3) and λ (2) ----
+  const long long c = (long long)(y0 + ty) * nx + (x0 + tx);
+  const Cons<double> Uc = load_cons(U, n, c);
+  {
+    const D5 w = cons2prim<double, false>(Uc, g, g);
+    S.Wx[0][ty][tx + 3] = w.r; S.Wx[1][ty][tx + 3] = w.vx; S.Wx[2][ty][tx + 3] = w.vy;
+    S.Wx[3][ty][tx + 3] = w.vz; S.Wx[4][ty][tx + 3] = w.p;
+    const Cons<double> l = load_cons(lam, n, c);
+    S.Lx[0][ty][tx + 2] = l.d; S.Lx[1][ty][tx + 2] = l.mx; S.Lx[2][ty][tx + 2] = l.my;
+    S.Lx[3][ty][tx + 2] = l.mz; S.Lx[4][ty][tx + 2] = l.e;
+  }
+  constexpr int NWX = 6 * JTY, NLX = 4 * JTY, NWY = 6 * JTX, NLY = 4 * JTX;
+  for (int h = tid; h < NWX + NLX + NWY + NLY; h += 256) {
+    if (h < NWX) {
+      const int r = h / 6, q = h % 6;
+      const int lc = q < 3 ? q : JTX + q;   // 0,1,2 | JTX+3..JTX+5
+      const D5 w = cons2prim<double, false>(load_cons(U, n, cidx(x0 - 3 + lc, y0 + r)), g, g);
+      S.Wx[0][r][lc] = w.r; S.Wx[1][r][lc] = w.vx; S.Wx[2][r][lc] = w.vy; S.Wx[3][r][lc] = w.vz; S.Wx[4][r][lc] = w.p;
+    } else if (h < NWX + NLX) {
+      const int hh = h - NWX;
+      const int r = hh / 4, q = hh % 4;
+      const int lc = q < 2 ? q : JTX + q;   // 0,1 | JTX+2, JTX+3
+      const Cons<double> l = load_cons(lam, n, cidx(x0 - 2 + lc, y0 + r));
+      S.Lx[0][r][lc] = l.d; S.Lx[1][r][lc] = l.mx; S.Lx[2][r][lc] = l.my; S.Lx[3][r][lc] = l.mz; S.Lx[4][r][lc] = l.e;
+    } else if (h < NWX + NLX + NWY) {
+      const int hh = h - NWX - NLX;
+      const int i = hh / JTX, cc = hh % JTX;
+      const int y = i < 3 ? y0 - 3 + i : y0 + JTY + (i - 3);
+      const D5 w = cons2prim<double, false>(load_cons(U, n, cidx(x0 + cc, y)), g, g);
+      S.Wy[0][i][cc] = w.r; S.Wy[1][i][cc] = w.vx; S.Wy[2][i][cc] = w.vy; S.Wy[3][i][cc] = w.vz; S.Wy[4][i][cc] = w.p;
+    } else {
+      const int hh = h - NWX - NLX - NWY;
+      const int i = hh / JTX, cc = hh % JTX;
+      const int y = i < 2 ? y0 - 2 + i : y0 + JTY + (i - 2);
+      const Cons<double> l = load_cons(lam, n, cidx(x0 + cc, y));
+      S.Ly[0][i][cc] = l.d; S.Ly[1][i][cc] = l.mx; S.Ly[2][i][cc] = l.my; S.Ly[3][i][cc] = l.mz; S.Ly[4][i][cc] = l.e;
+    }
+  }
+  __syncthreads();
+
+  D5 Wb = d5_zero();
+  // ---- x faces: ql = 0..JTX+2 (face between columns x0−2+ql and x0−1+ql) ----
+  for (int h = tid; h < JTY * (JTX + 3); h += 256) {
+    const int r = h / (JTX + 3), ql = h % (JTX + 3);
+    const D5 w0 = jat_get(S.Wx, r, ql), w1 = jat_get(S.Wx, r, ql + 1), w2 = jat_get(S.Wx, r, ql + 2),
+             w3 = jat_get(S.Wx, r, ql + 3);
+    const D5 WL = face_value(w1, slope(w0, w1, w2), 0.5);
+    const D5 WR = face_value(w2, slope(w1, w2, w3), -0.5);
+    const Cons<double> mu = mu_of(G.area[0], jat_lx(S, r, ql), jat_lx(S, r, ql + 1));
+    D5 bL = d5_zero(), bR = d5_zero();
+    hllc_vjp<0>(WL, WR, g, mu, bL, bR);
+    jat_put_g(S, r * (JTX + 3) + ql, bL, bR);
+  }
+  __syncthreads();
+  {
+    D5 w[5], gl[4], gr[4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = jat_get(S.Wx, ty, tx + 1 + k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      gl[k] = jat_g(S, 0, ty * (JTX + 3) + tx + k);
+      gr[k] = jat_g(S, 1, ty * (JTX + 3) + tx + k);
+    }
+    gather_line(w, gl, gr, Wb);
+  }
+  __syncthreads();
+  // ---- y faces: ql = 0..JTY+2 (face between rows y0−2+ql and y0−1+ql) ----
+  for (int h = tid; h < (JTY + 3) * JTX; h += 256) {
+    const int ql = h / JTX, cc = h % JTX;
+    const D5 w0 = jat_wy(S, ql - 3, cc), w1 = jat_wy(S, ql - 2, cc), w2 = jat_wy(S, ql - 1, cc),
+             w3 = jat_wy(S, ql, cc);
+    const D5 WL = face_value(w1, slope(w0, w1, w2), 0.5);
+    const D5 WR = face_value(w2, slope(w1, w2, w3), -0.5);
+    const Cons<double> mu = mu_of(G.area[1], jat_ly(S, ql - 2, cc), jat_ly(S, ql - 1, cc));
+    D5 bL = d5_zero(), bR = d5_zero();
+    hllc_vjp<1>(WL, WR, g, mu, bL, bR);
+    jat_put_g(S, ql * JTX + cc, bL, bR);
+  }
+  __syncthreads();
+  {
+    D5 w[5], gl[4], gr[4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) w[k] = jat_wy(S, ty - 2 + k, tx);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      gl[k] = jat_g(S, 0, (ty + k) * JTX + tx);
+      gr[k] = jat_g(S, 1, (ty + k) * JTX + tx);
+    }
+    gather_line(w, gl, gr, Wb);
+  }
+  const Cons<double> Ub = c2p_vjp(Uc, g, Wb);
+  const double sc = beta * (*dtp / (1.0 * G.V));
+  const double ub[5] = {Ub.d, Ub.mx, Ub.my, Ub.mz, Ub.e};
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    double v = a1 * base1[f * n + c];
+    if (base2) v += a2 * base2[f * n + c];
+    out[f * n + c] = v + sc * ub[f];
   }
 }
 

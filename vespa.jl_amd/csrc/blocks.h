@@ -413,7 +413,13 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
         }
         Cons<T> acc;
         acc.d = acc.mx = acc.my = acc.mz = acc.e = T(0);
-        if (lc < 0) {
+        if (lc < 0 && kind[0] == 0 && (RANK < 2 || kind[1] == 0) && (RANK < 3 || kind[2] == 0)) {
+          // interior cell: lo faces z, y, x, then hi faces x, y, z
+#pragma unroll
+          for (int d = RANK - 1; d >= 0; --d) acc_add(acc, flux(2 * d), -ar[d]);
+#pragma unroll
+          for (int d = 0; d < RANK; ++d) acc_add(acc, flux(2 * d + 1), ar[d]);
+        } else if (lc < 0) {
           // the uniform walk order of rk_stage_kernel
 #pragma unroll
           for (int d = RANK - 1; d >= 0; --d)

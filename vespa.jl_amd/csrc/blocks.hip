@@ -180,7 +180,7 @@ static int balanced(int n, int maxT) {
 }
 
 // Builds every table of a mesh. Returns 0 or a negative code (message set).
-static int build_tables(Topo& T, int zchunk, HostTables& H) {
+static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   const int nb = T.nb, rank = T.rank;
   // ---- candidate neighbours by face plane ----
   T.cand.assign(6 * nb, {});
@@ -457,7 +457,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H) {
       tx = balanced(bk.n[0], 64);
       ty = 1;
     } else {
-      tx = balanced(bk.n[0], kMaxE);
+      tx = balanced(bk.n[0], std::min(kMaxE, std::max(4, max_tx)));
       int maxty = kLanes / tx;
       if (maxty > kMaxE) maxty = kMaxE;
       ty = balanced(bk.n[1], maxty);
@@ -670,6 +670,7 @@ struct vh_blocks {
   int dtype = VH_F64, mode = VH_EXACT, device = 0;
   size_t esize = 8;
   int zchunk = 16;
+  int max_tx = 32;        // widest tile along x
   int max_wgs = 0;        // persistent workgroups (0: 3 per CU)
   int ncu = 256;
   int stage_wgs = 0;      // workgroups of the last all-blocks stage-2 launch
@@ -1031,7 +1032,7 @@ int vh_blocks_create(vh_blocks** out, int32_t rank, int32_t nblocks, int32_t max
   if (!(gamma > 1.0)) return fail(-1, "gamma must exceed 1%s", "");
   vh_blocks* m = new vh_blocks();
   int rc = make_topo(m->T, rank, nblocks, maxlevel, h0, level, dims, origin, bc);
-  if (!rc) rc = build_tables(m->T, m->zchunk, m->H);
+  if (!rc) rc = build_tables(m->T, m->zchunk, m->H, m->max_tx);
   if (rc) { delete m; return rc; }
   m->dtype = dtype;
   m->esize = dtype == VH_F64 ? 8 : 4;
@@ -1198,7 +1199,7 @@ int vh_blocks_set_launch(vh_blocks* m, int32_t zchunk, int32_t max_workgroups, i
   m->max_wgs = max_workgroups;
   if (zchunk != m->zchunk) {
     HostTables H;
-    int rc = build_tables(m->T, zchunk, H);
+    int rc = build_tables(m->T, zchunk, H, m->max_tx);
     if (rc) return rc;
     free_tables(m);
     m->H = std::move(H);
@@ -1206,6 +1207,24 @@ int vh_blocks_set_launch(vh_blocks* m, int32_t zchunk, int32_t max_workgroups, i
     rc = upload_tables(m);
     if (rc) return rc;
   }
+  return alloc_partials(m);
+}
+
+int vh_blocks_set_tiling(vh_blocks* m, int32_t max_tile_x) {
+  if (!m) return fail(-1, "null argument%s", "");
+  if (max_tile_x < 4 || max_tile_x > 32) return fail(-1, "max_tile_x must be 4..32%s (got %lld)", "", max_tile_x);
+  if (max_tile_x == m->max_tx) return 0;
+  VH_HIP(hipSetDevice(m->device));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  HostTables H;
+  int rc = build_tables(m->T, m->zchunk, H, max_tile_x);
+  if (rc) return rc;
+  drop_graph(m);
+  free_tables(m);
+  m->H = std::move(H);
+  m->max_tx = max_tile_x;
+  rc = upload_tables(m);
+  if (rc) return rc;
   return alloc_partials(m);
 }
 

@@ -41,5 +41,10 @@ void adj_launch_faces(hipStream_t s, const double* W, const double* lam, const G
 void adj_launch_cells(hipStream_t s, const double* U, const double* W, double* const gL[3], double* const gR[3],
                       const double* base1, double a1, const double* base2, double a2, double beta,
                       const double* dtp, const GridDev& G, double* out);
+// Fused single-kernel form for 2D grids periodic on both axes whose sizes are
+// multiples of the 32×8 tile; returns false (nothing launched) otherwise.
+bool adj_launch_jat_fused(hipStream_t s, const double* U, const double* lam, const double* base1, double a1,
+                          const double* base2, double a2, double beta, const double* dtp, const GridDev& G,
+                          double* out);
 
 }  // namespace vh

@@ -125,8 +125,16 @@ __device__ __forceinline__ T energy(const Prim<T>& W, T g, T igm1) {
 
 template <typename T>
 __device__ __forceinline__ T minmod(T a, T b) {
+#if VH_FAST_PROFILE
+  // max(min(a,b),0) + min(max(a,b),0): the smaller-magnitude argument when the
+  // signs agree, else 0 — four min/max and one add instead of a product, two
+  // compares and two 64-bit selects. Differs from the reference form only when
+  // a·b underflows to zero with both signs equal (a subnormal-scale slope).
+  return vmax(vmin(a, b), T(0)) + vmin(vmax(a, b), T(0));
+#else
   if (a * b <= T(0)) return T(0);
   return vabs(a) < vabs(b) ? a : b;
+#endif
 }
 
 // limited_slope(l, c, r) = minmod(c - l, r - c) per primitive component.

@@ -385,29 +385,40 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
       const int kz = k > 0 ? 0 : (G.bc[4] == BC_PERIODIC ? 2 : 1);
       Cons<T> acc;
       acc.d = acc.mx = acc.my = acc.mz = acc.e = T(0);
-      if constexpr (RANK > 2) {
-        if (kz == 0) acc_add(acc, fzlo, -a2);
-      }
-      if constexpr (RANK > 1) {
-        if (ky == 0) acc_add(acc, fylo, -a1);
-      }
-      if (kx == 0) acc_add(acc, fxlo, -a0);
-      acc_add(acc, fxhi, a0);
-      if (kx == 1) acc_add(acc, fxlo, -a0);
-      if constexpr (RANK > 1) {
-        acc_add(acc, fyhi, a1);
-        if (ky == 1) acc_add(acc, fylo, -a1);
-      }
-      if constexpr (RANK > 2) {
-        acc_add(acc, fzhi, a2);
-        if (kz == 1) acc_add(acc, fzlo, -a2);
-      }
-      if (kx == 2) acc_add(acc, fxlo, -a0);
-      if constexpr (RANK > 1) {
-        if (ky == 2) acc_add(acc, fylo, -a1);
-      }
-      if constexpr (RANK > 2) {
-        if (kz == 2) acc_add(acc, fzlo, -a2);
+      if (kx == 0 && (RANK < 2 || ky == 0) && (RANK < 3 || kz == 0)) {
+        // interior cell (every wave away from the lo domain faces): the same
+        // order as below without per-lane selects
+        if constexpr (RANK > 2) acc_add(acc, fzlo, -a2);
+        if constexpr (RANK > 1) acc_add(acc, fylo, -a1);
+        acc_add(acc, fxlo, -a0);
+        acc_add(acc, fxhi, a0);
+        if constexpr (RANK > 1) acc_add(acc, fyhi, a1);
+        if constexpr (RANK > 2) acc_add(acc, fzhi, a2);
+      } else {
+        if constexpr (RANK > 2) {
+          if (kz == 0) acc_add(acc, fzlo, -a2);
+        }
+        if constexpr (RANK > 1) {
+          if (ky == 0) acc_add(acc, fylo, -a1);
+        }
+        if (kx == 0) acc_add(acc, fxlo, -a0);
+        acc_add(acc, fxhi, a0);
+        if (kx == 1) acc_add(acc, fxlo, -a0);
+        if constexpr (RANK > 1) {
+          acc_add(acc, fyhi, a1);
+          if (ky == 1) acc_add(acc, fylo, -a1);
+        }
+        if constexpr (RANK > 2) {
+          acc_add(acc, fzhi, a2);
+          if (kz == 1) acc_add(acc, fzlo, -a2);
+        }
+        if (kx == 2) acc_add(acc, fxlo, -a0);
+        if constexpr (RANK > 1) {
+          if (ky == 2) acc_add(acc, fylo, -a1);
+        }
+        if constexpr (RANK > 2) {
+          if (kz == 2) acc_add(acc, fzlo, -a2);
+        }
       }
 
       Cons<T> o;

@@ -629,6 +629,7 @@ struct AdjWork {
   double* U1 = nullptr;
   double* dt = nullptr;
   size_t bytes = 0;
+  int fused = 1;   // use the single-kernel 2D periodic form where it applies
 };
 
 static void free_adj_work(vh_grid* g) {
@@ -670,6 +671,7 @@ static int ensure_adj_work(vh_grid* g) {
 static void enqueue_jat(vh_grid* g, const double* U, const double* lam_in, const double* base1, double a1,
                         const double* base2, double a2, double beta, const double* dtp, double* out) {
   AdjWork* w = g->adj;
+  if (w->fused && adj_launch_jat_fused(g->stream, U, lam_in, base1, a1, base2, a2, beta, dtp, g->G, out)) return;
   adj_launch_prim(g->stream, U, w->W, g->G);
   adj_launch_faces(g->stream, w->W, lam_in, g->G, w->gL, w->gR);
   adj_launch_cells(g->stream, U, w->W, w->gL, w->gR, base1, a1, base2, a2, beta, dtp, g->G, out);
@@ -721,6 +723,16 @@ int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, v
   enqueue_step_vjp(g, (const double*)dU, nullptr, g->adj->dt, (const double*)dWbar, (double*)dUbar);
   VH_HIP(hipGetLastError());
   VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_set_adjoint_form(vh_grid* g, int32_t fused) {
+  if (!g) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  int rc = ensure_adj_work(g);
+  if (rc) return rc;
+  VH_HIP(hipStreamSynchronize(g->stream));
+  g->adj->fused = fused ? 1 : 0;
   return 0;
 }
 
