@@ -197,7 +197,7 @@ def adjoint_leg(V, args):
         return float(np.sum((sim.state()[0] - target) ** 2))
 
     rng = np.random.default_rng(2024)
-    fd = []
+    fd = [{"adjoint": 0.0, "fd": 0.0, "rel": 0.0}] if args.fd_directions <= 0 else []
     for _ in range(args.fd_directions):
         v = rng.normal(size=U0.shape)
         v /= np.linalg.norm(v)
@@ -293,6 +293,8 @@ def main():
                     help="skip the FP64 run in the other arithmetic profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-blocks", action="store_true", help="skip the batched-blocks leg")
+    ap.add_argument("--skip-uniform", action="store_true",
+                    help="skip the 256^3 legs (profiling runs of the other legs only; no headline value)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint rollout leg")
     ap.add_argument("--no-sedov", action="store_true", help="skip the dynamic-AMR Sedov leg")
     ap.add_argument("--adjoint-n", type=int, default=512)
@@ -317,121 +319,123 @@ def main():
         raise SystemExit("no HIP device visible")
     from vespa_jl_amd import problems as P
 
-    n = args.n
-    ncell = n ** 3
-    log(f"[bench] building {n}^3 turbulence-like initial state")
-    U64 = P.turbulence_state(n)
+    out = {}
+    if not args.skip_uniform:
+        n = args.n
+        ncell = n ** 3
+        log(f"[bench] building {n}^3 turbulence-like initial state")
+        U64 = P.turbulence_state(n)
 
-    calib = None
-    if args.calib_copy:
-        nb = 1 << 30
-        a, b = V.DeviceBuffer(nb), V.DeviceBuffer(nb)
-        b.copy_from(a)
-        t0 = time.perf_counter()
-        for _ in range(3):
+        calib = None
+        if args.calib_copy:
+            nb = 1 << 30
+            a, b = V.DeviceBuffer(nb), V.DeviceBuffer(nb)
             b.copy_from(a)
-        el = (time.perf_counter() - t0) / 3
-        calib = {"copy_bytes": nb, "copy_GBps_read_plus_write": 2 * nb / el / 1e9}
-        a.free()
-        b.free()
-        log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
+            t0 = time.perf_counter()
+            for _ in range(3):
+                b.copy_from(a)
+            el = (time.perf_counter() - t0) / 3
+            calib = {"copy_bytes": nb, "copy_GBps_read_plus_write": 2 * nb / el / 1e9}
+            a.free()
+            b.free()
+            log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
 
-    log(f"[bench] FP64 run ({args.mode} profile)")
-    sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, args.mode)
-    launch = sim.launch()
-    wall, ev = timed_run(V, sim, args.steps, args.warmup)
-    value = ncell * args.steps / wall
-    ms_per_step = 1e3 * wall / args.steps
-    stage2_bytes = 15 * 8 * ncell
-    stage1_bytes = 10 * 8 * ncell
-    ach = stage2_bytes / (ev["stage2_ms"] * 1e-3) / 1e9
-    log(f"[bench] FP64 {value / 1e9:.3f} G zone-updates/s, {ms_per_step:.3f} ms/step, "
-        f"stage1 {ev['stage1_ms']:.3f} ms, stage2 {ev['stage2_ms']:.3f} ms, stage2 {ach:.0f} GB/s")
-    U64_end = sim.state() if not (args.no_fp32 and args.no_other_mode) else None
-    sim.close()
+        log(f"[bench] FP64 run ({args.mode} profile)")
+        sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, args.mode)
+        launch = sim.launch()
+        wall, ev = timed_run(V, sim, args.steps, args.warmup)
+        value = ncell * args.steps / wall
+        ms_per_step = 1e3 * wall / args.steps
+        stage2_bytes = 15 * 8 * ncell
+        stage1_bytes = 10 * 8 * ncell
+        ach = stage2_bytes / (ev["stage2_ms"] * 1e-3) / 1e9
+        log(f"[bench] FP64 {value / 1e9:.3f} G zone-updates/s, {ms_per_step:.3f} ms/step, "
+            f"stage1 {ev['stage1_ms']:.3f} ms, stage2 {ev['stage2_ms']:.3f} ms, stage2 {ach:.0f} GB/s")
+        U64_end = sim.state() if not (args.no_fp32 and args.no_other_mode) else None
+        sim.close()
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_rk_stage2_256cubed_f64.json")
-    if os.path.exists(tfile) and n == 256:
-        try:
-            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "traffic_rk_stage2_256cubed_f64.json")
+        if os.path.exists(tfile) and n == 256:
+            try:
+                traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+            except (ValueError, OSError):
+                traffic = None
 
-    out = {
-        "metric": "zone-updates/s & %HBM peak, 3D 256³ PLM+HLLC RK2; adjoint/forward time ratio",
-        "value": value,
-        "unit": "zone-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "arithmetic_profile": args.mode,
-        "data": "synthetic",
-        "config": {"workload": f"3D {n}^3 periodic smooth turbulence-like state (rho=1+0.1*delta, "
-                               f"|k|<=8 random-phase modes, v_rms=0.1 c_s, p=1), gamma=5/3, "
-                               f"PLM-minmod+HLLC SSP-RK2, CFL 0.3 with device dt every step, FP64",
-                   "grid": [n, n, n], "fields": "5 conserved, SoA", "tile": launch["tile"],
-                   "zchunk": launch["zchunk"], "workgroups_per_stage": launch["workgroups"]},
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "rk_stage_kernel<double,3,...,STAGE2=true>",
-                     "bytes_per_launch": stage2_bytes, "mean_launch_ms": ev["stage2_ms"]},
-        "step_detail": {"stage1_ms": ev["stage1_ms"], "stage2_ms": ev["stage2_ms"],
-                        "event_step_ms": ev["step_ms"],
-                        "stage1_GBps": stage1_bytes / (ev["stage1_ms"] * 1e-3) / 1e9,
-                        "step_algorithmic_GBps": 200 * ncell / (ms_per_step * 1e-3) / 1e9},
-    }
+        out = {
+            "metric": "zone-updates/s & %HBM peak, 3D 256³ PLM+HLLC RK2; adjoint/forward time ratio",
+            "value": value,
+            "unit": "zone-updates/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms_per_step,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "arithmetic_profile": args.mode,
+            "data": "synthetic",
+            "config": {"workload": f"3D {n}^3 periodic smooth turbulence-like state (rho=1+0.1*delta, "
+                                   f"|k|<=8 random-phase modes, v_rms=0.1 c_s, p=1), gamma=5/3, "
+                                   f"PLM-minmod+HLLC SSP-RK2, CFL 0.3 with device dt every step, FP64",
+                       "grid": [n, n, n], "fields": "5 conserved, SoA", "tile": launch["tile"],
+                       "zchunk": launch["zchunk"], "workgroups_per_stage": launch["workgroups"]},
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                         "kernel": "rk_stage_kernel<double,3,...,STAGE2=true>",
+                         "bytes_per_launch": stage2_bytes, "mean_launch_ms": ev["stage2_ms"]},
+            "step_detail": {"stage1_ms": ev["stage1_ms"], "stage2_ms": ev["stage2_ms"],
+                            "event_step_ms": ev["step_ms"],
+                            "stage1_GBps": stage1_bytes / (ev["stage1_ms"] * 1e-3) / 1e9,
+                            "step_algorithmic_GBps": 200 * ncell / (ms_per_step * 1e-3) / 1e9},
+        }
 
-    if calib is not None:
-        out["calibration"] = calib
+        if calib is not None:
+            out["calibration"] = calib
 
-    if not args.no_other_mode:
-        other = "exact" if args.mode == "fast" else "fast"
-        log(f"[bench] FP64 run ({other} profile)")
-        so = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, other)
-        wo, evo = timed_run(V, so, args.steps, args.warmup)
-        Uo_end = so.state()
-        so.close()
-        vo = ncell * args.steps / wo
-        rel = (np.abs(Uo_end - U64_end).reshape(5, -1).max(axis=1) /
-               np.abs(U64_end).reshape(5, -1).max(axis=1)) if U64_end is not None else None
-        out["fp64_" + other] = {"value": vo, "unit": "zone-updates/s", "ms_per_step": 1e3 * wo / args.steps,
-                                "stage1_ms": evo["stage1_ms"], "stage2_ms": evo["stage2_ms"],
-                                "stage2_GBps": stage2_bytes / (evo["stage2_ms"] * 1e-3) / 1e9,
-                                "max_rel_diff_vs_headline_per_field":
-                                    rel.tolist() if rel is not None else None}
-        log(f"[bench] FP64 {other}: {vo / 1e9:.3f} G zone-updates/s")
+        if not args.no_other_mode:
+            other = "exact" if args.mode == "fast" else "fast"
+            log(f"[bench] FP64 run ({other} profile)")
+            so = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, other)
+            wo, evo = timed_run(V, so, args.steps, args.warmup)
+            Uo_end = so.state()
+            so.close()
+            vo = ncell * args.steps / wo
+            rel = (np.abs(Uo_end - U64_end).reshape(5, -1).max(axis=1) /
+                   np.abs(U64_end).reshape(5, -1).max(axis=1)) if U64_end is not None else None
+            out["fp64_" + other] = {"value": vo, "unit": "zone-updates/s", "ms_per_step": 1e3 * wo / args.steps,
+                                    "stage1_ms": evo["stage1_ms"], "stage2_ms": evo["stage2_ms"],
+                                    "stage2_GBps": stage2_bytes / (evo["stage2_ms"] * 1e-3) / 1e9,
+                                    "max_rel_diff_vs_headline_per_field":
+                                        rel.tolist() if rel is not None else None}
+            log(f"[bench] FP64 {other}: {vo / 1e9:.3f} G zone-updates/s")
 
-    if not args.no_fp32:
-        log("[bench] FP32 run")
-        U32 = U64.astype(np.float32)
-        s32 = make_sim(V, n, np.float32, U32, args.tile, args.zchunk, args.mode)
-        wall32, ev32 = timed_run(V, s32, args.steps, args.warmup)
-        v32 = ncell * args.steps / wall32
-        U32_end = s32.state().astype(np.float64)
-        s32.close()
-        den = np.abs(U64_end).mean(axis=(1, 2, 3))
-        l1 = (np.abs(U32_end - U64_end).mean(axis=(1, 2, 3)) / den).tolist()
-        out["fp32"] = {"arithmetic_profile": args.mode, "value": v32, "unit": "zone-updates/s", "ratio_vs_fp64": v32 / value,
-                       "ms_per_step": 1e3 * wall32 / args.steps, "stage1_ms": ev32["stage1_ms"],
-                       "stage2_ms": ev32["stage2_ms"],
-                       "stage2_GBps": 15 * 4 * ncell / (ev32["stage2_ms"] * 1e-3) / 1e9,
-                       "rel_L1_vs_fp64_per_field": dict(zip(
-                           ["density", "momentum_x", "momentum_y", "momentum_z", "energy"], l1)),
-                       "after_steps": args.warmup + args.steps}
-        log(f"[bench] FP32 {v32 / 1e9:.3f} G zone-updates/s (x{v32 / value:.2f}), rel L1 {l1}")
+        if not args.no_fp32:
+            log("[bench] FP32 run")
+            U32 = U64.astype(np.float32)
+            s32 = make_sim(V, n, np.float32, U32, args.tile, args.zchunk, args.mode)
+            wall32, ev32 = timed_run(V, s32, args.steps, args.warmup)
+            v32 = ncell * args.steps / wall32
+            U32_end = s32.state().astype(np.float64)
+            s32.close()
+            den = np.abs(U64_end).mean(axis=(1, 2, 3))
+            l1 = (np.abs(U32_end - U64_end).mean(axis=(1, 2, 3)) / den).tolist()
+            out["fp32"] = {"arithmetic_profile": args.mode, "value": v32, "unit": "zone-updates/s", "ratio_vs_fp64": v32 / value,
+                           "ms_per_step": 1e3 * wall32 / args.steps, "stage1_ms": ev32["stage1_ms"],
+                           "stage2_ms": ev32["stage2_ms"],
+                           "stage2_GBps": 15 * 4 * ncell / (ev32["stage2_ms"] * 1e-3) / 1e9,
+                           "rel_L1_vs_fp64_per_field": dict(zip(
+                               ["density", "momentum_x", "momentum_y", "momentum_z", "energy"], l1)),
+                           "after_steps": args.warmup + args.steps}
+            log(f"[bench] FP32 {v32 / 1e9:.3f} G zone-updates/s (x{v32 / value:.2f}), rel L1 {l1}")
 
-    if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle, OpenMP)")
-        out["cpu_baseline"] = cpu_baseline(U64, n)
-        log(f"[bench] CPU {out['cpu_baseline']['value'] / 1e6:.2f} M zone-updates/s")
-    else:
-        out["cpu_baseline"] = None
+        if not args.no_cpu_baseline:
+            log("[bench] CPU baseline (oracle, OpenMP)")
+            out["cpu_baseline"] = cpu_baseline(U64, n)
+            log(f"[bench] CPU {out['cpu_baseline']['value'] / 1e6:.2f} M zone-updates/s")
+        else:
+            out["cpu_baseline"] = None
 
     if not args.no_blocks:
         out["batched_blocks"] = blocks_leg(V, args)

@@ -334,7 +334,7 @@ class BlockSimulation:
         from . import _lib
         self._check(self._lib.vh_blocks_set_mode(self._h, {"exact": _lib.VH_EXACT, "fast": _lib.VH_FAST}[mode]))
 
-    def set_launch(self, zchunk: int = 16, max_workgroups: int = 0, use_graph: bool = True):
+    def set_launch(self, zchunk: int = 8, max_workgroups: int = 0, use_graph: bool = True):
         self._check(self._lib.vh_blocks_set_launch(self._h, int(zchunk), int(max_workgroups), int(bool(use_graph))))
 
     def set_tiling(self, max_tile_x: int = 32):

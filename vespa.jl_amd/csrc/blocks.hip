@@ -669,7 +669,7 @@ struct vh_blocks {
   HostTables H;
   int dtype = VH_F64, mode = VH_EXACT, device = 0;
   size_t esize = 8;
-  int zchunk = 16;
+  int zchunk = 8;
   int max_tx = 32;        // widest tile along x
   int max_wgs = 0;        // persistent workgroups (0: 3 per CU)
   int ncu = 256;
