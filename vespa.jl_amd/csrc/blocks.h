@@ -206,23 +206,24 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
     Prim<T> wz0, wz1, wz2, sz0;
     Cons<T> fzlo;
     if constexpr (RANK == 3) {
-      Prim<T> wm[2];
-      Prim<T>* dst[5] = {&wm[0], &wm[1], &wz0, &wz1, &wz2};
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
+      auto zcol = [&](int kk) -> Prim<T> {
         int gh;
-        const long long cc = block_src<2>(bg, bk, B.repmap, bzl, sxc, syc, kz0 - 2 + q, gh);
+        const long long cc = block_src<2>(bg, bk, B.repmap, bzl, sxc, syc, kk, gh);
         Prim<T> w = cons2prim<T, FLOOR64>(load_cons(Us, n, cc), g, B.gamma);
         if (gh) w = ghost(w, gh == 1 ? bzl : bzh, 3);
-        *dst[q] = w;
-      }
-      const Prim<T> sm1 = slope(wm[0], wm[1], wz0);
-      sz0 = slope(wm[1], wz0, wz1);
+        return w;
+      };
+      const Prim<T> wm2 = zcol(kz0 - 2), wm1 = zcol(kz0 - 1);
+      wz0 = zcol(kz0);
+      wz1 = zcol(kz0 + 1);
+      wz2 = zcol(kz0 + 2);
+      const Prim<T> sm1 = slope(wm2, wm1, wz0);
+      sz0 = slope(wm1, wz0, wz1);
       if (bzl != BC_PERIODIC && kz0 == 0) {
         const Prim<T> R = face_value(wz0, sz0, T(-0.5));
         fzlo = hllc(ghost(R, bzl, 3), R, g, igm1, 3);
       } else {
-        fzlo = hllc(face_value(wm[1], sm1, T(0.5)), face_value(wz0, sz0, T(-0.5)), g, igm1, 3);
+        fzlo = hllc(face_value(wm1, sm1, T(0.5)), face_value(wz0, sz0, T(-0.5)), g, igm1, 3);
       }
     }
 

@@ -8,7 +8,10 @@
 #   * tier 2: `HIPGrid`, a device-resident uniform grid whose `hip_step!` /
 #     `hip_evolve!` replace `step!` / `evolve!` (src/driver.jl:333-359, 483-508)
 #     for RefMesh `UniformMesh` simulations, and `hip_set_mode!` selecting the
-#     exact (bit-identical) or fast arithmetic profile.
+#     exact (bit-identical) or fast arithmetic profile;
+#   * `HIPBlocks`: block-structured AMR meshes (hanging-node faces, flux
+#     registers, subcycling) advanced by one persistent launch per RK stage;
+#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls).
 #
 # Load with `include` after `using Vespa`; set ENV["VESPAHIP_LIB"] or keep
 # libvespahip.so next to the package's vespa.jl_amd/ directory.
@@ -145,6 +148,105 @@ function hip_conserved_totals(g::HIPGrid)
     return (mass = out[1], momentum_x = out[2], momentum_y = out[3], momentum_z = out[4], energy = out[5])
 end
 
-export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals
+# ------------------------------------------------------------------ batched blocks
+
+"""
+    HIPBlocks(rank, maxlevel, h0, γ, level, dims, origin, bc; eltype = Float64)
+
+Device-resident block mesh (block-structured AMR with hanging nodes): `level[b]`,
+`dims[:, b]` (cells per axis), `origin[:, b]` (integer lattice of the finest cells)
+and `bc[:, b]` (6 codes, used where no block touches a face). Leaves are ordered
+block by block, x fastest; faces follow HGBackend (lib/HGBackend/src/HGBackend.jl:202-276).
+"""
+mutable struct HIPBlocks
+    ptr::Ptr{Cvoid}
+    ncell::Int
+    eltype::DataType
+end
+
+function HIPBlocks(rank::Integer, maxlevel::Integer, h0::AbstractVector, γ::Real, level::AbstractVector,
+                   dims::AbstractMatrix, origin::AbstractMatrix, bc::AbstractMatrix; eltype = Float64)
+    nb = length(level)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    lv = Int32.(level); dm = Int32.(dims); og = Int64.(origin); bcs = Int32.(bc)
+    h0v = Float64[i <= length(h0) ? h0[i] : 1.0 for i in 1:3]
+    _check(ccall((:vh_blocks_create, libvespahip), Cint,
+                 (Ref{Ptr{Cvoid}}, Int32, Int32, Int32, Ptr{Float64}, Float64, Ptr{Int32}, Ptr{Int32}, Ptr{Int64},
+                  Ptr{Int32}, Int32),
+                 h, Int32(rank), Int32(nb), Int32(maxlevel), h0v, Float64(γ), lv, dm, og, bcs,
+                 eltype === Float64 ? VH_F64 : VH_F32))
+    n = Int(ccall((:vh_blocks_ncell, libvespahip), Int64, (Ptr{Cvoid},), h[]))
+    m = HIPBlocks(h[], n, eltype)
+    finalizer(m) do x
+        x.ptr == C_NULL || ccall((:vh_blocks_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    return m
+end
+
+"Upload the SoA conserved state (5 × ncell, field-major) of a block mesh."
+hip_upload!(m::HIPBlocks, U::Vector) = (GC.@preserve U _check(ccall((:vh_blocks_upload, libvespahip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U)); m)
+
+function hip_download(m::HIPBlocks)
+    U = Vector{m.eltype}(undef, 5 * m.ncell)
+    GC.@preserve U _check(ccall((:vh_blocks_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U))
+    return U
+end
+
+"step!(sim, dt; level) (src/driver.jl:333-378); level < 0 updates every leaf."
+hip_step!(m::HIPBlocks, dt::Real; level::Integer = -1) =
+    (_check(ccall((:vh_blocks_step, libvespahip), Cint, (Ptr{Cvoid}, Float64, Int32), m.ptr, Float64(dt),
+                  Int32(level))); m)
+
+"evolve! single-rate or subcycled with flux registers (src/driver.jl:483-508, 553-605; src/reflux.jl:51-103)."
+function hip_evolve!(m::HIPBlocks, cfl::Real, tfinal::Real; t0::Real = 0.0, subcycle::Bool = false,
+                     max_steps::Integer = 0)
+    t = Ref(Float64(t0)); k = Ref{Int64}(0)
+    _check(ccall((:vh_blocks_run, libvespahip), Cint,
+                 (Ptr{Cvoid}, Float64, Float64, Int64, Int32, Ref{Float64}, Ref{Int64}),
+                 m.ptr, Float64(cfl), Float64(tfinal), Int64(max_steps), Int32(subcycle), t, k))
+    return t[], k[]
+end
+
+# ------------------------------------------------------------------ reverse mode
+
+"""
+    hip_step_vjp(g::HIPGrid, U, dt, Ū) -> ∂L/∂U
+
+Pullback of `step!(sim, dt)` with dt held fixed (the body of a ChainRulesCore
+`rrule`): U and Ū are host vectors of 5 × ncell Float64 (field-major).
+"""
+function hip_step_vjp(g::HIPGrid, U::Vector{Float64}, dt::Real, Ū::Vector{Float64})
+    nb = 8 * length(U)
+    dU = Ref{Ptr{Cvoid}}(C_NULL); dW = Ref{Ptr{Cvoid}}(C_NULL); dO = Ref{Ptr{Cvoid}}(C_NULL)
+    for r in (dU, dW, dO)
+        _check(ccall((:vh_buffer_alloc, libvespahip), Cint, (Ref{Ptr{Cvoid}}, Int64), r, nb))
+    end
+    out = similar(U)
+    try
+        GC.@preserve U Ū out begin
+            _check(ccall((:vh_buffer_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), dU[], U, nb))
+            _check(ccall((:vh_buffer_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), dW[], Ū, nb))
+            _check(ccall((:vh_grid_step_vjp, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ptr{Cvoid}),
+                         g.ptr, dU[], Float64(dt), dW[], dO[]))
+            _check(ccall((:vh_buffer_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), out, dO[], nb))
+        end
+    finally
+        for r in (dU, dW, dO)
+            ccall((:vh_buffer_free, libvespahip), Cint, (Ptr{Cvoid},), r[])
+        end
+    end
+    return out
+end
+
+# The rrule a maintainer registers with ChainRulesCore (not loaded here):
+#   function ChainRulesCore.rrule(::typeof(hip_step_state), g::HIPGrid, U, dt)
+#       Unew = hip_step_state(g, U, dt)
+#       return Unew, Ū -> (NoTangent(), NoTangent(), hip_step_vjp(g, U, dt, Ū), NoTangent())
+#   end
+
+export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
+       HIPBlocks, hip_download, hip_step_vjp
 
 end # module
