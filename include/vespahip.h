@@ -122,6 +122,15 @@ int vh_grid_compute_dt(vh_grid* g, double cfl, double* dt);
  * Stream-ordered; does not synchronise. */
 int vh_grid_step(vh_grid* g, double dt);
 
+/* step!(sim, dt; bflux) (src/driver.jl:333-359 with src/reflux.jl:105-132):
+ * one step that also adds ½dt·F·area of every domain-boundary face, per RK
+ * stage, to the device register dbflux (5 values of the state dtype per slot,
+ * vh_grid_bflux_count slots: planes ordered axis 0 lo, axis 0 hi, axis 1 lo, ...,
+ * each column-major over the other axes; periodic planes stay untouched). The
+ * register accumulates; the caller zeroes it. Stream-ordered. */
+int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux);
+int64_t vh_grid_bflux_count(const vh_grid* g);
+
 /* evolve!-style loop with the CFL dt computed on the device each step
  * (src/driver.jl:493-506). tfinal > 0: run while t < tfinal*(1-1e-12) with
  * dt = min(cfl/invdt, tfinal - t), synchronising once per step to test t;

@@ -8,7 +8,8 @@ load it with ``importlib`` under the module name ``vespa_jl_amd`` (as the tests,
 """
 from ._lib import (DeviceBuffer, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_EXACT, VH_F32, VH_F64, VH_FAST,
                    VespaHIPError, check, device_count, load_library)
-from .hydro import (HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
+from .hydro import (BoundaryFluxRegister, HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
+                    bflux_count, bflux_plane,
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
                     dump_fields, evolve, prim2cons_array, run_steps, sod_problem, step)
 
@@ -16,6 +17,7 @@ __all__ = [
     "DeviceBuffer",
     "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_EXACT", "VH_F32", "VH_F64", "VH_FAST",
     "VespaHIPError", "check", "device_count", "load_library",
+    "BoundaryFluxRegister", "bflux_count", "bflux_plane",
     "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",
     "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",
     "dump_fields", "evolve", "prim2cons_array", "run_steps", "sod_problem", "step",

@@ -32,6 +32,7 @@ EXPORTED = (
     "vh_blocks_info", "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling",
     "vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
     "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form",
+    "vh_grid_step_bflux", "vh_grid_bflux_count",
 )
 
 
@@ -130,6 +131,10 @@ def load_library(path: str | None = None):
     L.vh_adjoint_backward.argtypes = [vp, vp, vp]
     L.vh_adjoint_info.argtypes = [vp, dp, dp, i64p, dp]
     L.vh_grid_set_adjoint_form.argtypes = [vp, C.c_int32]
+    L.vh_grid_step_bflux.argtypes = [vp, C.c_double, vp]
+    L.vh_grid_step_bflux.restype = C.c_int
+    L.vh_grid_bflux_count.argtypes = [vp]
+    L.vh_grid_bflux_count.restype = C.c_int64
     for n in ("vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
               "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form"):
         getattr(L, n).restype = C.c_int

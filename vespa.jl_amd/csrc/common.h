@@ -46,6 +46,7 @@ struct GridDev {
   double V;        // cell volume (geometry precision)
   double w[3];     // cell widths
   double area[3];  // face areas
+  void* bflux;     // boundary-flux register (5 Tf per boundary face slot) or null
 };
 
 // ---- batched blocks (block-structured AMR) ----

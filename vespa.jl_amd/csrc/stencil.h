@@ -136,6 +136,25 @@ __device__ __forceinline__ Cons<T> axis_face(int gpos, int n, int bclo, int bchi
   return hllc(L, R, g, igm1, dim);
 }
 
+// Boundary-flux register (src/reflux.jl:105-132): b[slot] += (s·F)·area for a
+// domain-boundary face, s = ½dt per RK stage. Slots: planes ordered (axis 0 lo,
+// axis 0 hi, axis 1 lo, ...), each column-major over the other axes; idx is the
+// face's index in its plane. Each boundary face is evaluated by one lane per
+// stage, so the read-modify-write needs no atomics.
+template <typename T>
+__device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, long long idx, const Cons<T>& F,
+                                          T s, T a) {
+  long long base = 0;
+  for (int e = 0; e < axis; ++e) base += 2 * (G.ncell / G.n[e]);
+  if (side) base += G.ncell / G.n[axis];
+  T* b = (T*)G.bflux + 5 * (base + idx);
+  b[0] = b[0] + (s * F.d) * a;
+  b[1] = b[1] + (s * F.mx) * a;
+  b[2] = b[2] + (s * F.my) * a;
+  b[3] = b[3] + (s * F.mz) * a;
+  b[4] = b[4] + (s * F.e) * a;
+}
+
 // One RK stage. STAGE2 = false: Uout = Us − (dt/V)·acc(Us).
 // STAGE2 = true: Uout = ½·U0 + ½·(Us − (dt/V)·acc(Us)) written over U0 (point
 // reads only), plus the per-workgroup max of the CFL term of the new state.
@@ -199,6 +218,7 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
     if (bz != BC_PERIODIC && kz0 == 0) {
       const Prim<T> R = face_value(wz0, sz0, T(-0.5));
       fzlo = hllc(ghost(R, bz, 3), R, g, igm1, 3);
+      if (G.bflux && active) bflux_add(G, 2, 0, (long long)gy * nx + gx, fzlo, T(0.5 * dt), a2);
     } else {
       fzlo = hllc(face_value(wm1, sm1, T(0.5)), face_value(wz0, sz0, T(-0.5)), g, igm1, 3);
     }
@@ -316,6 +336,8 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
       VH_GET5(L, S.xh, [ty][tx]);
       const Cons<T> F = axis_face(gx, nx, bx, G.bc[1], L, xlo, g, igm1, 1);
       VH_PUTF(S.xh, [ty][tx], F);
+      if (G.bflux && bx != BC_PERIODIC && (gx == 0 || gx == nx))
+        bflux_add(G, 0, gx == nx, (long long)k * ny + gy, F, T(0.5 * dt), a0);
     }
     if constexpr (RANK > 1) {
       if (gx < nx && gy <= ny) {
@@ -323,6 +345,8 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
         VH_GET5(L, S.yh, [ty][tx]);
         const Cons<T> F = axis_face(gy, ny, by, G.bc[3], L, ylo, g, igm1, 2);
         VH_PUTF(S.yh, [ty][tx], F);
+        if (G.bflux && by != BC_PERIODIC && (gy == 0 || gy == ny))
+          bflux_add(G, 1, gy == ny, (long long)k * nx + gx, F, T(0.5 * dt), a1);
       }
     }
     // Tile hi-edge faces: x face TX of each row, y face TY of each column.
@@ -338,6 +362,8 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
           VH_GET5(R, S.xl, [r]);
           const Cons<T> F = axis_face(gp, nx, bx, G.bc[1], L, R, g, igm1, 1);
           VH_PUTF(S.xh, [r][TX], F);
+          if (G.bflux && bx != BC_PERIODIC && gp == nx)
+            bflux_add(G, 0, 1, (long long)k * ny + y0 + r, F, T(0.5 * dt), a0);
         }
       } else {
         const int cc = h - NEX;
@@ -348,6 +374,8 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
           VH_GET5(R, S.yl, [cc]);
           const Cons<T> F = axis_face(gp, ny, by, G.bc[3], L, R, g, igm1, 2);
           VH_PUTF(S.yh, [TY][cc], F);
+          if (G.bflux && by != BC_PERIODIC && gp == ny)
+            bflux_add(G, 1, 1, (long long)k * nx + x0 + cc, F, T(0.5 * dt), a1);
         }
       }
     }
@@ -359,9 +387,10 @@ __global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T*
       sz1 = slope(wz0, wz1, wz2);
       const int bz = G.bc[4];
       const Prim<T> L = face_value(wz0, sz0, T(0.5));
-      if (bz != BC_PERIODIC && k == nz - 1)
+      if (bz != BC_PERIODIC && k == nz - 1) {
         fzhi = hllc(L, ghost(L, G.bc[5], 3), g, igm1, 3);
-      else
+        if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
+      } else
         fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
     }
     __syncthreads();

@@ -464,6 +464,24 @@ int vh_grid_step(vh_grid* g, double dt) {
   return 0;
 }
 
+int64_t vh_grid_bflux_count(const vh_grid* g) {
+  if (!g) return -1;
+  long long t = 0;
+  for (int d = 0; d < g->G.rank; ++d) t += 2 * (g->G.ncell / g->G.n[d]);
+  return t;
+}
+
+int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux) {
+  if (!g || !dbflux) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, dt);
+  g->G.bflux = dbflux;
+  enqueue_step(g);
+  g->G.bflux = nullptr;
+  VH_HIP(hipGetLastError());
+  return 0;
+}
+
 int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double* t_io, int64_t* steps) {
   if (!g || !t_io) return fail(-1, "null argument%s", "");
   const bool clip = tfinal > 0.0;

@@ -346,12 +346,78 @@ def compute_dt(sim: Simulation) -> float:
     return dt.value
 
 
-def step(sim: Simulation, dt: float) -> Simulation:
-    """One SSP-RK2 step (src/driver.jl:333-359); advances sim.t and the step count."""
-    check(sim._lib.vh_grid_step(sim._h, float(dt)))
+def step(sim: Simulation, dt: float, bflux: "BoundaryFluxRegister | None" = None) -> Simulation:
+    """One SSP-RK2 step (src/driver.jl:333-359); advances sim.t and the step
+    count. With a BoundaryFluxRegister, ½dt·F·area of every domain-boundary face
+    is added to it per RK stage (src/reflux.jl:105-132)."""
+    if bflux is None:
+        check(sim._lib.vh_grid_step(sim._h, float(dt)))
+    else:
+        if bflux.flux.dtype != sim.eltype or bflux.flux.shape != (bflux_count(sim), 5):
+            raise ValueError("register does not match this simulation")
+        buf = _lib.DeviceBuffer(bflux.flux.nbytes).upload(bflux.flux)
+        check(sim._lib.vh_grid_step_bflux(sim._h, float(dt), buf.ptr))
+        buf.download(bflux.flux)
+        buf.free()
     sim.t += dt
     sim.step_count += 1
     return sim
+
+
+def bflux_count(sim: Simulation) -> int:
+    return int(sim._lib.vh_grid_bflux_count(sim._h))
+
+
+class BoundaryFluxRegister:
+    """Time-integrated flux ∫F·area dt through each domain-boundary face
+    (src/reflux.jl:105-122), dense: one row of 5 values per boundary face slot,
+    planes ordered (axis 1 lo, axis 1 hi, axis 2 lo, ...), each column-major over
+    the other axes (dim 1 fastest)."""
+
+    def __init__(self, sim: Simulation):
+        self.dims = tuple(sim.mesh.dims)
+        self.flux = np.zeros((bflux_count(sim), 5), dtype=sim.eltype)
+
+    def plane(self, axis: int, side: str) -> np.ndarray:
+        """The (axis, side) plane (axis 1-based, side :lo/:hi as "lo"/"hi") as an
+        array shaped like the face plane (other axes, dim 1 fastest) × 5."""
+        rank = len(self.dims)
+        n = int(np.prod(self.dims))
+        base = sum(2 * (n // self.dims[e]) for e in range(axis - 1))
+        size = n // self.dims[axis - 1]
+        if side == "hi":
+            base += size
+        other = [self.dims[e] for e in range(rank) if e != axis - 1]
+        return self.flux[base:base + size].reshape(tuple(reversed(other)) + (5,) if other else (5,))
+
+    def lookup(self, axis: int, side: str, cell) -> np.ndarray:
+        """Recorded flux of the boundary face of `cell` (1-based CartesianIndex
+        tuple) on (axis, side)."""
+        P = self.plane(axis, side)
+        idx = tuple(int(cell[e]) - 1 for e in range(len(self.dims)) if e != axis - 1)
+        return P[tuple(reversed(idx))] if idx else P
+
+
+def bflux_plane(rank: int, dim: int, start, stop, g0, flux_off: int, comp: int, Vcell: float, lookup) -> np.ndarray:
+    """Rasterize one (dim, side) face plane in Enzo's flat layout
+    (src/reflux.jl:169-191): column-major over the orthogonal dims (dim 0
+    fastest), the flux dim collapsed; each entry is the recorded flux of
+    component `comp` (1-based, 0 → zeros) divided by Vcell. `lookup(I)` returns
+    the recorded 5-vector for the active cell I (1-based tuple) or None."""
+    Dim = [stop[d] - start[d] + 1 for d in range(rank)]
+    n = int(np.prod(Dim))
+    out = np.empty(n)
+    for lin in range(n):
+        rem = lin
+        active = []
+        for d in range(rank):
+            od = rem % Dim[d]
+            rem //= Dim[d]
+            g = start[d] + od
+            active.append(flux_off if d == dim else g - g0[d] + 1)
+        F = lookup(tuple(active))
+        out[lin] = float(F[comp - 1]) / float(Vcell) if (F is not None and comp > 0) else 0.0
+    return out
 
 
 def run_steps(sim: Simulation, nsteps: int) -> Simulation:
