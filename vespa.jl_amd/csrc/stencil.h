@@ -37,6 +37,10 @@
 #ifndef VH_MIN_WAVES
 #define VH_MIN_WAVES 3
 #endif
+// Float32 state: fewer registers per value, so more waves can be resident.
+#ifndef VH_MIN_WAVES_F32
+#define VH_MIN_WAVES_F32 4
+#endif
 // 1: issue the loads of the cell's own state at the start of the plane (more
 // latency hidden, more registers live across the plane); 0: load it in the
 // update phase.
@@ -159,7 +163,7 @@ __device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, 
 // STAGE2 = true: Uout = ½·U0 + ½·(Us − (dt/V)·acc(Us)) written over U0 (point
 // reads only), plus the per-workgroup max of the CFL term of the new state.
 template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64>
-__global__ void __launch_bounds__(TX* TY, VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
+__global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
                                                           T* Uout, const double* __restrict__ dtp,
                                                           double* __restrict__ partials, GridDev G) {
   constexpr int NT = TX * TY;
