@@ -408,6 +408,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             any = true;
           }
         if (!any) continue;
+        if ((long long)H.lcmap.size() + ps > 0x7fffffffLL) return fail(-1, "mesh too large%s", "");
         bk.lcoff[slot] = (int)H.lcmap.size();
         H.lcmap.insert(H.lcmap.end(), pm.begin(), pm.end());
         bk.haslinks = 1;
