@@ -127,7 +127,8 @@ int vh_grid_step(vh_grid* g, double dt);
  * stage, to the device register dbflux (5 values of the state dtype per slot,
  * vh_grid_bflux_count slots: planes ordered axis 0 lo, axis 0 hi, axis 1 lo, ...,
  * each column-major over the other axes; periodic planes stay untouched). The
- * register accumulates; the caller zeroes it. Stream-ordered. */
+ * register accumulates; the caller zeroes it. Returns after the step has
+ * completed, so the register can be read at once. */
 int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux);
 int64_t vh_grid_bflux_count(const vh_grid* g);
 

@@ -31,17 +31,18 @@ namespace VH_PROFILE_NS {
 
 constexpr int BNT = 256;     // lanes per workgroup
 constexpr int BW = 432;      // max (TY+4)*(TX+4) over the tile shapes used
-constexpr int BF = 288;      // max (TX+1)*TY and TX*(TY+1)
-constexpr int BE = 32;       // max TX and TY
+constexpr int BFS = 592;     // max TY*(TX+2) + (TY+2)*TX (TX, TY <= 32, TX*TY <= 256)
 
+// Face slots of a TX×TY tile, one array per field (as PlaneLDS in stencil.h):
+//   x slot (r, s) = r*(TX+2) + s, s = 0..TX: hi face state of cell s-1 of row r,
+//     then the flux of x face s; s = TX+1: lo face state of halo cell TX;
+//   y slot (r, c) = TY*(TX+2) + r*TX + c, r = 0..TY: hi face state of cell r-1
+//     of column c, then the flux of y face r; r = TY+1: lo state of halo cell TY.
+// The final workgroup reduction reuses the start of w (after a barrier).
 template <typename T>
-struct BlockLDS {
+struct alignas(16) BlockLDS {
   T w[5][BW];
-  T xh[5][BF];
-  T yh[5][BF];
-  T xl[5][BE];
-  T yl[5][BE];
-  double red[16];
+  T f[5][BFS];
 };
 
 template <typename T>
@@ -143,26 +144,6 @@ __device__ __forceinline__ void add_link_group(Cons<T>& acc, const BlocksDev& B,
   }
 }
 
-template <typename T>
-__device__ __forceinline__ Prim<T> lds_face_state(const T (*a)[BF], int i) {
-  Prim<T> w;
-  w.r = a[0][i]; w.vx = a[1][i]; w.vy = a[2][i]; w.vz = a[3][i]; w.p = a[4][i];
-  return w;
-}
-template <typename T>
-__device__ __forceinline__ Cons<T> lds_flux(const T (*a)[BF], int i) {
-  Cons<T> f;
-  f.d = a[0][i]; f.mx = a[1][i]; f.my = a[2][i]; f.mz = a[3][i]; f.e = a[4][i];
-  return f;
-}
-template <typename T>
-__device__ __forceinline__ void lds_put_flux(T (*a)[BF], int i, const Cons<T>& f) {
-  a[0][i] = f.d; a[1][i] = f.mx; a[2][i] = f.my; a[3][i] = f.mz; a[4][i] = f.e;
-}
-template <typename T>
-__device__ __forceinline__ void lds_put_state(T (*a)[BF], int i, const Prim<T>& w) {
-  a[0][i] = w.r; a[1][i] = w.vx; a[2][i] = w.vy; a[3][i] = w.vz; a[4][i] = w.p;
-}
 
 // One RK stage over a list of tiles (persistent: workgroup w takes tiles w,
 // w + gridDim.x, ...). STAGE2 as in rk_stage_kernel; the CFL partial of the
@@ -181,6 +162,7 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
   double cfl_max = 0.0;
 
   for (int t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
+    CflMax<T, RANK> cfl;   // this tile's CFL numerators (one block: one set of widths)
     const TileDev tl = B.tiles[t];
     const BlockDev* __restrict__ bk = B.blocks + tl.block;
     BlockGeo bg;
@@ -258,9 +240,12 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
         }
         bl_putw(S, (ty + 2) * WX + tx + 2, wc);
       }
+      // Halo, rim and edge work goes to lanes counted from a wave that moves
+      // by one per plane, so every SIMD gets its share.
+      const int lt = (tid + 64 * ((k + t) & 3)) & (BNT - 1);
       const int NHX = 4 * TY;
       const int NHY = (RANK > 1) ? 4 * TX : 0;
-      for (int h = tid; h < NHX + NHY; h += BNT) {
+      for (int h = lt; h < NHX + NHY; h += BNT) {
         int lx, ly;
         Prim<T> w;
         if (h < NHX) {
@@ -288,78 +273,73 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
       __syncthreads();
 
       // ---- PA: slopes once per cell; publish hi face states ----
+      const T* wf = &S.w[0][0];
+      T* ff = &S.f[0][0];
+      const int NXS = TY * (TX + 2);
       Prim<T> xlo, ylo;
       if (lane) {
         const int o = (ty + 2) * WX + tx + 2;
-        const Prim<T> sx = slope(bl_getw(S, o - 1), wc, bl_getw(S, o + 1));
+        const Prim<T> sx = slope(lds_get5(wf, BW, o - 1), wc, lds_get5(wf, BW, o + 1));
         xlo = face_value(wc, sx, T(-0.5));
-        lds_put_state(S.xh, ty * (TX + 1) + tx + 1, face_value(wc, sx, T(0.5)));
+        lds_put5(ff, BFS, ty * (TX + 2) + tx + 1, face_value(wc, sx, T(0.5)));
         if constexpr (RANK > 1) {
-          const Prim<T> sy = slope(bl_getw(S, o - WX), wc, bl_getw(S, o + WX));
+          const Prim<T> sy = slope(lds_get5(wf, BW, o - WX), wc, lds_get5(wf, BW, o + WX));
           ylo = face_value(wc, sy, T(-0.5));
-          lds_put_state(S.yh, (ty + 1) * TX + tx, face_value(wc, sy, T(0.5)));
+          lds_put5(ff, BFS, NXS + (ty + 1) * TX + tx, face_value(wc, sy, T(0.5)));
         }
       }
+      // Halo rim through one code path: x hi state of cell -1 and lo state of
+      // cell TX per row, y hi state of cell -1 and lo state of cell TY per column.
       const int NRX = 2 * TY;
       const int NRY = (RANK > 1) ? 2 * TX : 0;
-      for (int h = tid; h < NRX + NRY; h += BNT) {
-        if (h < NRX) {
-          const int r = h >> 1;
-          const int base = (r + 2) * WX;
-          if ((h & 1) == 0) {
-            const Prim<T> wm = bl_getw(S, base + 1);
-            lds_put_state(S.xh, r * (TX + 1), face_value(wm, slope(bl_getw(S, base), wm, bl_getw(S, base + 2)), T(0.5)));
-          } else {
-            const Prim<T> wp = bl_getw(S, base + TX + 2);
-            const Prim<T> v = face_value(wp, slope(bl_getw(S, base + TX + 1), wp, bl_getw(S, base + TX + 3)), T(-0.5));
-            S.xl[0][r] = v.r; S.xl[1][r] = v.vx; S.xl[2][r] = v.vy; S.xl[3][r] = v.vz; S.xl[4][r] = v.p;
-          }
+      for (int h = lt; h < NRX + NRY; h += BNT) {
+        const bool isy = h >= NRX;
+        const int q = isy ? h - NRX : h;
+        const int line = q >> 1;
+        const bool hi = q & 1;
+        int ic, dst;
+        if (!isy) {
+          ic = (line + 2) * WX + (hi ? TX + 2 : 1);
+          dst = line * (TX + 2) + (hi ? TX + 1 : 0);
         } else {
-          const int hh = h - NRX;
-          const int cc = hh >> 1;
-          if ((hh & 1) == 0) {
-            const Prim<T> wm = bl_getw(S, WX + cc + 2);
-            lds_put_state(S.yh, cc, face_value(wm, slope(bl_getw(S, cc + 2), wm, bl_getw(S, 2 * WX + cc + 2)), T(0.5)));
-          } else {
-            const Prim<T> wp = bl_getw(S, (TY + 2) * WX + cc + 2);
-            const Prim<T> v = face_value(
-                wp, slope(bl_getw(S, (TY + 1) * WX + cc + 2), wp, bl_getw(S, (TY + 3) * WX + cc + 2)), T(-0.5));
-            S.yl[0][cc] = v.r; S.yl[1][cc] = v.vx; S.yl[2][cc] = v.vy; S.yl[3][cc] = v.vz; S.yl[4][cc] = v.p;
-          }
+          ic = (hi ? TY + 2 : 1) * WX + line + 2;
+          dst = NXS + (hi ? TY + 1 : 0) * TX + line;
         }
+        const int d = isy ? WX : 1;
+        const Prim<T> wm = lds_get5(wf, BW, ic);
+        const Prim<T> sl = slope(lds_get5(wf, BW, ic - d), wm, lds_get5(wf, BW, ic + d));
+        lds_put5(ff, BFS, dst, face_value(wm, sl, hi ? T(-0.5) : T(0.5)));
       }
       __syncthreads();
 
       // ---- PB: face fluxes (each face of the tile once) ----
       if (lane && gy < ny && gx <= nx) {
-        const int ix = ty * (TX + 1) + tx;
-        lds_put_flux(S.xh, ix, axis_face(gx, nx, bxl, bxh, lds_face_state(S.xh, ix), xlo, g, igm1, 1));
+        const int ix = ty * (TX + 2) + tx;
+        lds_putf(ff, BFS, ix, axis_face<false>(gx, nx, bxl, bxh, lds_get5(ff, BFS, ix), xlo, g, igm1, 1));
       }
       if constexpr (RANK > 1) {
         if (lane && gx < nx && gy <= ny) {
-          const int iy = ty * TX + tx;
-          lds_put_flux(S.yh, iy, axis_face(gy, ny, byl, byh, lds_face_state(S.yh, iy), ylo, g, igm1, 2));
+          const int iy = NXS + ty * TX + tx;
+          lds_putf(ff, BFS, iy, axis_face<false>(gy, ny, byl, byh, lds_get5(ff, BFS, iy), ylo, g, igm1, 2));
         }
       }
+      // Tile hi-edge faces (x face TX of each row, y face TY of each column)
+      // through one code path, a y face in the frame of an x face.
       const int NEX = TY;
       const int NEY = (RANK > 1) ? TX : 0;
-      for (int h = tid; h < NEX + NEY; h += BNT) {
-        if (h < NEX) {
-          const int r = h, gp = x0 + TX;
-          if (y0 + r < ny && gp <= nx) {
-            const int ix = r * (TX + 1) + TX;
-            Prim<T> R;
-            R.r = S.xl[0][r]; R.vx = S.xl[1][r]; R.vy = S.xl[2][r]; R.vz = S.xl[3][r]; R.p = S.xl[4][r];
-            lds_put_flux(S.xh, ix, axis_face(gp, nx, bxl, bxh, lds_face_state(S.xh, ix), R, g, igm1, 1));
-          }
-        } else {
-          const int cc = h - NEX, gp = y0 + TY;
-          if (x0 + cc < nx && gp <= ny) {
-            const int iy = TY * TX + cc;
-            Prim<T> R;
-            R.r = S.yl[0][cc]; R.vx = S.yl[1][cc]; R.vy = S.yl[2][cc]; R.vz = S.yl[3][cc]; R.p = S.yl[4][cc];
-            lds_put_flux(S.yh, iy, axis_face(gp, ny, byl, byh, lds_face_state(S.yh, iy), R, g, igm1, 2));
-          }
+      for (int h = lt; h < NEX + NEY; h += BNT) {
+        const bool isy = h >= NEX;
+        const int line = isy ? h - NEX : h;
+        const int gp = isy ? y0 + TY : x0 + TX;
+        const int na = isy ? ny : nx;
+        const bool inside = isy ? (x0 + line < nx) : (y0 + line < ny);
+        if (inside && gp <= na) {
+          const int sl = isy ? NXS + TY * TX + line : line * (TX + 2) + TX;
+          const int sr = isy ? sl + TX : sl + 1;
+          const Prim<T> L = swap_xy(lds_get5(ff, BFS, sl), isy);
+          const Prim<T> R = swap_xy(lds_get5(ff, BFS, sr), isy);
+          lds_putf(ff, BFS, sl,
+                   swap_xy(axis_face<false>(gp, na, isy ? byl : bxl, isy ? byh : bxh, L, R, g, igm1, 1), isy));
         }
       }
 
@@ -380,13 +360,13 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
         const Cons<T> uown = load_cons(Us, n, c);
         Cons<T> u0own;
         if constexpr (STAGE2) u0own = load_cons(U0, n, c);
-        const int ix = ty * (TX + 1) + tx;
-        const int iy = ty * TX + tx;
+        const int ix = ty * (TX + 2) + tx;
+        const int iy = NXS + ty * TX + tx;
         auto flux = [&](int s) -> Cons<T> {
-          if (s == 0) return lds_flux(S.xh, ix);
-          if (s == 1) return lds_flux(S.xh, ix + 1);
-          if (s == 2) return lds_flux(S.yh, iy);
-          if (s == 3) return lds_flux(S.yh, iy + TX);
+          if (s == 0) return lds_getf(ff, BFS, ix);
+          if (s == 1) return lds_getf(ff, BFS, ix + 1);
+          if (s == 2) return lds_getf(ff, BFS, iy);
+          if (s == 3) return lds_getf(ff, BFS, iy + TX);
           if (s == 4) return fzlo;
           return fzhi;
         };
@@ -479,12 +459,7 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
           o.my = hh * u0own.my + hh * (uown.my - invV * acc.my);
           o.mz = hh * u0own.mz + hh * (uown.mz - invV * acc.mz);
           o.e = hh * u0own.e + hh * (uown.e - invV * acc.e);
-          const Prim<T> W = cons2prim<T, FLOOR64>(o, g, B.gamma);
-          const T cs = vsqrt(g * W.p / W.r);
-          double m = double(vabs(W.vx) + cs) / bk->w[0];
-          if (RANK > 1) m = vmax(m, double(vabs(W.vy) + cs) / bk->w[1]);
-          if (RANK > 2) m = vmax(m, double(vabs(W.vz) + cs) / bk->w[2]);
-          cfl_max = vmax(cfl_max, m);
+          cfl.add(cons2prim<T, FLOOR64>(o, g, B.gamma), g);
         }
         store_cons(Uout, n, c, o);
       }
@@ -501,10 +476,11 @@ __global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T*
         }
       }
     }
+    if constexpr (STAGE2) cfl_max = vmax(cfl_max, cfl.value(bk->w));
   }
 
   if constexpr (STAGE2) {
-    const double m = block_max<BNT>(cfl_max, S.red);
+    const double m = block_max<BNT>(cfl_max, reinterpret_cast<double*>(&S.w[0][0]));
     if (tid == 0) partials[blockIdx.x] = m;
   }
 }

@@ -36,7 +36,7 @@ namespace vh {
 enum { NB_SAME = 0, NB_COARSER = 1, NB_FINER = 2, NB_BOUNDARY = 3 };
 
 // Tile limits of block_stage_kernel (BNT lanes, LDS sized for these bounds).
-static const int kLanes = 256, kMaxW = 432, kMaxF = 288, kMaxE = 32;
+static const int kLanes = 256, kMaxW = 432, kMaxFS = 592, kMaxE = 32;
 
 struct Topo {
   int rank = 0, nb = 0, maxlevel = 0;
@@ -463,7 +463,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
       if (maxty > kMaxE) maxty = kMaxE;
       ty = balanced(bk.n[1], maxty);
     }
-    if (tx * ty > kLanes || (tx + 4) * (ty + 4) > kMaxW || (tx + 1) * ty > kMaxF || tx * (ty + 1) > kMaxF ||
+    if (tx * ty > kLanes || (tx + 4) * (ty + 4) > kMaxW || ty * (tx + 2) + (rank > 1 ? (ty + 2) * tx : 0) > kMaxFS ||
         (rank > 1 && tx > kMaxE) || ty > kMaxE)
       return fail(-1, "no tile shape fits block %s%lld", "", b);
     int zc = rank == 3 ? (zchunk > 0 ? zchunk : bk.n[2]) : 1;

@@ -47,6 +47,7 @@ struct GridDev {
   double w[3];     // cell widths
   double area[3];  // face areas
   void* bflux;     // boundary-flux register (5 Tf per boundary face slot) or null
+  long long bbase[6];  // first slot of each boundary plane (axis 0 lo, axis 0 hi, axis 1 lo, ...)
 };
 
 // ---- batched blocks (block-structured AMR) ----

@@ -62,9 +62,10 @@ __device__ __forceinline__ T fast_div(T a, T b) {
   const T q = a * r;
   return vfma(vfma(-b, q, a), r, q);
 }
-// sqrt(x/y) for positive x, y: x·rsqrt(x·y) with two Newton steps on the rsqrt.
+// sqrt(x/y) for x >= 0, y > 0: x·rsqrt(x·y) with two Newton steps on the rsqrt
+// (x·y is kept at or above the smallest normal so that x = 0 gives 0).
 __device__ __forceinline__ double fast_sqrt_ratio(double x, double y) {
-  const double a = x * y;
+  const double a = vmax(x * y, 2.2250738585072014e-308);
   double s = __builtin_amdgcn_rsq(a);
   const double h = 0.5 * a;
   s = s * vfma(-h * s, s, 1.5);
@@ -72,7 +73,7 @@ __device__ __forceinline__ double fast_sqrt_ratio(double x, double y) {
   return x * s;
 }
 __device__ __forceinline__ float fast_sqrt_ratio(float x, float y) {
-  const float a = x * y;
+  const float a = vmax(x * y, 1.17549435e-38f);
   float s = __builtin_amdgcn_rsqf(a);
   const float h = 0.5f * a;
   s = s * vfma(-h * s, s, 1.5f);
@@ -123,19 +124,20 @@ __device__ __forceinline__ T energy(const Prim<T>& W, T g, T igm1) {
 #endif
 }
 
+#if VH_FAST_PROFILE
+// minmod(a, b) is the median of (a, b, 0): the smaller-magnitude argument when
+// the signs agree, else 0. Float64: max(min(a,b), min(max(a,b), 0)), four
+// min/max; Float32: one v_med3_f32. Differs from the reference form only when
+// a·b underflows to zero with both signs equal (a subnormal-scale slope).
+__device__ __forceinline__ double minmod(double a, double b) { return vmax(vmin(a, b), vmin(vmax(a, b), 0.0)); }
+__device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.0f); }
+#else
 template <typename T>
 __device__ __forceinline__ T minmod(T a, T b) {
-#if VH_FAST_PROFILE
-  // max(min(a,b),0) + min(max(a,b),0): the smaller-magnitude argument when the
-  // signs agree, else 0 — four min/max and one add instead of a product, two
-  // compares and two 64-bit selects. Differs from the reference form only when
-  // a·b underflows to zero with both signs equal (a subnormal-scale slope).
-  return vmax(vmin(a, b), T(0)) + vmin(vmax(a, b), T(0));
-#else
   if (a * b <= T(0)) return T(0);
   return vabs(a) < vabs(b) ? a : b;
-#endif
 }
+#endif
 
 // limited_slope(l, c, r) = minmod(c - l, r - c) per primitive component.
 template <typename T>
@@ -149,15 +151,25 @@ __device__ __forceinline__ Prim<T> slope(const Prim<T>& l, const Prim<T>& c, con
   return s;
 }
 
-// _face_value: W + h·slope with h = ±½ (src/driver.jl:227-233).
+// _face_value: W + h·slope with h = ±½ (src/driver.jl:227-233). The fast
+// profile states the fused form so that the two faces of a cell cost one FMA
+// each rather than a shared product and two adds.
 template <typename T>
 __device__ __forceinline__ Prim<T> face_value(const Prim<T>& w, const Prim<T>& s, T h) {
   Prim<T> o;
+#if VH_FAST_PROFILE
+  o.r = vfma(h, s.r, w.r);
+  o.vx = vfma(h, s.vx, w.vx);
+  o.vy = vfma(h, s.vy, w.vy);
+  o.vz = vfma(h, s.vz, w.vz);
+  o.p = vfma(h, s.p, w.p);
+#else
   o.r = w.r + h * s.r;
   o.vx = w.vx + h * s.vx;
   o.vy = w.vy + h * s.vy;
   o.vz = w.vz + h * s.vz;
   o.p = w.p + h * s.p;
+#endif
   return o;
 }
 

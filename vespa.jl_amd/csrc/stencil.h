@@ -47,29 +47,127 @@
 #ifndef VH_EARLY_OWN
 #define VH_EARLY_OWN 0
 #endif
+// 1: load the halo cells of the next plane while the current one is processed
+// (tiles whose halo fits one cell per lane); 0: load them when they are staged.
+#ifndef VH_HALO_PREFETCH
+#define VH_HALO_PREFETCH 0
+#endif
 
 namespace vh {
 namespace VH_PROFILE_NS {
 
-// Per-cell CFL term of compute_dt (src/driver.jl:178-185), Tr = double.
+// Running maximum of the CFL term of compute_dt (src/driver.jl:178-185):
+// max over cells and axes of Tr(|v_d| + c)/w_d with Tr = double. The per-axis
+// numerators are reduced first and divided by the width once: for w > 0 a
+// correctly rounded x ↦ x/w is monotone, so max_i(a_i/w) = (max_i a_i)/w
+// exactly, and the result equals the per-cell form bit for bit. The sound
+// speed is the profile's (exact: sqrt((γp)/ρ); fast: the refined rsqrt form).
 template <typename T, int RANK>
-__device__ __forceinline__ double cfl_term(const Prim<T>& W, T g, const GridDev& G) {
-  const T c = vsqrt(g * W.p / W.r);
-  double m = double(vabs(W.vx) + c) / G.w[0];
-  if (RANK > 1) m = vmax(m, double(vabs(W.vy) + c) / G.w[1]);
-  if (RANK > 2) m = vmax(m, double(vabs(W.vz) + c) / G.w[2]);
-  return m;
+struct CflMax {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  __device__ __forceinline__ void add(const Prim<T>& W, T g) {
+    const T c = sound(g, W.p, W.r);
+    a0 = vmax(a0, double(vabs(W.vx) + c));
+    if (RANK > 1) a1 = vmax(a1, double(vabs(W.vy) + c));
+    if (RANK > 2) a2 = vmax(a2, double(vabs(W.vz) + c));
+  }
+  __device__ __forceinline__ double value(const double* w) const {
+    double m = a0 / w[0];
+    if (RANK > 1) m = vmax(m, a1 / w[1]);
+    if (RANK > 2) m = vmax(m, a2 / w[2]);
+    return m;
+  }
+};
+
+// Cell index of position i along an axis of n cells. PER: the axis is periodic
+// (wrap, no ghost) and i is a stencil position, i >= -n; positions from 2n on
+// lie past every cell a face of the grid reads (partial tiles of small grids)
+// and are clamped to a valid cell. Else the generic boundary rule of resolve().
+template <bool PER>
+__device__ __forceinline__ int resolve_axis(int i, int n, int bclo, int& gh) {
+  if constexpr (PER) {
+    gh = 0;
+    i = i < 0 ? i + n : (i >= n ? i - n : i);
+    return i < n ? i : n - 1;
+  } else {
+    return resolve(i, n, bclo, gh);
+  }
 }
 
-template <typename T, int TX, int TY>
-struct PlaneLDS {
-  T w[5][TY + 4][TX + 4];   // primitive state, halo 2 in x and y
-  T xh[5][TY][TX + 1];      // x: hi face state of cell s-1 → then flux of x face s
-  T yh[5][TY + 1][TX];      // y: hi face state of cell s-1 → then flux of y face s
-  T xl[5][TY];              // x: lo face state of halo cell TX
-  T yl[5][TX];              // y: lo face state of halo cell TY
-  double red[16];
+// Face slots share one array per field so that the x and y halo-rim work runs
+// through a single code path:
+//   x slot (r, s), s = 0..TX:  hi face state of cell s-1 of row r, then the flux
+//                              of x face s; s = TX+1: lo face state of halo cell TX;
+//   y slot (r, c), r = 0..TY:  hi face state of cell r-1 of column c, then the
+//                              flux of y face r; r = TY+1: lo face state of halo cell TY.
+// The workgroup reduction at the end of stage 2 reuses the start of w (it runs
+// after a barrier that follows the last read of w), which keeps the 32×8
+// Float64 tile at 40 KiB: four workgroups per compute unit.
+template <typename T, int TX, int TY, int RANK>
+struct alignas(16) PlaneLDS {
+  static constexpr int HY = RANK > 1 ? 2 : 0;    // halo rows in y
+  static constexpr int WX = TX + 4;              // row pitch of w
+  static constexpr int WS = (TY + 2 * HY) * WX;  // field pitch of w
+  static constexpr int NXS = TY * (TX + 2);
+  static constexpr int NYS = RANK > 1 ? (TY + 2) * TX : 0;
+  static constexpr int FS = NXS + NYS;           // field pitch of f
+  __device__ static constexpr int xs(int r, int sl) { return r * (TX + 2) + sl; }
+  __device__ static constexpr int ys(int r, int c) { return NXS + r * TX + c; }
+  T w[5][TY + 2 * HY][TX + 4];   // primitive state, halo 2 in x (and y)
+  T f[5][FS];                    // face slots
 };
+
+template <typename T>
+__device__ __forceinline__ Prim<T> lds_get5(const T* a, int pitch, int i) {
+  Prim<T> v;
+  v.r = a[i];
+  v.vx = a[pitch + i];
+  v.vy = a[2 * pitch + i];
+  v.vz = a[3 * pitch + i];
+  v.p = a[4 * pitch + i];
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ void lds_put5(T* a, int pitch, int i, const Prim<T>& v) {
+  a[i] = v.r;
+  a[pitch + i] = v.vx;
+  a[2 * pitch + i] = v.vy;
+  a[3 * pitch + i] = v.vz;
+  a[4 * pitch + i] = v.p;
+}
+template <typename T>
+__device__ __forceinline__ Cons<T> lds_getf(const T* a, int pitch, int i) {
+  Cons<T> v;
+  v.d = a[i];
+  v.mx = a[pitch + i];
+  v.my = a[2 * pitch + i];
+  v.mz = a[3 * pitch + i];
+  v.e = a[4 * pitch + i];
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ void lds_putf(T* a, int pitch, int i, const Cons<T>& v) {
+  a[i] = v.d;
+  a[pitch + i] = v.mx;
+  a[2 * pitch + i] = v.my;
+  a[3 * pitch + i] = v.mz;
+  a[4 * pitch + i] = v.e;
+}
+// Exchange the x and y components (a y face evaluated in the frame of an x face).
+template <typename T>
+__device__ __forceinline__ Prim<T> swap_xy(const Prim<T>& w, bool on) {
+  Prim<T> o = w;
+  o.vx = on ? w.vy : w.vx;
+  o.vy = on ? w.vx : w.vy;
+  return o;
+}
+template <typename T>
+__device__ __forceinline__ Cons<T> swap_xy(const Cons<T>& f, bool on) {
+  Cons<T> o = f;
+  o.mx = on ? f.my : f.mx;
+  o.my = on ? f.mx : f.my;
+  return o;
+}
 
 #define VH_PUT5(arr, idx, v) \
   do {                       \
@@ -79,61 +177,33 @@ struct PlaneLDS {
     arr[3] idx = (v).vz;     \
     arr[4] idx = (v).p;      \
   } while (0)
-#define VH_GET5(v, arr, idx) \
-  do {                       \
-    (v).r = arr[0] idx;      \
-    (v).vx = arr[1] idx;     \
-    (v).vy = arr[2] idx;     \
-    (v).vz = arr[3] idx;     \
-    (v).p = arr[4] idx;      \
-  } while (0)
-#define VH_PUTF(arr, idx, f) \
-  do {                       \
-    arr[0] idx = (f).d;      \
-    arr[1] idx = (f).mx;     \
-    arr[2] idx = (f).my;     \
-    arr[3] idx = (f).mz;     \
-    arr[4] idx = (f).e;      \
-  } while (0)
-#define VH_GETF(f, arr, idx) \
-  do {                       \
-    (f).d = arr[0] idx;      \
-    (f).mx = arr[1] idx;     \
-    (f).my = arr[2] idx;     \
-    (f).mz = arr[3] idx;     \
-    (f).e = arr[4] idx;      \
-  } while (0)
-
-template <typename T, int TX, int TY>
-__device__ __forceinline__ Prim<T> lds_w(const PlaneLDS<T, TX, TY>& S, int sy, int sx) {
-  Prim<T> w;
-  VH_GET5(w, S.w, [sy][sx]);
-  return w;
-}
 
 // State of a column cell at plane index kk (any integer) with the z boundary
 // rule applied: periodic wrap, or the ghost of the nearest boundary cell.
-template <typename T, bool FLOOR64>
+template <typename T, bool FLOOR64, bool PER>
 __device__ __forceinline__ Prim<T> column_w(const Cons<T>& u, const GridDev& G, int kk, T g) {
-  int gh;
-  (void)resolve(kk, G.n[2], G.bc[4], gh);
   Prim<T> w = cons2prim<T, FLOOR64>(u, g, G.gamma);
-  if (gh) w = ghost(w, G.bc[4 + gh - 1], 3);
+  if constexpr (!PER) {
+    int gh;
+    (void)resolve(kk, G.n[2], G.bc[4], gh);
+    if (gh) w = ghost(w, G.bc[4 + gh - 1], 3);
+  }
   return w;
 }
+template <bool PER>
 __device__ __forceinline__ long long column_src(const GridDev& G, long long col, int kk) {
   int gh;
-  const int src = resolve(kk, G.n[2], G.bc[4], gh);
+  const int src = resolve_axis<PER>(kk, G.n[2], G.bc[4], gh);
   return col + (long long)src * ((long long)G.n[0] * G.n[1]);
 }
 
 // Flux of the face between cells gpos-1 and gpos along an axis with n cells,
 // given the hi state of gpos-1 (L) and the lo state of gpos (R); applies the
 // domain-boundary rule of _flux_face! (src/driver.jl:255-281).
-template <typename T>
+template <bool PER, typename T>
 __device__ __forceinline__ Cons<T> axis_face(int gpos, int n, int bclo, int bchi, Prim<T> L, Prim<T> R, T g,
                                              T igm1, int dim) {
-  if (bclo != BC_PERIODIC) {
+  if (!PER && bclo != BC_PERIODIC) {
     if (gpos == 0) L = ghost(R, bclo, dim);
     else if (gpos == n) R = ghost(L, bchi, dim);
   }
@@ -143,15 +213,13 @@ __device__ __forceinline__ Cons<T> axis_face(int gpos, int n, int bclo, int bchi
 // Boundary-flux register (src/reflux.jl:105-132): b[slot] += (s·F)·area for a
 // domain-boundary face, s = ½dt per RK stage. Slots: planes ordered (axis 0 lo,
 // axis 0 hi, axis 1 lo, ...), each column-major over the other axes; idx is the
-// face's index in its plane. Each boundary face is evaluated by one lane per
-// stage, so the read-modify-write needs no atomics.
+// face's index in its plane; G.bbase holds each plane's first slot. Each
+// boundary face is evaluated by one lane per stage, so the read-modify-write
+// needs no atomics.
 template <typename T>
 __device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, long long idx, const Cons<T>& F,
                                           T s, T a) {
-  long long base = 0;
-  for (int e = 0; e < axis; ++e) base += 2 * (G.ncell / G.n[e]);
-  if (side) base += G.ncell / G.n[axis];
-  T* b = (T*)G.bflux + 5 * (base + idx);
+  T* b = (T*)G.bflux + 5 * (G.bbase[2 * axis + side] + idx);
   b[0] = b[0] + (s * F.d) * a;
   b[1] = b[1] + (s * F.mx) * a;
   b[2] = b[2] + (s * F.my) * a;
@@ -162,12 +230,15 @@ __device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, 
 // One RK stage. STAGE2 = false: Uout = Us − (dt/V)·acc(Us).
 // STAGE2 = true: Uout = ½·U0 + ½·(Us − (dt/V)·acc(Us)) written over U0 (point
 // reads only), plus the per-workgroup max of the CFL term of the new state.
-template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64>
+// PER: every axis of the grid is periodic (no ghosts, no boundary faces, no
+// boundary-flux register), resolved at compile time.
+template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64, bool PER>
 __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
                                                           T* Uout, const double* __restrict__ dtp,
                                                           double* __restrict__ partials, GridDev G) {
   constexpr int NT = TX * TY;
-  __shared__ PlaneLDS<T, TX, TY> S;
+  __shared__ PlaneLDS<T, TX, TY, RANK> S;
+  using L_ = PlaneLDS<T, TX, TY, RANK>;
   const T g = T(G.gamma);
   const T igm1 = T(G.igm1);
   const int tid = threadIdx.x;
@@ -196,30 +267,30 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
   const bool active = gx < nx && gy < ny;
   // The lane's own column (clamped or wrapped so every lane reads valid memory).
   int ghx, ghy;
-  const int sxc = resolve(gx, nx, G.bc[0], ghx);
-  const int syc = resolve(gy, ny, G.bc[2], ghy);
+  const int sxc = resolve_axis<PER>(gx, nx, G.bc[0], ghx);
+  const int syc = resolve_axis<PER>(gy, ny, G.bc[2], ghy);
   const long long col = (long long)syc * nx + sxc;
 
   // ---- z pipeline prologue ----
   Prim<T> wz0, wz1, wz2, sz0;
   Cons<T> fzlo;
   if constexpr (RANK == 3) {
-    const Cons<T> um2 = load_cons(Us, n, column_src(G, col, kz0 - 2));
-    const Cons<T> um1 = load_cons(Us, n, column_src(G, col, kz0 - 1));
-    const Cons<T> u0 = load_cons(Us, n, column_src(G, col, kz0));
-    const Cons<T> u1 = load_cons(Us, n, column_src(G, col, kz0 + 1));
-    const Cons<T> u2 = load_cons(Us, n, column_src(G, col, kz0 + 2));
-    const Prim<T> wm2 = column_w<T, FLOOR64>(um2, G, kz0 - 2, g);
-    const Prim<T> wm1 = column_w<T, FLOOR64>(um1, G, kz0 - 1, g);
-    wz0 = column_w<T, FLOOR64>(u0, G, kz0, g);
-    wz1 = column_w<T, FLOOR64>(u1, G, kz0 + 1, g);
-    wz2 = column_w<T, FLOOR64>(u2, G, kz0 + 2, g);
+    const Cons<T> um2 = load_cons(Us, n, column_src<PER>(G, col, kz0 - 2));
+    const Cons<T> um1 = load_cons(Us, n, column_src<PER>(G, col, kz0 - 1));
+    const Cons<T> u0 = load_cons(Us, n, column_src<PER>(G, col, kz0));
+    const Cons<T> u1 = load_cons(Us, n, column_src<PER>(G, col, kz0 + 1));
+    const Cons<T> u2 = load_cons(Us, n, column_src<PER>(G, col, kz0 + 2));
+    const Prim<T> wm2 = column_w<T, FLOOR64, PER>(um2, G, kz0 - 2, g);
+    const Prim<T> wm1 = column_w<T, FLOOR64, PER>(um1, G, kz0 - 1, g);
+    wz0 = column_w<T, FLOOR64, PER>(u0, G, kz0, g);
+    wz1 = column_w<T, FLOOR64, PER>(u1, G, kz0 + 1, g);
+    wz2 = column_w<T, FLOOR64, PER>(u2, G, kz0 + 2, g);
     // The slope of the ghost plane kz0-1 is never used when kz0-1 is outside a
     // non-periodic domain: that face is a boundary face (lo state = ghost).
     const Prim<T> sm1 = slope(wm2, wm1, wz0);
     sz0 = slope(wm1, wz0, wz1);
     const int bz = G.bc[4];
-    if (bz != BC_PERIODIC && kz0 == 0) {
+    if (!PER && bz != BC_PERIODIC && kz0 == 0) {
       const Prim<T> R = face_value(wz0, sz0, T(-0.5));
       fzlo = hllc(ghost(R, bz, 3), R, g, igm1, 3);
       if (G.bflux && active) bflux_add(G, 2, 0, (long long)gy * nx + gx, fzlo, T(0.5 * dt), a2);
@@ -228,14 +299,58 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     }
   }
 
-  double cfl_max = 0.0;
+  CflMax<T, RANK> cfl;
+
+  // Halo cells (cross-shaped, two deep in x and y) of a plane: item h of
+  // NHX + NHY, staged by lane h counted from the plane's rotating first wave.
+  constexpr int NHX = 4 * TY;
+  constexpr int NHY = (RANK > 1) ? 4 * TX : 0;
+  constexpr bool PREFETCH = VH_HALO_PREFETCH && (NHX + NHY <= NT);
+  auto lane_item = [&](int kk) { return (tid + 64 * ((kk + (int)blockIdx.x) % (NT / 64))) % NT; };
+  auto halo_pos = [&](int h, int& lx, int& ly) {
+    if (h < NHX) {
+      const int r = h / 4, q = h % 4;
+      ly = r;
+      lx = q < 2 ? q - 2 : TX + (q - 2);
+    } else {
+      const int hh = h - NHX;
+      const int r = hh / TX, q = hh % TX;
+      lx = q;
+      ly = r < 2 ? r - 2 : TY + (r - 2);
+    }
+  };
+  // Stage a halo cell given its conserved state (ghost rule applied here).
+  auto stage_halo = [&](int lx, int ly, const Cons<T>& u) {
+    int hgx, hgy;
+    (void)resolve_axis<PER>(x0 + lx, nx, G.bc[0], hgx);
+    (void)resolve_axis<PER>(y0 + ly, ny, G.bc[2], hgy);
+    Prim<T> w = cons2prim<T, FLOOR64>(u, g, G.gamma);
+    if (hgx) w = ghost(w, G.bc[hgx - 1], 1);
+    if (hgy) w = ghost(w, G.bc[2 + hgy - 1], 2);
+    VH_PUT5(S.w, [ly + L_::HY][lx + 2], w);
+  };
+  auto halo_src = [&](int lx, int ly, int kk) -> long long {
+    int hgx, hgy;
+    const int sx = resolve_axis<PER>(x0 + lx, nx, G.bc[0], hgx);
+    const int sy = resolve_axis<PER>(y0 + ly, ny, G.bc[2], hgy);
+    return (long long)kk * plane + (long long)sy * nx + sx;
+  };
+  Cons<T> uhalo;
+  if constexpr (PREFETCH) {
+    const int h = lane_item(kz0);
+    if (h < NHX + NHY) {
+      int lx, ly;
+      halo_pos(h, lx, ly);
+      uhalo = load_cons(Us, n, halo_src(lx, ly, kz0));
+    }
+  }
 
   for (int k = kz0; k < kz1; ++k) {
     const long long c = (long long)k * plane + (long long)gy * nx + gx;
     // Early loads: the next z plane of the column and this cell's own state.
     Cons<T> unext, uown, u0own;
     if constexpr (RANK == 3) {
-      if (k + 1 < kz1) unext = load_cons(Us, n, column_src(G, col, k + 3));
+      if (k + 1 < kz1) unext = load_cons(Us, n, column_src<PER>(G, col, k + 3));
     }
 #if VH_EARLY_OWN
     if (active) {
@@ -255,132 +370,114 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       Prim<T> w = wc;
       if (ghx) w = ghost(w, G.bc[ghx - 1], 1);
       if (ghy) w = ghost(w, G.bc[2 + ghy - 1], 2);
-      VH_PUT5(S.w, [ty + 2][tx + 2], w);
+      VH_PUT5(S.w, [ty + L_::HY][tx + 2], w);
       wc = w;
     }
-    constexpr int NHX = 4 * TY;
-    constexpr int NHY = (RANK > 1) ? 4 * TX : 0;
-    for (int h = tid; h < NHX + NHY; h += NT) {
-      int lx, ly;
-      if (h < NHX) {
-        const int r = h / 4, q = h % 4;
-        ly = r;
-        lx = q < 2 ? q - 2 : TX + (q - 2);
-      } else {
-        const int hh = h - NHX;
-        const int r = hh / TX, q = hh % TX;
-        lx = q;
-        ly = r < 2 ? r - 2 : TY + (r - 2);
+    // The halo, rim and edge work below goes to lanes counted from wave `rot`,
+    // which moves by one wave per plane so every SIMD gets its share.
+    const int lt = lane_item(k);
+    if constexpr (PREFETCH) {
+      if (lt < NHX + NHY) {
+        int lx, ly;
+        halo_pos(lt, lx, ly);
+        stage_halo(lx, ly, uhalo);
       }
-      int hgx, hgy;
-      const int sx = resolve(x0 + lx, nx, G.bc[0], hgx);
-      const int sy = resolve(y0 + ly, ny, G.bc[2], hgy);
-      Prim<T> w = cons2prim<T, FLOOR64>(load_cons(Us, n, (long long)k * plane + (long long)sy * nx + sx), g,
-                                        G.gamma);
-      if (hgx) w = ghost(w, G.bc[hgx - 1], 1);
-      if (hgy) w = ghost(w, G.bc[2 + hgy - 1], 2);
-      VH_PUT5(S.w, [ly + 2][lx + 2], w);
+      // the next plane's halo cell of this lane, loaded while this plane runs
+      const int hn = lane_item(k + 1);
+      if (k + 1 < kz1 && hn < NHX + NHY) {
+        int lx, ly;
+        halo_pos(hn, lx, ly);
+        uhalo = load_cons(Us, n, halo_src(lx, ly, k + 1));
+      }
+    } else {
+      for (int h = lt; h < NHX + NHY; h += NT) {
+        int lx, ly;
+        halo_pos(h, lx, ly);
+        stage_halo(lx, ly, load_cons(Us, n, halo_src(lx, ly, k)));
+      }
     }
     __syncthreads();
 
     // ---- PA: slopes once per cell; publish hi face states ----
+    const T* wf = &S.w[0][0][0];
+    T* ff = &S.f[0][0];
     Prim<T> xlo, ylo;
     {
-      const Prim<T> sx = slope(lds_w(S, ty + 2, tx + 1), wc, lds_w(S, ty + 2, tx + 3));
+      const int ic = (ty + L_::HY) * L_::WX + tx + 2;
+      const Prim<T> sx = slope(lds_get5(wf, L_::WS, ic - 1), wc, lds_get5(wf, L_::WS, ic + 1));
       xlo = face_value(wc, sx, T(-0.5));
-      const Prim<T> xhi = face_value(wc, sx, T(0.5));
-      VH_PUT5(S.xh, [ty][tx + 1], xhi);
+      lds_put5(ff, L_::FS, L_::xs(ty, tx + 1), face_value(wc, sx, T(0.5)));
       if constexpr (RANK > 1) {
-        const Prim<T> sy = slope(lds_w(S, ty + 1, tx + 2), wc, lds_w(S, ty + 3, tx + 2));
+        const Prim<T> sy = slope(lds_get5(wf, L_::WS, ic - L_::WX), wc, lds_get5(wf, L_::WS, ic + L_::WX));
         ylo = face_value(wc, sy, T(-0.5));
-        const Prim<T> yhi = face_value(wc, sy, T(0.5));
-        VH_PUT5(S.yh, [ty + 1][tx], yhi);
+        lds_put5(ff, L_::FS, L_::ys(ty + 1, tx), face_value(wc, sy, T(0.5)));
       }
     }
-    // Halo rim: x hi state of cell -1 and lo state of cell TX per row; y hi
-    // state of cell -1 and lo state of cell TY per column.
+    // Halo rim, one code path for the four kinds: x hi state of cell -1 and lo
+    // state of cell TX per row, y hi state of cell -1 and lo state of cell TY
+    // per column.
     constexpr int NRX = 2 * TY;
     constexpr int NRY = (RANK > 1) ? 2 * TX : 0;
-    for (int h = tid; h < NRX + NRY; h += NT) {
-      if (h < NRX) {
-        const int r = h >> 1;
-        if ((h & 1) == 0) {   // cell -1: slots 0, 1, 2
-          const Prim<T> wm = lds_w(S, r + 2, 1);
-          const Prim<T> s = slope(lds_w(S, r + 2, 0), wm, lds_w(S, r + 2, 2));
-          const Prim<T> v = face_value(wm, s, T(0.5));
-          VH_PUT5(S.xh, [r][0], v);
-        } else {              // cell TX: slots TX+1, TX+2, TX+3
-          const Prim<T> wp = lds_w(S, r + 2, TX + 2);
-          const Prim<T> s = slope(lds_w(S, r + 2, TX + 1), wp, lds_w(S, r + 2, TX + 3));
-          const Prim<T> v = face_value(wp, s, T(-0.5));
-          VH_PUT5(S.xl, [r], v);
-        }
+    for (int h = lt; h < NRX + NRY; h += NT) {
+      const bool isy = h >= NRX;
+      const int q = isy ? h - NRX : h;
+      const int line = q >> 1;
+      const bool hi = q & 1;   // the lo state of the cell past the tile's hi edge
+      int ic, dst;
+      if (!isy) {
+        ic = (line + L_::HY) * L_::WX + (hi ? TX + 2 : 1);
+        dst = L_::xs(line, hi ? TX + 1 : 0);
       } else {
-        const int hh = h - NRX;
-        const int cc = hh >> 1;
-        if ((hh & 1) == 0) {
-          const Prim<T> wm = lds_w(S, 1, cc + 2);
-          const Prim<T> s = slope(lds_w(S, 0, cc + 2), wm, lds_w(S, 2, cc + 2));
-          const Prim<T> v = face_value(wm, s, T(0.5));
-          VH_PUT5(S.yh, [0][cc], v);
-        } else {
-          const Prim<T> wp = lds_w(S, TY + 2, cc + 2);
-          const Prim<T> s = slope(lds_w(S, TY + 1, cc + 2), wp, lds_w(S, TY + 3, cc + 2));
-          const Prim<T> v = face_value(wp, s, T(-0.5));
-          VH_PUT5(S.yl, [cc], v);
-        }
+        ic = (hi ? TY + 2 : 1) * L_::WX + line + 2;
+        dst = L_::ys(hi ? TY + 1 : 0, line);
       }
+      const int d = isy ? L_::WX : 1;
+      const Prim<T> wm = lds_get5(wf, L_::WS, ic);
+      const Prim<T> sl = slope(lds_get5(wf, L_::WS, ic - d), wm, lds_get5(wf, L_::WS, ic + d));
+      lds_put5(ff, L_::FS, dst, face_value(wm, sl, hi ? T(-0.5) : T(0.5)));
     }
     __syncthreads();
 
     // ---- PB: face fluxes (each face once), written over the consumed state ----
     const int bx = G.bc[0], by = G.bc[2];
     if (gy < ny && gx <= nx) {
-      Prim<T> L;
-      VH_GET5(L, S.xh, [ty][tx]);
-      const Cons<T> F = axis_face(gx, nx, bx, G.bc[1], L, xlo, g, igm1, 1);
-      VH_PUTF(S.xh, [ty][tx], F);
-      if (G.bflux && bx != BC_PERIODIC && (gx == 0 || gx == nx))
+      const int sl = L_::xs(ty, tx);
+      const Cons<T> F = axis_face<PER>(gx, nx, bx, G.bc[1], lds_get5(ff, L_::FS, sl), xlo, g, igm1, 1);
+      lds_putf(ff, L_::FS, sl, F);
+      if (!PER && G.bflux && bx != BC_PERIODIC && (gx == 0 || gx == nx))
         bflux_add(G, 0, gx == nx, (long long)k * ny + gy, F, T(0.5 * dt), a0);
     }
     if constexpr (RANK > 1) {
       if (gx < nx && gy <= ny) {
-        Prim<T> L;
-        VH_GET5(L, S.yh, [ty][tx]);
-        const Cons<T> F = axis_face(gy, ny, by, G.bc[3], L, ylo, g, igm1, 2);
-        VH_PUTF(S.yh, [ty][tx], F);
-        if (G.bflux && by != BC_PERIODIC && (gy == 0 || gy == ny))
+        const int sl = L_::ys(ty, tx);
+        const Cons<T> F = axis_face<PER>(gy, ny, by, G.bc[3], lds_get5(ff, L_::FS, sl), ylo, g, igm1, 2);
+        lds_putf(ff, L_::FS, sl, F);
+        if (!PER && G.bflux && by != BC_PERIODIC && (gy == 0 || gy == ny))
           bflux_add(G, 1, gy == ny, (long long)k * nx + gx, F, T(0.5 * dt), a1);
       }
     }
-    // Tile hi-edge faces: x face TX of each row, y face TY of each column.
+    // Tile hi-edge faces, x face TX of each row and y face TY of each column,
+    // through one code path: a y face is evaluated in the frame of an x face
+    // (x and y components exchanged on the way in and out).
     constexpr int NEX = TY;
     constexpr int NEY = (RANK > 1) ? TX : 0;
-    for (int h = tid; h < NEX + NEY; h += NT) {
-      if (h < NEX) {
-        const int r = h;
-        const int gp = x0 + TX;
-        if (y0 + r < ny && gp <= nx) {
-          Prim<T> L, R;
-          VH_GET5(L, S.xh, [r][TX]);
-          VH_GET5(R, S.xl, [r]);
-          const Cons<T> F = axis_face(gp, nx, bx, G.bc[1], L, R, g, igm1, 1);
-          VH_PUTF(S.xh, [r][TX], F);
-          if (G.bflux && bx != BC_PERIODIC && gp == nx)
-            bflux_add(G, 0, 1, (long long)k * ny + y0 + r, F, T(0.5 * dt), a0);
-        }
-      } else {
-        const int cc = h - NEX;
-        const int gp = y0 + TY;
-        if (x0 + cc < nx && gp <= ny) {
-          Prim<T> L, R;
-          VH_GET5(L, S.yh, [TY][cc]);
-          VH_GET5(R, S.yl, [cc]);
-          const Cons<T> F = axis_face(gp, ny, by, G.bc[3], L, R, g, igm1, 2);
-          VH_PUTF(S.yh, [TY][cc], F);
-          if (G.bflux && by != BC_PERIODIC && gp == ny)
-            bflux_add(G, 1, 1, (long long)k * nx + x0 + cc, F, T(0.5 * dt), a1);
-        }
+    for (int h = lt; h < NEX + NEY; h += NT) {
+      const bool isy = h >= NEX;
+      const int line = isy ? h - NEX : h;
+      const int gp = isy ? y0 + TY : x0 + TX;
+      const int na = isy ? ny : nx;
+      const bool inside = isy ? (x0 + line < nx) : (y0 + line < ny);
+      if (inside && gp <= na) {
+        const int sl = isy ? L_::ys(TY, line) : L_::xs(line, TX);
+        const int sr = isy ? L_::ys(TY + 1, line) : L_::xs(line, TX + 1);
+        const Prim<T> L = swap_xy(lds_get5(ff, L_::FS, sl), isy);
+        const Prim<T> R = swap_xy(lds_get5(ff, L_::FS, sr), isy);
+        const Cons<T> F = swap_xy(axis_face<PER>(gp, na, isy ? by : bx, isy ? G.bc[3] : G.bc[1], L, R, g, igm1, 1),
+                                  isy);
+        lds_putf(ff, L_::FS, sl, F);
+        if (!PER && G.bflux && (isy ? by : bx) != BC_PERIODIC && gp == na)
+          bflux_add(G, isy ? 1 : 0, 1, isy ? (long long)k * nx + x0 + line : (long long)k * ny + y0 + line, F,
+                    T(0.5 * dt), isy ? a1 : a0);
       }
     }
 
@@ -391,7 +488,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       sz1 = slope(wz0, wz1, wz2);
       const int bz = G.bc[4];
       const Prim<T> L = face_value(wz0, sz0, T(0.5));
-      if (bz != BC_PERIODIC && k == nz - 1) {
+      if (!PER && bz != BC_PERIODIC && k == nz - 1) {
         fzhi = hllc(L, ghost(L, G.bc[5], 3), g, igm1, 3);
         if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
       } else
@@ -406,16 +503,16 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       if constexpr (STAGE2) u0own = load_cons(U0, n, c);
 #endif
       Cons<T> fxlo, fxhi, fylo, fyhi;
-      VH_GETF(fxlo, S.xh, [ty][tx]);
-      VH_GETF(fxhi, S.xh, [ty][tx + 1]);
+      fxlo = lds_getf(ff, L_::FS, L_::xs(ty, tx));
+      fxhi = lds_getf(ff, L_::FS, L_::xs(ty, tx + 1));
       if constexpr (RANK > 1) {
-        VH_GETF(fylo, S.yh, [ty][tx]);
-        VH_GETF(fyhi, S.yh, [ty + 1][tx]);
+        fylo = lds_getf(ff, L_::FS, L_::ys(ty, tx));
+        fyhi = lds_getf(ff, L_::FS, L_::ys(ty + 1, tx));
       }
       // Lo-face kinds: 0 interior (partner earlier), 1 domain boundary, 2 periodic wrap.
-      const int kx = gx > 0 ? 0 : (bx == BC_PERIODIC ? 2 : 1);
-      const int ky = gy > 0 ? 0 : (by == BC_PERIODIC ? 2 : 1);
-      const int kz = k > 0 ? 0 : (G.bc[4] == BC_PERIODIC ? 2 : 1);
+      const int kx = gx > 0 ? 0 : (PER || bx == BC_PERIODIC ? 2 : 1);
+      const int ky = gy > 0 ? 0 : (PER || by == BC_PERIODIC ? 2 : 1);
+      const int kz = k > 0 ? 0 : (PER || G.bc[4] == BC_PERIODIC ? 2 : 1);
       Cons<T> acc;
       acc.d = acc.mx = acc.my = acc.mz = acc.e = T(0);
       if (kx == 0 && (RANK < 2 || ky == 0) && (RANK < 3 || kz == 0)) {
@@ -470,7 +567,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
         o.my = h * u0own.my + h * (uown.my - invV * acc.my);
         o.mz = h * u0own.mz + h * (uown.mz - invV * acc.mz);
         o.e = h * u0own.e + h * (uown.e - invV * acc.e);
-        cfl_max = vmax(cfl_max, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(o, g, G.gamma), g, G));
+        cfl.add(cons2prim<T, FLOOR64>(o, g, G.gamma), g);
       }
       store_cons(Uout, n, c, o);
     }
@@ -480,12 +577,13 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       sz0 = sz1;
       wz0 = wz1;
       wz1 = wz2;
-      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64>(unext, G, k + 3, g);
+      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64, PER>(unext, G, k + 3, g);
     }
   }
 
   if constexpr (STAGE2) {
-    const double m = block_max<NT>(cfl_max, S.red);
+    static_assert(sizeof(S.w) >= (NT / 64) * sizeof(double), "reduction scratch");
+    const double m = block_max<NT>(cfl.value(G.w), reinterpret_cast<double*>(&S.w[0][0][0]));
     if (tid == 0) partials[blockIdx.x] = m;
   }
 }
@@ -496,10 +594,10 @@ __global__ void __launch_bounds__(256) cfl_kernel(const T* __restrict__ U, GridD
                                                   double* __restrict__ partials) {
   __shared__ double red[16];
   const T g = T(G.gamma);
-  double m = 0.0;
+  CflMax<T, RANK> cfl;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < G.ncell; c += (long long)gridDim.x * 256)
-    m = vmax(m, cfl_term<T, RANK>(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g, G));
-  m = block_max<256>(m, red);
+    cfl.add(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g);
+  const double m = block_max<256>(cfl.value(G.w), red);
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 

@@ -354,6 +354,14 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
     n *= G.n[d];
   }
   G.ncell = n;
+  {
+    long long base = 0;
+    for (int d = 0; d < 3; ++d) {
+      G.bbase[2 * d] = base;
+      G.bbase[2 * d + 1] = base + n / G.n[d];
+      if (d < rank) base += 2 * (n / G.n[d]);
+    }
+  }
   G.gamma = gamma;
   G.igm1 = 1.0 / (gamma - 1.0);
   // lib/RefMesh/src/RefMesh.jl:127 and :135-142 (left folds in Float64).
@@ -479,6 +487,8 @@ int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux) {
   enqueue_step(g);
   g->G.bflux = nullptr;
   VH_HIP(hipGetLastError());
+  // The register is the caller's buffer: complete it before returning.
+  VH_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
 
