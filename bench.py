@@ -108,9 +108,10 @@ def blocks_leg(V, args):
         esz = np.dtype(dtype).itemsize
         r = {"tables": {k: info[k] for k in ("tiles", "link_faces", "register_cells")},
              "zchunk": info["zchunk"], "workgroups": info["workgroups"]}
+        methods = args.block_methods.split(",")
         for method, nst in (("graph", args.block_steps), ("persistent", args.block_steps),
                             ("per_block", max(2, args.block_steps // 4))):
-            if key == "f32" and method != "graph":
+            if method not in methods or (key == "f32" and method != methods[0]):
                 continue
             t = s.time_steps(nst, method=method, stage2=(method == "graph"))
             r[method] = {"ms_per_step": t["step_ms"], "zone_updates_per_s": m.ncell / (t["step_ms"] * 1e-3),
@@ -120,7 +121,7 @@ def blocks_leg(V, args):
                 r["stage2_GBps"] = 15 * esz * m.ncell / (t["stage2_ms"] * 1e-3) / 1e9
             log(f"[bench] blocks {key} {method}: {t['step_ms']:.3f} ms/step, "
                 f"{m.ncell / (t['step_ms'] * 1e-3) / 1e9:.2f} G zone-updates/s")
-        if key == "f64":
+        if key == "f64" and "graph" in r and "per_block" in r:
             r["speedup_graph_vs_per_block"] = r["per_block"]["ms_per_step"] / r["graph"]["ms_per_step"]
         s.close()
         res[key] = r
@@ -277,6 +278,78 @@ def sedov_leg(V, args):
     return out
 
 
+def gravity_leg(V, args):
+    """Self-gravity on a 3D periodic box (n^3, gamma=5/3, G=1): a Jeans-unstable
+    density ripple rho = 1 + 1e-3 (cos 2πx + cos 2πy + cos 2πz) plus 1e-4 random
+    cells, p = 0.06, v = 0. Times the device CG Poisson solve from φ = 0 (tol 1e-8)
+    and warm-started after a step, and the SSP-RK2 step with the gravity source
+    against the same step without gravity, FP64 fast profile."""
+    import numpy as np
+    n = args.gravity_n
+    x = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    rng = np.random.default_rng(5)
+    rho = 1.0 + 1e-3 * (np.cos(2 * np.pi * X) + np.cos(2 * np.pi * Y) + np.cos(2 * np.pi * Z)) \
+        + 1e-4 * rng.standard_normal(X.shape)
+    W = np.stack([rho, 0 * X, 0 * X, 0 * X, 0.06 + 0 * X])
+    U = V.prim2cons_array(W, 5.0 / 3.0, np.float64)
+    del W, X, Y, Z, rho
+    prob = V.Problem("jeans3d", (n,) * 3, ((0.0, 1.0),) * 3, 5.0 / 3.0, "periodic",
+                     lambda x, y, z: (1.0, 0.0, 0.0, 0.0, 1.0), 1.0, 0.3)
+    sim = V.Simulation(V.UniformMesh((n,) * 3, prob.domain), prob)
+    sim.set_mode(args.mode)
+    sim.upload(U)
+    # the same step without gravity, for the source's cost
+    r_plain = sim.time_steps(args.gravity_steps)
+    sim.upload(U)
+    grav = V.enable_gravity(sim, G=1.0, bcs="periodic", tol=1e-8, maxiter=2000)
+    sim.sync()
+    t0 = time.perf_counter()
+    it_cold, rr_cold = grav.solve()
+    cold = time.perf_counter() - t0
+    r_grav = sim.time_steps(args.gravity_steps)
+    t0 = time.perf_counter()
+    it_warm, rr_warm = grav.solve()
+    warm = time.perf_counter() - t0
+    ncell = n ** 3
+    per_it = cold / max(it_cold, 1)
+    # algorithmic HBM bytes per CG iteration: matvec 16 B/cell (read p, write Ap),
+    # update 48 B/cell (read φ, r, p, Ap; write φ, r), direction 24 B/cell (read r, p; write p)
+    bytes_it = 88 * ncell
+    res = {"workload": f"3D {n}^3 periodic Jeans ripple, gamma=5/3, G=1, CG tol 1e-8, FP64 {args.mode} profile",
+           "cells": ncell,
+           "poisson_cold": {"iterations": it_cold, "relres": rr_cold, "seconds": cold,
+                            "ms_per_iteration": 1e3 * per_it, "GBps_algorithmic": bytes_it / per_it / 1e9},
+           "poisson_warm_after_step": {"iterations": it_warm, "relres": rr_warm, "seconds": warm},
+           "step_ms_with_gravity": r_grav["step_ms"], "step_ms_without": r_plain["step_ms"],
+           "zone_updates_per_s_with_gravity": ncell / (r_grav["step_ms"] * 1e-3)}
+    log(f"[bench] gravity {n}^3: CG {it_cold} it in {cold * 1e3:.1f} ms ({1e3 * per_it:.3f} ms/it, "
+        f"{bytes_it / per_it / 1e9:.0f} GB/s algorithmic), warm {it_warm} it; step {r_grav['step_ms']:.3f} ms "
+        f"with gravity vs {r_plain['step_ms']:.3f} ms without")
+    if not args.no_cpu_baseline:
+        if ROOT not in sys.path:
+            sys.path.insert(0, ROOT)
+        from oracle import oracle as O
+        m = 64
+        g = O.make_grid((m,) * 3, [(0.0, 1.0)] * 3, "periodic", 5.0 / 3.0)
+        xs = (np.arange(m) + 0.5) / m
+        Zs, Ys, Xs = np.meshgrid(xs, xs, xs, indexing="ij")
+        Us = np.zeros((5, m, m, m))
+        Us[0] = 1.0 + 1e-3 * (np.cos(2 * np.pi * Xs) + np.cos(2 * np.pi * Ys) + np.cos(2 * np.pi * Zs))
+        Us[4] = 0.06 / (5.0 / 3.0 - 1)
+        phi = np.zeros(m ** 3)
+        t0 = time.perf_counter()
+        it, _ = O.poisson_solve(g, 7, Us, 1.0, True, 2000, 1e-8, phi)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": m ** 3 * it / el, "unit": "cell-iterations/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle CG solve of a {m}^3 ripple from phi=0, {it} iterations, {el:.2f} s"}
+        res["gpu_cell_iterations_per_s"] = ncell / per_it
+        log(f"[bench] gravity CPU {m ** 3 * it / el / 1e6:.1f} M cell-iterations/s vs GPU {ncell / per_it / 1e9:.2f} G")
+    grav.close()
+    sim.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,11 +370,16 @@ def main():
                     help="skip the 256^3 legs (profiling runs of the other legs only; no headline value)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint rollout leg")
     ap.add_argument("--no-sedov", action="store_true", help="skip the dynamic-AMR Sedov leg")
+    ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity leg")
+    ap.add_argument("--gravity-n", type=int, default=256)
+    ap.add_argument("--gravity-steps", type=int, default=20)
     ap.add_argument("--adjoint-n", type=int, default=512)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--blocks", type=int, default=3000, help="blocks of the batched-blocks leg")
     ap.add_argument("--block-steps", type=int, default=20)
+    ap.add_argument("--block-methods", default="graph,persistent,per_block",
+                    help="launch methods of the blocks leg, in order (Float32 runs the first only)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also time 3 device-to-device copies of 1 GiB (counter calibration)")
     args = ap.parse_args()
@@ -443,6 +521,8 @@ def main():
         out["adjoint"] = adjoint_leg(V, args)
     if not args.no_sedov:
         out["sedov_amr"] = sedov_leg(V, args)
+    if not args.no_gravity:
+        out["self_gravity"] = gravity_leg(V, args)
 
     print(json.dumps(out), flush=True)
 

@@ -141,6 +141,37 @@ int64_t vh_grid_bflux_count(const vh_grid* g);
 int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double* t_io,
                 int64_t* steps);
 
+/* Self-gravity (src/gravity.jl). With gravity on, every RK stage adds the source
+ * (0, -rho*g*V, -(rho*v.g)*V) of g = -grad(phi) to the flux sum
+ * (apply_gravity_source!, src/gravity.jl:412-447), compute_dt folds in the
+ * free-fall limiter sqrt(4*pi*G*rho)/0.5 (src/gravity.jl:449-452), and
+ * vh_grid_run solves the Poisson equation before each step (src/driver.jl:498). */
+
+/* enable_gravity!(sim; G, bcs, maxiter, tol, project_mean) (src/gravity.jl:63-80):
+ * allocates phi (zero) and the CG work vectors. phi_periodic: bit d set when phi
+ * is periodic along axis d (else homogeneous Neumann). Synchronises. */
+int vh_grid_enable_gravity(vh_grid* g, double G, int32_t phi_periodic, int64_t maxiter, double tol,
+                           int32_t project_mean);
+int vh_grid_disable_gravity(vh_grid* g);
+
+/* solve_poisson!(sim, grav) (src/gravity.jl:248-322): conjugate gradients on
+ * A = -lap*V for the current density, warm-started from phi, Float64 dot
+ * products, mean projection when enabled. Returns iterations and the final
+ * relative residual; synchronises. */
+int vh_grid_solve_poisson(vh_grid* g, int64_t* iters, double* relres);
+
+/* phi (ncell values of the state dtype, x fastest) to / from host memory. */
+int vh_grid_phi_upload(vh_grid* g, const void* host);
+int vh_grid_phi_download(vh_grid* g, void* host);
+
+/* apply_laplacian!(sim, grav, x, out) (src/gravity.jl:99-128): out = A*x on
+ * device arrays of ncell values. Synchronises. */
+int vh_grid_apply_laplacian(vh_grid* g, const void* dx, void* dout);
+
+/* gravity_accel for every cell (src/gravity.jl:367-410): 3*ncell values (g_x,
+ * g_y, g_z fields) into device memory. Synchronises. */
+int vh_grid_gravity_accel(vh_grid* g, void* dg);
+
 /* conserved_totals(sim) (src/diagnostics.jl:16-31): sum of U*V per field in
  * Float64 (tree order). Synchronises. */
 int vh_grid_totals(vh_grid* g, double* out5);

@@ -75,6 +75,16 @@ def lib():
             getattr(L, f"vo_step_omp_{suf}").restype = C.c_int
             getattr(L, f"vo_compute_dt_omp_{suf}").argtypes = [gp, tp, C.c_double, C.c_int, C.c_int]
             getattr(L, f"vo_compute_dt_omp_{suf}").restype = C.c_double
+            getattr(L, f"vo_poisson_apply_{suf}").argtypes = [gp, C.c_int, tp, tp]
+            getattr(L, f"vo_poisson_rhs_{suf}").argtypes = [gp, tp, C.c_double, C.c_int, tp]
+            getattr(L, f"vo_poisson_solve_{suf}").argtypes = [gp, C.c_int, tp, C.c_double, C.c_int, C.c_int64,
+                                                              C.c_double, tp, tp, dp]
+            getattr(L, f"vo_poisson_solve_{suf}").restype = C.c_int64
+            getattr(L, f"vo_grav_accel_{suf}").argtypes = [gp, C.c_int, tp, tp]
+            getattr(L, f"vo_step_grav_{suf}").argtypes = [gp, tp, C.c_double, tp, C.c_int, C.c_int, tp]
+            getattr(L, f"vo_step_grav_{suf}").restype = C.c_int
+            getattr(L, f"vo_compute_dt_grav_{suf}").argtypes = [gp, tp, C.c_double, C.c_int, C.c_double]
+            getattr(L, f"vo_compute_dt_grav_{suf}").restype = C.c_double
         L.vo_minmod_f64.argtypes = [C.c_double, C.c_double]
         L.vo_minmod_f64.restype = C.c_double
         L.vo_minmod_f32.argtypes = [C.c_float, C.c_float]
@@ -402,3 +412,65 @@ class BlockOracle:
         k = getattr(self.L, f"bo_evolve_{_suf(U.dtype)}")(self.h, U.ctypes.data, tfinal, cfl, int(max_steps),
                                                          int(subcycle), int(floor64), C.byref(t))
         return t.value, int(k)
+
+
+# ---------------- self-gravity (src/gravity.jl) ----------------
+
+def gper_mask(rank: int, bcs) -> int:
+    """Bit mask of the axes along which φ is periodic; bcs: "periodic"/"neumann"
+    for every axis, or one name per axis."""
+    if isinstance(bcs, str):
+        bcs = [bcs] * rank
+    return sum(1 << d for d in range(rank) if bcs[d] == "periodic")
+
+
+def poisson_apply(g: Grid, gper: int, x: np.ndarray) -> np.ndarray:
+    """A·x with A = −∇²·V (src/gravity.jl:99-128, 326-336)."""
+    x = np.ascontiguousarray(x)
+    out = np.empty_like(x)
+    getattr(lib(), f"vo_poisson_apply_{_suf(x.dtype)}")(C.byref(g), int(gper), _ptr(x), _ptr(out))
+    return out
+
+
+def poisson_rhs(g: Grid, U: np.ndarray, G: float, project_mean: bool) -> np.ndarray:
+    U = np.ascontiguousarray(U)
+    b = np.empty(ncell(g), dtype=U.dtype)
+    getattr(lib(), f"vo_poisson_rhs_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(G), int(project_mean), _ptr(b))
+    return b
+
+
+def poisson_solve(g: Grid, gper: int, U: np.ndarray, G: float, project_mean: bool, maxiter: int, tol: float,
+                  phi: np.ndarray):
+    """CG solve of ∇²φ = 4πGρ into phi (warm start, in place); returns (iters, relres)."""
+    U = np.ascontiguousarray(U)
+    assert phi.flags.c_contiguous and phi.dtype == U.dtype
+    work = np.empty(4 * ncell(g), dtype=U.dtype)
+    rr = C.c_double()
+    it = getattr(lib(), f"vo_poisson_solve_{_suf(U.dtype)}")(C.byref(g), int(gper), _ptr(U), float(G),
+                                                            int(project_mean), int(maxiter), float(tol),
+                                                            _ptr(phi), _ptr(work), C.byref(rr))
+    return int(it), rr.value
+
+
+def grav_accel(g: Grid, gper: int, phi: np.ndarray) -> np.ndarray:
+    phi = np.ascontiguousarray(phi)
+    out = np.empty(3 * ncell(g), dtype=phi.dtype)
+    getattr(lib(), f"vo_grav_accel_{_suf(phi.dtype)}")(C.byref(g), int(gper), _ptr(phi), _ptr(out))
+    return out.reshape(3, -1)
+
+
+def step_grav(g: Grid, U: np.ndarray, dt: float, gper: int, phi: np.ndarray, floor64=False) -> None:
+    """In-place SSP-RK2 step with the gravity source of the given φ."""
+    assert U.flags.c_contiguous
+    phi = np.ascontiguousarray(phi, dtype=U.dtype)
+    work = np.empty(10 * ncell(g), dtype=U.dtype)
+    rc = getattr(lib(), f"vo_step_grav_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(dt), _ptr(work), int(floor64),
+                                                         int(gper), _ptr(phi))
+    if rc != 0:
+        raise ValueError("invalid grid")
+
+
+def compute_dt_grav(g: Grid, U: np.ndarray, cfl: float, G: float, floor64=False) -> float:
+    U = np.ascontiguousarray(U)
+    return float(getattr(lib(), f"vo_compute_dt_grav_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(cfl),
+                                                                       int(floor64), float(G)))

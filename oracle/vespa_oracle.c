@@ -11,7 +11,8 @@
  *   (_plm_slope, _face_value, _flux_face!, accumulate_flux!, _euler_apply!,
  *   _rk2_combine!, step!, compute_dt, evolve!), src/diagnostics.jl
  *   (conserved_totals), src/exact_riemann.jl (Toro exact solver), and
- *   lib/RefMesh/src/RefMesh.jl (neighbor, face_area, cell_volume, for_each_face).
+ *   lib/RefMesh/src/RefMesh.jl (neighbor, face_area, cell_volume, for_each_face),
+ *   and src/gravity.jl (Poisson CG, g = −∇φ, gravity source, free-fall limiter).
  *
  * The reference itself is Julia and no Julia toolchain exists here, so the
  * restatement is pinned by closed forms and invariants in tests/ (exact Riemann
@@ -137,6 +138,7 @@ static inline double vo_jmin_f64_(double x, double y) {
 #define OSQRT sqrt
 #define OFABS fabs
 #include "oracle_impl.h"
+#include "gravity_impl.h"
 #undef OT
 #undef OSUF
 #undef OSQRT
@@ -147,6 +149,7 @@ static inline double vo_jmin_f64_(double x, double y) {
 #define OSQRT sqrtf
 #define OFABS fabsf
 #include "oracle_impl.h"
+#include "gravity_impl.h"
 #undef OT
 #undef OSUF
 #undef OSQRT

@@ -110,6 +110,12 @@ def test_rollout_gradient_matches_oracle_and_fd(O):
         assert abs(np.sum(G * v) - fd) / abs(fd) < 1e-5
     after = ro.info()
     assert after["forward_ms"] > 0 and after["backward_ms"] > 0 and after["device_bytes"] > 0
+    # the backward sweep is replayed from its captured graph: same gradient again,
+    # and again after a fresh forward run from the same state
+    np.testing.assert_array_equal(ro.backward(w), G)
+    sim.upload(U0)
+    np.testing.assert_array_equal(ro.forward(), UF)
+    np.testing.assert_array_equal(ro.backward(w), G)
     ro.close()
     sim.close()
 

@@ -12,6 +12,7 @@ from .hydro import (BoundaryFluxRegister, HIPFlux, IdealHydro, Outflow, Periodic
                     bflux_count, bflux_plane,
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
                     dump_fields, evolve, prim2cons_array, run_steps, sod_problem, step)
+from .gravity import GravityField, enable_gravity, solve_poisson
 
 __all__ = [
     "DeviceBuffer",
@@ -21,4 +22,5 @@ __all__ = [
     "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",
     "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",
     "dump_fields", "evolve", "prim2cons_array", "run_steps", "sod_problem", "step",
+    "GravityField", "enable_gravity", "solve_poisson",
 ]

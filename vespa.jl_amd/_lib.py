@@ -33,6 +33,8 @@ EXPORTED = (
     "vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
     "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form",
     "vh_grid_step_bflux", "vh_grid_bflux_count",
+    "vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
+    "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel",
 )
 
 
@@ -135,6 +137,16 @@ def load_library(path: str | None = None):
     L.vh_grid_step_bflux.restype = C.c_int
     L.vh_grid_bflux_count.argtypes = [vp]
     L.vh_grid_bflux_count.restype = C.c_int64
+    L.vh_grid_enable_gravity.argtypes = [vp, C.c_double, C.c_int32, C.c_int64, C.c_double, C.c_int32]
+    L.vh_grid_disable_gravity.argtypes = [vp]
+    L.vh_grid_solve_poisson.argtypes = [vp, i64p, dp]
+    L.vh_grid_phi_upload.argtypes = [vp, vp]
+    L.vh_grid_phi_download.argtypes = [vp, vp]
+    L.vh_grid_apply_laplacian.argtypes = [vp, vp, vp]
+    L.vh_grid_gravity_accel.argtypes = [vp, vp]
+    for n in ("vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
+              "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel"):
+        getattr(L, n).restype = C.c_int
     for n in ("vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
               "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form"):
         getattr(L, n).restype = C.c_int

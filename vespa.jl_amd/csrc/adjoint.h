@@ -150,7 +150,13 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   const double tL = rL * unL, tR = rR * unR;
   const double num = pR - pL + tL * (SL - unL) - tR * (SR - unR);
   const double den = rL * (SL - unL) - rR * (SR - unR);
+#if VH_FAST_PROFILE
+  // one refined reciprocal per denominator, shared by the value and its adjoint
+  const double deninv = recip(den);
+  const double Ss = num * deninv;
+#else
   const double Ss = adiv(num, den);
+#endif
   const bool useL = Ss >= 0.0;
   const D5 W = useL ? WL : WR;
   D5 Wb = d5_zero();
@@ -159,10 +165,20 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   const double E = energy64(W, g);
   const double q = r * (SK - un);
   const double d3 = SK - Ss;
+  const double rinv = recip(r);
+  const double qinv = recip(q);
+  const double d3inv = recip(d3);
+#if VH_FAST_PROFILE
+  const double factor = q * d3inv;
+  const double vs0 = A == 0 ? Ss : W.vx, vs1 = A == 1 ? Ss : W.vy, vs2 = A == 2 ? Ss : W.vz;
+  const double Aa = Ss - un, Bb = Ss + p * qinv;
+  const double Es = E * rinv + Aa * Bb;
+#else
   const double factor = adiv(q, d3);
   const double vs0 = A == 0 ? Ss : W.vx, vs1 = A == 1 ? Ss : W.vy, vs2 = A == 2 ? Ss : W.vz;
   const double Aa = Ss - un, Bb = Ss + adiv(p, q);
   const double Es = adiv(E, r) + Aa * Bb;
+#endif
   // out = F(W) + SK·(Us − Uk), Us = factor·(1, vs, Es), Uk = (r, r·v, E)
   const double SKb0 = ob.d * (factor - r) + ob.mx * (factor * vs0 - r * W.vx) + ob.my * (factor * vs1 - r * W.vy) +
                       ob.mz * (factor * vs2 - r * W.vz) + ob.e * (factor * Es - E);
@@ -181,21 +197,18 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   Wb.vz -= u3 * r;
   Eb -= u4;
   // Es = E/r + Aa·Bb
-  const double rinv = recip(r);
   Eb += Esb * rinv;
   rb -= Esb * E * rinv * rinv;
   const double Ab = Esb * Bb, Bbb = Esb * Aa;
   Ssb += Ab;
   unb -= Ab;
   Ssb += Bbb;
-  const double qinv = recip(q);
   pb += Bbb * qinv;
   qb -= Bbb * p * qinv * qinv;
   if (A == 0) { Ssb += vsb0; } else { Wb.vx += vsb0; }
   if (A == 1) { Ssb += vsb1; } else { Wb.vy += vsb1; }
   if (A == 2) { Ssb += vsb2; } else { Wb.vz += vsb2; }
   // factor = q/(SK − Ss)
-  const double d3inv = recip(d3);
   qb += factorb * d3inv;
   const double d3b = -factorb * factor * d3inv;
   SKb += d3b;
@@ -212,7 +225,9 @@ __device__ void hllc_vjp(const D5& WL, const D5& WR, double g, const Cons<double
   else d5_add(WRb, Wb);
   double SLb = useL ? SKb : 0.0, SRb = useL ? 0.0 : SKb;
   // Ss = num/den
+#if !VH_FAST_PROFILE
   const double deninv = recip(den);
+#endif
   const double numb = Ssb * deninv, denb = -Ssb * Ss * deninv;
   double rLb = 0, rRb = 0, pLb = 0, pRb = 0, unLb = 0, unRb = 0, cLb = 0, cRb = 0;
   pRb += numb;
@@ -619,8 +634,12 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
     S.Lx[0][ty][tx + 2] = l.d; S.Lx[1][ty][tx + 2] = l.mx; S.Lx[2][ty][tx + 2] = l.my;
     S.Lx[3][ty][tx + 2] = l.mz; S.Lx[4][ty][tx + 2] = l.e;
   }
+  // Loops whose item count is not a multiple of 256 start at a wave that
+  // depends on the workgroup, so the partial last pass lands on different SIMDs
+  // for the workgroups that share a compute unit.
+  const int lt = (tid + 64 * (int)((blockIdx.x + blockIdx.y * gridDim.x) & 3)) & 255;
   constexpr int NWX = 6 * JTY, NLX = 4 * JTY, NWY = 6 * JTX, NLY = 4 * JTX;
-  for (int h = tid; h < NWX + NLX + NWY + NLY; h += 256) {
+  for (int h = lt; h < NWX + NLX + NWY + NLY; h += 256) {
     if (h < NWX) {
       const int r = h / 6, q = h % 6;
       const int lc = q < 3 ? q : JTX + q;   // 0,1,2 | JTX+3..JTX+5
@@ -650,7 +669,7 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
 
   D5 Wb = d5_zero();
   // ---- x faces: ql = 0..JTX+2 (face between columns x0−2+ql and x0−1+ql) ----
-  for (int h = tid; h < JTY * (JTX + 3); h += 256) {
+  for (int h = lt; h < JTY * (JTX + 3); h += 256) {
     const int r = h / (JTX + 3), ql = h % (JTX + 3);
     const D5 w0 = jat_get(S.Wx, r, ql), w1 = jat_get(S.Wx, r, ql + 1), w2 = jat_get(S.Wx, r, ql + 2),
              w3 = jat_get(S.Wx, r, ql + 3);
@@ -675,7 +694,7 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
   }
   __syncthreads();
   // ---- y faces: ql = 0..JTY+2 (face between rows y0−2+ql and y0−1+ql) ----
-  for (int h = tid; h < (JTY + 3) * JTX; h += 256) {
+  for (int h = (lt + 128) & 255; h < (JTY + 3) * JTX; h += 256) {
     const int ql = h / JTX, cc = h % JTX;
     const D5 w0 = jat_wy(S, ql - 3, cc), w1 = jat_wy(S, ql - 2, cc), w2 = jat_wy(S, ql - 1, cc),
              w3 = jat_wy(S, ql, cc);

@@ -48,6 +48,10 @@ struct GridDev {
   double area[3];  // face areas
   void* bflux;     // boundary-flux register (5 Tf per boundary face slot) or null
   long long bbase[6];  // first slot of each boundary plane (axis 0 lo, axis 0 hi, axis 1 lo, ...)
+  const void* phi; // gravitational potential (ncell Tf) when self-gravity is on, else null
+  double gravG;    // gravitational constant
+  int gper;        // bit d: φ periodic along axis d
+  int pad_;
 };
 
 // ---- batched blocks (block-structured AMR) ----

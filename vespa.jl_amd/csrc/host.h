@@ -48,3 +48,32 @@ bool adj_launch_jat_fused(hipStream_t s, const double* U, const double* lam, con
                           double* out);
 
 }  // namespace vh
+
+namespace vh {
+
+// Self-gravity on uniform grids (gravity.hip). φ and the CG vectors are stored
+// at the state's precision; dot products and the matrix-vector sums in Float64.
+struct GravParams {
+  double G = 1.0;
+  int gper = 0;            // bit d: φ periodic along axis d
+  int project_mean = 1;
+  long long maxiter = 500;
+  double tol = 1e-8;
+  int chunk = 16;          // iterations enqueued between host checks
+};
+struct GravBuffers {
+  void* phi = nullptr;     // ncell values
+  void* work = nullptr;    // 4·ncell values: r, p, Ap, b
+  const void* rho = nullptr;
+  void* state = nullptr;   // CG state block (device)
+  double* scal = nullptr;  // 8 scalars (device)
+  double* partials = nullptr;  // 4096 per-workgroup partial sums (device)
+  void* host_state = nullptr;  // pinned copy of the state block
+};
+constexpr int kGravStateBytes = 64;
+void grav_apply(hipStream_t s, int dtype, const GridDev& G, int gper, const void* x, void* out);
+void grav_accel(hipStream_t s, int dtype, const GridDev& G, int gper, const void* phi, void* g);
+int grav_solve(hipStream_t s, int dtype, const GridDev& G, const GravParams& P, GravBuffers& B, long long* iters,
+               double* relres);
+
+}  // namespace vh

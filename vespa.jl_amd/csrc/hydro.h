@@ -44,13 +44,13 @@ __device__ __forceinline__ float vsqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Reciprocal from the hardware estimate refined by two Newton steps
-// (e = 1 − b·r; r += r·e), then a/b as a·r with one residual correction.
+// Reciprocal from the hardware estimate (relative error below 2^-24 on gfx950)
+// refined by one third-order step: e = 1 − b·r, r ← r + r·(e + e²), error
+// O(e³); then a/b as a·r with one residual correction.
 __device__ __forceinline__ double fast_rcp(double b) {
-  double r = __builtin_amdgcn_rcp(b);
-  r = vfma(vfma(-b, r, 1.0), r, r);
-  r = vfma(vfma(-b, r, 1.0), r, r);
-  return r;
+  const double r = __builtin_amdgcn_rcp(b);
+  const double e = vfma(-b, r, 1.0);
+  return vfma(r, vfma(e, e, e), r);
 }
 __device__ __forceinline__ float fast_rcp(float b) {
   float r = __builtin_amdgcn_rcpf(b);
@@ -62,15 +62,14 @@ __device__ __forceinline__ T fast_div(T a, T b) {
   const T q = a * r;
   return vfma(vfma(-b, q, a), r, q);
 }
-// sqrt(x/y) for x >= 0, y > 0: x·rsqrt(x·y) with two Newton steps on the rsqrt
+// sqrt(x/y) for x >= 0, y > 0: x·rsqrt(x·y), the rsqrt estimate (relative error
+// below 2^-24) refined by one third-order step: e = 1 − a·s², s ← s + s·e·(½ + ⅜e)
 // (x·y is kept at or above the smallest normal so that x = 0 gives 0).
 __device__ __forceinline__ double fast_sqrt_ratio(double x, double y) {
   const double a = vmax(x * y, 2.2250738585072014e-308);
-  double s = __builtin_amdgcn_rsq(a);
-  const double h = 0.5 * a;
-  s = s * vfma(-h * s, s, 1.5);
-  s = s * vfma(-h * s, s, 1.5);
-  return x * s;
+  const double s = __builtin_amdgcn_rsq(a);
+  const double e = vfma(-(a * s), s, 1.0);
+  return x * vfma(s, e * vfma(e, 0.375, 0.5), s);
 }
 __device__ __forceinline__ float fast_sqrt_ratio(float x, float y) {
   const float a = vmax(x * y, 1.17549435e-38f);

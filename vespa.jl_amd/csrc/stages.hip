@@ -27,25 +27,27 @@ __global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL
   out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
 }
 
-template <typename T, int RANK, int TX, int TY, bool FLOOR64, bool PER>
+template <typename T, int RANK, int TX, int TY, bool FLOOR64, bool PER, bool GRAV>
 static void stage_tp(bool stage2, int blocks, hipStream_t s, const void* Us, const void* U0, void* Uout,
                      const double* dtp, double* partials, const GridDev& G) {
   if (stage2)
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64, PER>), dim3(blocks), dim3(TX * TY), 0, s,
-                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64, PER, GRAV>), dim3(blocks), dim3(TX * TY), 0,
+                       s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
   else
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64, PER>), dim3(blocks), dim3(TX * TY), 0, s,
-                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64, PER, GRAV>), dim3(blocks), dim3(TX * TY), 0,
+                       s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
 }
 
-// Grids periodic on every axis run the kernel with the boundary logic compiled out.
+// Grids periodic on every axis run the kernel with the boundary logic compiled
+// out; with self-gravity the gravity form runs (boundary logic kept).
 template <typename T, int RANK, int TX, int TY, bool FLOOR64>
 static void stage_t(bool stage2, int blocks, hipStream_t s, const void* Us, const void* U0, void* Uout,
                     const double* dtp, double* partials, const GridDev& G) {
   bool per = true;
   for (int d = 0; d < RANK; ++d) per = per && G.bc[2 * d] == BC_PERIODIC && G.bc[2 * d + 1] == BC_PERIODIC;
-  if (per) stage_tp<T, RANK, TX, TY, FLOOR64, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
-  else stage_tp<T, RANK, TX, TY, FLOOR64, false>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  if (G.phi) stage_tp<T, RANK, TX, TY, FLOOR64, false, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  else if (per) stage_tp<T, RANK, TX, TY, FLOOR64, true, false>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  else stage_tp<T, RANK, TX, TY, FLOOR64, false, false>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
 }
 
 template <typename T, bool FLOOR64>

@@ -32,6 +32,7 @@
 // the floating-point sum matches the reference scatter bit for bit.
 #pragma once
 
+#include "gravity.h"
 #include "hydro.h"
 
 #ifndef VH_MIN_WAVES
@@ -62,19 +63,23 @@ namespace VH_PROFILE_NS {
 // correctly rounded x ↦ x/w is monotone, so max_i(a_i/w) = (max_i a_i)/w
 // exactly, and the result equals the per-cell form bit for bit. The sound
 // speed is the profile's (exact: sqrt((γp)/ρ); fast: the refined rsqrt form).
+// With self-gravity the free-fall limiter √(4πGρ)/η, η = ½ (src/gravity.jl:449-452),
+// joins the maximum; it is monotone in ρ, so it is evaluated once at the largest ρ.
 template <typename T, int RANK>
 struct CflMax {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, rmax = 0.0;
   __device__ __forceinline__ void add(const Prim<T>& W, T g) {
     const T c = sound(g, W.p, W.r);
     a0 = vmax(a0, double(vabs(W.vx) + c));
     if (RANK > 1) a1 = vmax(a1, double(vabs(W.vy) + c));
     if (RANK > 2) a2 = vmax(a2, double(vabs(W.vz) + c));
+    rmax = vmax(rmax, double(W.r));
   }
-  __device__ __forceinline__ double value(const double* w) const {
+  __device__ __forceinline__ double value(const double* w, bool grav = false, double Gc = 0.0) const {
     double m = a0 / w[0];
     if (RANK > 1) m = vmax(m, a1 / w[1]);
     if (RANK > 2) m = vmax(m, a2 / w[2]);
+    if (grav) m = vmax(m, rmax > 0.0 ? __builtin_sqrt(4.0 * 3.141592653589793 * Gc * rmax) / 0.5 : 0.0);
     return m;
   }
 };
@@ -231,8 +236,10 @@ __device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, 
 // STAGE2 = true: Uout = ½·U0 + ½·(Us − (dt/V)·acc(Us)) written over U0 (point
 // reads only), plus the per-workgroup max of the CFL term of the new state.
 // PER: every axis of the grid is periodic (no ghosts, no boundary faces, no
-// boundary-flux register), resolved at compile time.
-template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64, bool PER>
+// boundary-flux register), resolved at compile time. GRAV: self-gravity on
+// (G.phi set): the gravity source joins the flux sum and the free-fall limiter
+// the CFL maximum.
+template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64, bool PER, bool GRAV>
 __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
                                                           T* Uout, const double* __restrict__ dtp,
                                                           double* __restrict__ partials, GridDev G) {
@@ -551,6 +558,8 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
         }
       }
 
+      if constexpr (GRAV) grav_source_add<T, RANK>(acc, uown, (const T*)G.phi, G, c, gx, gy, k);
+
       Cons<T> o;
       if constexpr (!STAGE2) {
         // _euler_apply!: dst = base − invV·acc (src/driver.jl:297-307).
@@ -583,7 +592,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
 
   if constexpr (STAGE2) {
     static_assert(sizeof(S.w) >= (NT / 64) * sizeof(double), "reduction scratch");
-    const double m = block_max<NT>(cfl.value(G.w), reinterpret_cast<double*>(&S.w[0][0][0]));
+    const double m = block_max<NT>(cfl.value(G.w, GRAV, G.gravG), reinterpret_cast<double*>(&S.w[0][0][0]));
     if (tid == 0) partials[blockIdx.x] = m;
   }
 }
@@ -597,7 +606,7 @@ __global__ void __launch_bounds__(256) cfl_kernel(const T* __restrict__ U, GridD
   CflMax<T, RANK> cfl;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < G.ncell; c += (long long)gridDim.x * 256)
     cfl.add(cons2prim<T, FLOOR64>(load_cons(U, G.ncell, c), g, G.gamma), g);
-  const double m = block_max<256>(cfl.value(G.w), red);
+  const double m = block_max<256>(cfl.value(G.w, G.phi != nullptr, G.gravG), red);
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 

@@ -126,7 +126,29 @@ struct vh_grid {
   double* totals_partials;
   hipStream_t stream;
   struct AdjWork* adj;   // reverse-mode work buffers (allocated on first use)
+  struct GravWork* grav; // self-gravity state (vh_grid_enable_gravity)
 };
+
+// Self-gravity of a uniform-grid context (the reference's GravityField,
+// src/gravity.jl:30-45): φ, CG work vectors and solver settings.
+struct GravWork {
+  GravParams P;
+  GravBuffers B;
+};
+
+static void free_grav(vh_grid* g) {
+  GravWork* w = g->grav;
+  if (!w) return;
+  if (w->B.phi) (void)hipFree(w->B.phi);
+  if (w->B.work) (void)hipFree(w->B.work);
+  if (w->B.state) (void)hipFree(w->B.state);
+  if (w->B.scal) (void)hipFree(w->B.scal);
+  if (w->B.partials) (void)hipFree(w->B.partials);
+  if (w->B.host_state) (void)hipHostFree(w->B.host_state);
+  delete w;
+  g->grav = nullptr;
+  g->G.phi = nullptr;
+}
 
 static const int kMaxCflBlocks = 2048;
 
@@ -160,6 +182,16 @@ static int configure(vh_grid* g, int variant, int zchunk) {
     g->npartials = need;
   }
   return 0;
+}
+
+static void enqueue_stage_dt(vh_grid* g, bool stage2, const void* Us, const void* U0, void* Uout,
+                             const double* dtp) {
+  if (g->mode == VH_FAST)
+    fast::launch_stage(g->dtype, g->G.rank, g->variant, stage2, g->stage_blocks, g->stream, Us, U0, Uout, dtp,
+                       g->partials, g->G);
+  else
+    exact::launch_stage(g->dtype, g->G.rank, g->variant, stage2, g->stage_blocks, g->stream, Us, U0, Uout, dtp,
+                        g->partials, g->G);
 }
 
 static void enqueue_stage(vh_grid* g, bool stage2, const void* Us, const void* U0, void* Uout) {
@@ -375,9 +407,10 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
     G.area[a] = ar;
   }
   if (hipGetDevice(&g->device) != hipSuccess) { delete g; return fail(-2, "no HIP device%s", ""); }
-  // Default launch shape per rank (measured on MI355X at 256^3 / 512^2; the
-  // autotuner in the host package can override it per workload).
-  int rc = configure(g, rank == 3 ? 2 : 0, rank == 3 ? 16 : 0);
+  // Default launch shape per rank and state type (measured on MI355X at 256^3 /
+  // 512^2: Float64 16x16 cells, Float32 32x8, 16 z planes per workgroup); the
+  // autotuner in the host package can override it per workload.
+  int rc = configure(g, rank == 3 ? (g->dtype == VH_F64 ? 2 : 0) : 0, rank == 3 ? 16 : 0);
   if (rc) { vh_grid_destroy(g); return rc; }
   const size_t bytes = (size_t)5 * n * g->esize;
   if (hipMalloc(&g->U, bytes) != hipSuccess || hipMalloc(&g->U1, bytes) != hipSuccess ||
@@ -405,6 +438,7 @@ int vh_grid_destroy(vh_grid* g) {
   if (!g) return 0;
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   free_adj_work(g);
+  free_grav(g);
   if (g->U) (void)hipFree(g->U);
   if (g->U1) (void)hipFree(g->U1);
   if (g->partials) (void)hipFree(g->partials);
@@ -506,6 +540,13 @@ int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double
   while (true) {
     if (max_steps > 0 && k >= max_steps) break;
     if (clip && !(t < tfinal * (1 - 1e-12))) break;
+    if (g->grav) {   // φ(ρⁿ), held over the step (src/driver.jl:498)
+      long long it;
+      double rr;
+      g->grav->B.rho = g->U;
+      int rc = grav_solve(g->stream, g->dtype == VH_F64 ? 0 : 1, g->G, g->grav->P, g->grav->B, &it, &rr);
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, tfinal,
                        clip ? 1 : 0, g->clock);
     enqueue_step(g);
@@ -522,6 +563,106 @@ int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double
   VH_HIP(hipStreamSynchronize(g->stream));
   *t_io = clk[0];
   if (steps) *steps = k;
+  return 0;
+}
+
+// ---- self-gravity (src/gravity.jl) ----
+
+int vh_grid_enable_gravity(vh_grid* g, double G, int32_t phi_periodic, int64_t maxiter, double tol,
+                           int32_t project_mean) {
+  if (!g) return fail(-1, "null argument%s", "");
+  if (phi_periodic < 0 || phi_periodic >= (1 << g->G.rank))
+    return fail(-1, "phi_periodic has bits beyond the grid rank%s (%lld)", "", phi_periodic);
+  if (maxiter < 0 || !(tol >= 0.0)) return fail(-1, "need maxiter >= 0 and tol >= 0%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  free_grav(g);
+  GravWork* w = new GravWork();
+  g->grav = w;
+  w->P.G = G;
+  w->P.gper = phi_periodic;
+  w->P.project_mean = project_mean ? 1 : 0;
+  w->P.maxiter = maxiter;
+  w->P.tol = tol;
+  const long long n = g->G.ncell;
+  hipError_t e = hipMalloc(&w->B.phi, g->esize * n);
+  if (e == hipSuccess) e = hipMalloc(&w->B.work, 4 * g->esize * n);
+  if (e == hipSuccess) e = hipMalloc(&w->B.state, kGravStateBytes);
+  if (e == hipSuccess) e = hipMalloc(&w->B.scal, 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&w->B.partials, 4096 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&w->B.host_state, kGravStateBytes);
+  if (e == hipSuccess) e = hipMemsetAsync(w->B.phi, 0, g->esize * n, g->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(w->B.scal, 0, 8 * sizeof(double), g->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(w->B.state, 0, kGravStateBytes, g->stream);
+  // Σ V accumulated cell by cell in Float64, as the reference's projections do.
+  double sumv = 0.0;
+  for (long long i = 0; i < n; ++i) sumv += g->G.V;
+  if (e == hipSuccess) e = hipMemcpyAsync(w->B.scal + 1, &sumv, sizeof(double), hipMemcpyHostToDevice, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) {
+    free_grav(g);
+    return fail(-2, "device allocation for self-gravity failed: %s", hipGetErrorString(e));
+  }
+  g->G.phi = w->B.phi;
+  g->G.gravG = G;
+  g->G.gper = phi_periodic;
+  return 0;
+}
+
+int vh_grid_disable_gravity(vh_grid* g) {
+  if (!g) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  free_grav(g);
+  return 0;
+}
+
+int vh_grid_solve_poisson(vh_grid* g, int64_t* iters, double* relres) {
+  if (!g || !iters || !relres) return fail(-1, "null argument%s", "");
+  if (!g->grav) return fail(-1, "self-gravity is not enabled%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  long long it = 0;
+  g->grav->B.rho = g->U;
+  int rc = grav_solve(g->stream, g->dtype == VH_F64 ? 0 : 1, g->G, g->grav->P, g->grav->B, &it, relres);
+  *iters = it;
+  return rc;
+}
+
+int vh_grid_phi_upload(vh_grid* g, const void* host) {
+  if (!g || !host) return fail(-1, "null argument%s", "");
+  if (!g->grav) return fail(-1, "self-gravity is not enabled%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipMemcpyAsync(g->grav->B.phi, host, g->esize * g->G.ncell, hipMemcpyHostToDevice, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_phi_download(vh_grid* g, void* host) {
+  if (!g || !host) return fail(-1, "null argument%s", "");
+  if (!g->grav) return fail(-1, "self-gravity is not enabled%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  VH_HIP(hipMemcpyAsync(host, g->grav->B.phi, g->esize * g->G.ncell, hipMemcpyDeviceToHost, g->stream));
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_apply_laplacian(vh_grid* g, const void* dx, void* dout) {
+  if (!g || !dx || !dout) return fail(-1, "null argument%s", "");
+  if (!g->grav) return fail(-1, "self-gravity is not enabled%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  grav_apply(g->stream, g->dtype == VH_F64 ? 0 : 1, g->G, g->grav->P.gper, dx, dout);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(g->stream));
+  return 0;
+}
+
+int vh_grid_gravity_accel(vh_grid* g, void* dg) {
+  if (!g || !dg) return fail(-1, "null argument%s", "");
+  if (!g->grav) return fail(-1, "self-gravity is not enabled%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  grav_accel(g->stream, g->dtype == VH_F64 ? 0 : 1, g->G, g->grav->P.gper, g->grav->B.phi, dg);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(g->stream));
   return 0;
 }
 
@@ -736,14 +877,48 @@ struct vh_adjoint {
   double* seg1 = nullptr;
   double* w = nullptr;
   size_t bytes = 0;
+  double* segL = nullptr;    // the last segment's U_k, kept from the forward run
+  double* segL1 = nullptr;   // and its stage-1 states U1_k
+  hipGraphExec_t graph = nullptr;   // the backward sweep (fixed buffers), captured once
+  long long graph_key = -1;         // launch configuration the graph was captured with
   float fwd_ms = 0, bwd_ms = 0;
   bool have_forward = false;
 };
+
+// Reverse sweep over the segments, last first: the last segment's states come
+// from the forward run; every other segment is recomputed from its checkpoint
+// (U_k and U1_k), then the step VJPs run backwards through it on a->w.
+static void enqueue_backward_sweep(vh_adjoint* a) {
+  vh_grid* g = a->g;
+  const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
+  const size_t n5 = 5 * (size_t)g->G.ncell;
+  for (int s = a->nseg - 1; s >= 0; --s) {
+    const int64_t k0 = (int64_t)s * a->every;
+    const int64_t k1 = std::min<int64_t>(a->nsteps, k0 + a->every);
+    double* su = a->seg;
+    double* su1 = a->seg1;
+    if (s == a->nseg - 1) {
+      su = a->segL;
+      su1 = a->segL1;
+    } else {
+      (void)hipMemcpyAsync(a->seg, (char*)a->ckpt + nb * s, nb, hipMemcpyDeviceToDevice, g->stream);
+      for (int64_t k = k0; k < k1; ++k) {
+        double* Uk = a->seg + n5 * (k - k0);
+        double* U1k = a->seg1 + n5 * (k - k0);
+        enqueue_stage_dt(g, false, Uk, Uk, U1k, a->dts + k);
+        if (k + 1 < k1) enqueue_stage_dt(g, true, U1k, Uk, a->seg + n5 * (k + 1 - k0), a->dts + k);
+      }
+    }
+    for (int64_t k = k1 - 1; k >= k0; --k)
+      enqueue_step_vjp(g, su + n5 * (k - k0), su1 + n5 * (k - k0), a->dts + k, a->w, a->w);
+  }
+}
 
 extern "C" {
 
 int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, void* dUbar) {
   if (!g || !dU || !dWbar || !dUbar) return fail(-1, "null argument%s", "");
+  if (g->grav) return fail(-1, "the reverse mode does not cover self-gravity%s", "");
   VH_HIP(hipSetDevice(g->device));
   int rc = ensure_adj_work(g);
   if (rc) return rc;
@@ -771,7 +946,10 @@ int vh_adjoint_destroy(vh_adjoint* a) {
   (void)hipFree(a->ckpt);
   (void)hipFree(a->seg);
   (void)hipFree(a->seg1);
+  (void)hipFree(a->segL);
+  (void)hipFree(a->segL1);
   (void)hipFree(a->w);
+  if (a->graph) (void)hipGraphExecDestroy(a->graph);
   delete a;
   return 0;
 }
@@ -781,6 +959,7 @@ int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t ever
   *out = nullptr;
   if (nsteps < 1 || every < 1) return fail(-1, "need nsteps >= 1 and every >= 1%s", "");
   if (g->dtype != VH_F64) return fail(-1, "the reverse mode is Float64 only%s", "");
+  if (g->grav) return fail(-1, "the reverse mode does not cover self-gravity%s", "");
   VH_HIP(hipSetDevice(g->device));
   int rc = ensure_adj_work(g);
   if (rc) return rc;
@@ -794,12 +973,14 @@ int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t ever
   if (e == hipSuccess) e = hipMalloc(&a->ckpt, nb * a->nseg);
   if (e == hipSuccess) e = hipMalloc(&a->seg, nb * a->every);
   if (e == hipSuccess) e = hipMalloc(&a->seg1, nb * a->every);
+  if (e == hipSuccess) e = hipMalloc(&a->segL, nb * a->every);
+  if (e == hipSuccess) e = hipMalloc(&a->segL1, nb * a->every);
   if (e == hipSuccess) e = hipMalloc(&a->w, nb);
   if (e != hipSuccess) {
     vh_adjoint_destroy(a);
     return fail(-2, "device allocation for the rollout failed: %s", hipGetErrorString(e));
   }
-  a->bytes = sizeof(double) * nsteps + nb * (a->nseg + 2 * (size_t)a->every + 1) + g->adj->bytes;
+  a->bytes = sizeof(double) * nsteps + nb * (a->nseg + 4 * (size_t)a->every + 1) + g->adj->bytes;
   *out = a;
   return 0;
 }
@@ -807,6 +988,7 @@ int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t ever
 int vh_adjoint_forward(vh_adjoint* a, double cfl) {
   if (!a) return fail(-1, "null argument%s", "");
   vh_grid* g = a->g;
+  if (g->grav) return fail(-1, "the reverse mode does not cover self-gravity%s", "");
   VH_HIP(hipSetDevice(g->device));
   const size_t nb = sizeof(double) * 5 * (size_t)g->G.ncell;
   hipEvent_t e0, e1;
@@ -816,10 +998,25 @@ int vh_adjoint_forward(vh_adjoint* a, double cfl) {
   VH_HIP(hipMemcpyAsync(a->ckpt, g->U, nb, hipMemcpyDeviceToDevice, g->stream));
   enqueue_cfl(g);
   int np = g->cfl_blocks;
+  // The last segment's states are kept as the forward run produces them (the
+  // backward pass starts there and needs no recompute for it).
+  const size_t n5 = 5 * (size_t)g->G.ncell;
+  const int64_t klast = (int64_t)(a->nseg - 1) * a->every;
   for (int64_t k = 0; k < a->nsteps; ++k) {
     hipLaunchKernelGGL(dt_finalize_kernel, dim3(1), dim3(256), 0, g->stream, g->partials, np, cfl, 0.0, 0, g->clock);
     hipLaunchKernelGGL(copy_value_kernel, dim3(1), dim3(1), 0, g->stream, g->clock + 1, a->dts + k);
-    enqueue_step(g);
+    if (k < klast) {
+      enqueue_step(g);
+    } else {
+      const int64_t j = k - klast;
+      if (j == 0) VH_HIP(hipMemcpyAsync(a->segL, g->U, nb, hipMemcpyDeviceToDevice, g->stream));
+      double* Uk = a->segL + n5 * j;
+      double* U1k = a->segL1 + n5 * j;
+      double* Un = k + 1 < a->nsteps ? a->segL + n5 * (j + 1) : (double*)g->U;
+      enqueue_stage(g, false, Uk, Uk, U1k);
+      enqueue_stage(g, true, U1k, Uk, Un);
+      hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(1), 0, g->stream, g->clock);
+    }
     np = g->stage_blocks;
     if ((k + 1) % a->every == 0 && k + 1 < a->nsteps)
       VH_HIP(hipMemcpyAsync((char*)a->ckpt + nb * ((k + 1) / a->every), g->U, nb, hipMemcpyDeviceToDevice, g->stream));
@@ -846,34 +1043,27 @@ int vh_adjoint_backward(vh_adjoint* a, const void* dWbar, void* dUbar) {
   VH_HIP(hipEventCreate(&e1));
   VH_HIP(hipEventRecord(e0, g->stream));
   VH_HIP(hipMemcpyAsync(a->w, dWbar, nb, hipMemcpyDeviceToDevice, g->stream));
-  for (int s = a->nseg - 1; s >= 0; --s) {
-    const int64_t k0 = (int64_t)s * a->every;
-    const int64_t k1 = std::min<int64_t>(a->nsteps, k0 + a->every);
-    VH_HIP(hipMemcpyAsync(a->seg, (char*)a->ckpt + nb * s, nb, hipMemcpyDeviceToDevice, g->stream));
-    // recompute the segment's states U_k and stage-1 states U1_k
-    for (int64_t k = k0; k < k1; ++k) {
-      const double* Uk = a->seg + n5 * (k - k0);
-      double* U1k = a->seg1 + n5 * (k - k0);
-      const double* dtp = a->dts + k;
-      if (g->mode == VH_FAST)
-        fast::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, Uk, Uk, U1k, dtp,
-                           g->partials, g->G);
-      else
-        exact::launch_stage(VH_F64, g->G.rank, g->variant, false, g->stage_blocks, g->stream, Uk, Uk, U1k, dtp,
-                            g->partials, g->G);
-      if (k + 1 < k1) {
-        double* Un = a->seg + n5 * (k + 1 - k0);
-        if (g->mode == VH_FAST)
-          fast::launch_stage(VH_F64, g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1k, Uk, Un, dtp,
-                             g->partials, g->G);
-        else
-          exact::launch_stage(VH_F64, g->G.rank, g->variant, true, g->stage_blocks, g->stream, U1k, Uk, Un, dtp,
-                              g->partials, g->G);
-      }
-    }
-    for (int64_t k = k1 - 1; k >= k0; --k)
-      enqueue_step_vjp(g, a->seg + n5 * (k - k0), a->seg1 + n5 * (k - k0), a->dts + k, a->w, a->w);
+  const long long key = (((long long)g->mode * 64 + g->variant) * 2 + g->adj->fused) * (1LL << 32) + g->stage_blocks;
+  if (a->graph && a->graph_key != key) {
+    (void)hipGraphExecDestroy(a->graph);
+    a->graph = nullptr;
   }
+  if (!a->graph) {
+    // The sweep reads and writes only the rollout's own buffers, so one capture
+    // serves every later call: launch gaps between its ~4 kernels per step go away.
+    hipGraph_t gr = nullptr;
+    VH_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_backward_sweep(a);
+    VH_HIP(hipStreamEndCapture(g->stream, &gr));
+    hipError_t e = hipGraphInstantiate(&a->graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      a->graph = nullptr;
+      return fail(-2, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    a->graph_key = key;
+  }
+  VH_HIP(hipGraphLaunch(a->graph, g->stream));
   VH_HIP(hipMemcpyAsync(dUbar, a->w, nb, hipMemcpyDeviceToDevice, g->stream));
   VH_HIP(hipEventRecord(e1, g->stream));
   VH_HIP(hipGetLastError());

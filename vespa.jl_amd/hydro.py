@@ -247,6 +247,7 @@ class Simulation:
         self.flux = flux if flux is not None else HIPFlux()
         self.t = 0.0
         self.step_count = 0
+        self.grav = None   # GravityField when self-gravity is on (gravity.enable_gravity)
         self._lib = load_library()
         rank = mesh.rank
         dims = (C.c_int64 * 3)(*[mesh.dims[d] if d < rank else 1 for d in range(3)])
@@ -447,6 +448,8 @@ def evolve(sim: Simulation, max_steps: int = 0, callback=None, callback_every: i
         tf = sim.problem.tfinal
         n = 0
         while sim.t < tf * (1 - 1e-12) and (max_steps <= 0 or n < max_steps):
+            if sim.grav is not None:
+                sim.grav.solve()   # φ(ρⁿ), held over the step (src/driver.jl:498)
             dt = min(compute_dt(sim), tf - sim.t)
             step(sim, dt)
             n += 1

@@ -11,7 +11,11 @@
 #     exact (bit-identical) or fast arithmetic profile;
 #   * `HIPBlocks`: block-structured AMR meshes (hanging-node faces, flux
 #     registers, subcycling) advanced by one persistent launch per RK stage;
-#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls).
+#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls);
+#   * `hip_enable_gravity!` / `hip_solve_poisson!`: self-gravity of a HIPGrid
+#     (enable_gravity! / solve_poisson!, src/gravity.jl:63-80, 248-322): the
+#     device CG Poisson solve, the gravity source in every RK stage and the
+#     free-fall limiter in the device dt.
 #
 # Load with `include` after `using Vespa`; set ENV["VESPAHIP_LIB"] or keep
 # libvespahip.so next to the package's vespa.jl_amd/ directory.
@@ -68,6 +72,7 @@ mutable struct HIPGrid
     ptr::Ptr{Cvoid}
     ncell::Int
     eltype::DataType
+    rank::Int
 end
 
 function HIPGrid(sim::Vespa.Simulation)
@@ -89,7 +94,7 @@ function HIPGrid(sim::Vespa.Simulation)
     _check(ccall((:vh_grid_create, libvespahip), Cint,
                  (Ref{Ptr{Cvoid}}, Int32, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Float64, Int32),
                  h, Int32(N), dims, lo, hi, bc, Float64(sim.γ), T === Float64 ? VH_F64 : VH_F32))
-    g = HIPGrid(h[], prod(dims), T)
+    g = HIPGrid(h[], prod(dims), T, N)
     finalizer(g) do x
         x.ptr == C_NULL || ccall((:vh_grid_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
         x.ptr = C_NULL
@@ -240,6 +245,42 @@ function hip_step_vjp(g::HIPGrid, U::Vector{Float64}, dt::Real, Ū::Vector{Float
     return out
 end
 
+"""
+    hip_enable_gravity!(g::HIPGrid; G=1.0, bcs=Periodic(), maxiter=500, tol, project_mean)
+
+enable_gravity! (src/gravity.jl:63-80) for the device grid: φ starts at zero;
+non-periodic φ sides are homogeneous Neumann. `hip_evolve!` then solves the
+Poisson equation before each step (src/driver.jl:498).
+"""
+function hip_enable_gravity!(g::HIPGrid; G::Real = 1.0, bcs = Periodic(), maxiter::Integer = 500,
+                             tol::Real = (g.eltype === Float32 ? 1.0e-5 : 1.0e-8),
+                             project_mean::Bool = (bcs isa Periodic))
+    rank = g.rank
+    mask = 0
+    for d in 1:rank
+        b = bcs isa Tuple ? bcs[d] : bcs
+        b isa Periodic && (mask |= 1 << (d - 1))
+    end
+    _check(ccall((:vh_grid_enable_gravity, libvespahip), Cint, (Ptr{Cvoid}, Float64, Int32, Int64, Float64, Int32),
+                 g.ptr, Float64(G), Int32(mask), Int64(maxiter), Float64(tol), Int32(project_mean)))
+    return g
+end
+
+"solve_poisson! (src/gravity.jl:248-322) on the device: returns (iterations, relative residual)."
+function hip_solve_poisson!(g::HIPGrid)
+    it = Ref{Int64}(0)
+    rr = Ref{Float64}(0.0)
+    _check(ccall((:vh_grid_solve_poisson, libvespahip), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Float64}), g.ptr, it, rr))
+    return (it[], rr[])
+end
+
+"φ of the device grid as a flat host vector (x fastest)."
+function hip_phi(g::HIPGrid)
+    out = Vector{g.eltype}(undef, g.ncell)
+    GC.@preserve out _check(ccall((:vh_grid_phi_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, out))
+    return out
+end
+
 # The rrule a maintainer registers with ChainRulesCore (not loaded here):
 #   function ChainRulesCore.rrule(::typeof(hip_step_state), g::HIPGrid, U, dt)
 #       Unew = hip_step_state(g, U, dt)
@@ -247,6 +288,6 @@ end
 #   end
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
-       HIPBlocks, hip_download, hip_step_vjp
+       HIPBlocks, hip_download, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!, hip_phi
 
 end # module
