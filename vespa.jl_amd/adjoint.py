@@ -35,7 +35,7 @@ def _require_f64(sim):
 
 def set_form(sim, fused: bool) -> None:
     """Select the kernel form of the reverse mode for this simulation: the
-    single tiled kernel (2D, periodic, sizes multiple of 32×8) or the
+    single tiled kernel (2D, periodic, sizes multiple of 32×16) or the
     three-pass form."""
     _require_f64(sim)
     check(load_library().vh_grid_set_adjoint_form(sim._h, int(bool(fused))))

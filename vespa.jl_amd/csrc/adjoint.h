@@ -520,51 +520,65 @@ __global__ void __launch_bounds__(256) adj_cell_kernel(const double* __restrict_
 // computes W for the tile and a cross-shaped halo of three cells, every face
 // adjoint its cells need (the tile's faces plus one more on each side), and the
 // gather, in LDS; one launch instead of four, no face buffers in HBM.
-constexpr int JTX = 32, JTY = 8;
+constexpr int JTX = 32, JTY = 16, JNT = 512;
+constexpr int JNXF = JTY * (JTX + 3);   // x faces of a tile (one extra on the lo side, two on the hi)
+constexpr int JNYF = (JTY + 3) * JTX;   // y faces of a tile
 
+// One 32×16 tile per workgroup of 512 lanes (one workgroup per compute unit:
+// 256 VGPRs per lane, 150 KiB of LDS).
 struct JatLDS {
-  double Wx[5][JTY][JTX + 6];   // W, rows y0..y0+7, cols x0-3..x0+34
-  double Wy[5][6][JTX];         // W, rows y0-3..y0-1 and y0+8..y0+10, cols x0..x0+31
-  double Lx[5][JTY][JTX + 4];   // λ, rows y0..y0+7, cols x0-2..x0+33
-  double Ly[5][4][JTX];         // λ, rows y0-2, y0-1, y0+8, y0+9
-  double G[10][(JTY + 3) * JTX];  // face-state adjoints (left 0..4, right 5..9) of the current axis
+  double Wx[5][JTY][JTX + 6];   // W, rows y0..y0+15, cols x0-3..x0+34
+  double Wy[5][6][JTX];         // W, rows y0-3..y0-1 and y0+16..y0+18, cols x0..x0+31
+  double Lx[5][JTY][JTX + 4];   // λ, rows y0..y0+15, cols x0-2..x0+33
+  double Ly[5][4][JTX];         // λ, rows y0-2, y0-1, y0+16, y0+17
+  double Gx[10][JNXF];          // x-face state adjoints (left 0..4, right 5..9)
+  double Gy[10][JNYF];          // y-face state adjoints
 };
 
-__device__ __forceinline__ D5 jat_get(const double (*a)[JTY][JTX + 6], int r, int c) {
+// Flat LDS offsets of the cross-shaped W and λ halos: (row, col) in tile
+// coordinates, row ∈ [−3, JTY+2] with col ∈ [0, JTX) or row ∈ [0, JTY) with
+// col ∈ [−3, JTX+2]; returns the address of field 0 and the field pitch.
+__device__ __forceinline__ const double* jat_w_at(const JatLDS& S, int row, int col, int& pitch) {
+  if (row >= 0 && row < JTY) {
+    pitch = JTY * (JTX + 6);
+    return &S.Wx[0][row][col + 3];
+  }
+  pitch = 6 * JTX;
+  return &S.Wy[0][row < 0 ? row + 3 : 3 + row - JTY][col];
+}
+__device__ __forceinline__ const double* jat_l_at(const JatLDS& S, int row, int col, int& pitch) {
+  if (row >= 0 && row < JTY) {
+    pitch = JTY * (JTX + 4);
+    return &S.Lx[0][row][col + 2];
+  }
+  pitch = 4 * JTX;
+  return &S.Ly[0][row < 0 ? row + 2 : 2 + row - JTY][col];
+}
+__device__ __forceinline__ D5 jat_w(const JatLDS& S, int row, int col) {
+  int p;
+  const double* a = jat_w_at(S, row, col, p);
   D5 w;
-  w.r = a[0][r][c]; w.vx = a[1][r][c]; w.vy = a[2][r][c]; w.vz = a[3][r][c]; w.p = a[4][r][c];
+  w.r = a[0]; w.vx = a[p]; w.vy = a[2 * p]; w.vz = a[3 * p]; w.p = a[4 * p];
   return w;
 }
-// W at tile row lr ∈ [−3, JTY+2] of tile column c ∈ [0, JTX).
-__device__ __forceinline__ D5 jat_wy(const JatLDS& S, int lr, int c) {
-  if (lr >= 0 && lr < JTY) return jat_get(S.Wx, lr, c + 3);
-  const int i = lr < 0 ? lr + 3 : 3 + lr - JTY;
-  D5 w;
-  w.r = S.Wy[0][i][c]; w.vx = S.Wy[1][i][c]; w.vy = S.Wy[2][i][c]; w.vz = S.Wy[3][i][c]; w.p = S.Wy[4][i][c];
-  return w;
-}
-__device__ __forceinline__ Cons<double> jat_lx(const JatLDS& S, int r, int lc) {
+__device__ __forceinline__ Cons<double> jat_l(const JatLDS& S, int row, int col) {
+  int p;
+  const double* a = jat_l_at(S, row, col, p);
   Cons<double> l;
-  l.d = S.Lx[0][r][lc]; l.mx = S.Lx[1][r][lc]; l.my = S.Lx[2][r][lc]; l.mz = S.Lx[3][r][lc]; l.e = S.Lx[4][r][lc];
+  l.d = a[0]; l.mx = a[p]; l.my = a[2 * p]; l.mz = a[3 * p]; l.e = a[4 * p];
   return l;
 }
-// λ at tile row lr ∈ [−2, JTY+1] of tile column c.
-__device__ __forceinline__ Cons<double> jat_ly(const JatLDS& S, int lr, int c) {
-  if (lr >= 0 && lr < JTY) return jat_lx(S, lr, c + 2);
-  const int i = lr < 0 ? lr + 2 : 2 + lr - JTY;
-  Cons<double> l;
-  l.d = S.Ly[0][i][c]; l.mx = S.Ly[1][i][c]; l.my = S.Ly[2][i][c]; l.mz = S.Ly[3][i][c]; l.e = S.Ly[4][i][c];
-  return l;
-}
-__device__ __forceinline__ D5 jat_g(const JatLDS& S, int side, int i) {
+__device__ __forceinline__ D5 jat_gx(const JatLDS& S, int side, int i) {
   const int o = 5 * side;
   D5 w;
-  w.r = S.G[o][i]; w.vx = S.G[o + 1][i]; w.vy = S.G[o + 2][i]; w.vz = S.G[o + 3][i]; w.p = S.G[o + 4][i];
+  w.r = S.Gx[o][i]; w.vx = S.Gx[o + 1][i]; w.vy = S.Gx[o + 2][i]; w.vz = S.Gx[o + 3][i]; w.p = S.Gx[o + 4][i];
   return w;
 }
-__device__ __forceinline__ void jat_put_g(JatLDS& S, int i, const D5& l, const D5& r) {
-  S.G[0][i] = l.r; S.G[1][i] = l.vx; S.G[2][i] = l.vy; S.G[3][i] = l.vz; S.G[4][i] = l.p;
-  S.G[5][i] = r.r; S.G[6][i] = r.vx; S.G[7][i] = r.vy; S.G[8][i] = r.vz; S.G[9][i] = r.p;
+__device__ __forceinline__ D5 jat_gy(const JatLDS& S, int side, int i) {
+  const int o = 5 * side;
+  D5 w;
+  w.r = S.Gy[o][i]; w.vx = S.Gy[o + 1][i]; w.vy = S.Gy[o + 2][i]; w.vz = S.Gy[o + 3][i]; w.p = S.Gy[o + 4][i];
+  return w;
 }
 __device__ __forceinline__ D5 half_diff(const D5& a, const D5& b) {
   D5 s;
@@ -583,6 +597,18 @@ __device__ __forceinline__ Cons<double> mu_of(double a, const Cons<double>& l, c
   m.mz = a * l.mz - a * r.mz;
   m.e = a * l.e - a * r.e;
   return m;
+}
+__device__ __forceinline__ D5 d5_swap_xy(const D5& w, bool on) {
+  D5 o = w;
+  o.vx = on ? w.vy : w.vx;
+  o.vy = on ? w.vx : w.vy;
+  return o;
+}
+__device__ __forceinline__ Cons<double> c_swap_xy(const Cons<double>& f, bool on) {
+  Cons<double> o = f;
+  o.mx = on ? f.my : f.mx;
+  o.my = on ? f.mx : f.my;
+  return o;
 }
 
 // Slope adjoint terms gathered by a cell from its own slope and its two
@@ -603,7 +629,13 @@ __device__ __forceinline__ void gather_line(const D5 w[5], const D5 gl[4], const
   d5_add(Wb, rb3);
 }
 
-__global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double* __restrict__ U,
+// out = a1·base1 + a2·base2 + beta·(dt/V)·A'(U)ᵀλ for a 2D grid periodic on both
+// axes, one launch. Phases (two barriers): load the tile's W (halo 3) and λ (halo
+// 2) in a cross shape; evaluate every x and y face VJP of the tile plus the faces
+// its cells' slope adjoints reach (560 + 608 faces, three passes over the 512
+// lanes; a y face runs in the frame of an x face, x and y exchanged); gather
+// both axes per cell and map the primitive adjoint to conserved variables.
+__global__ void __launch_bounds__(JNT, 1) adj_jat2d_periodic_kernel(const double* __restrict__ U,
                                                                  const double* __restrict__ lam,
                                                                  const double* base1, double a1,
                                                                  const double* base2, double a2, double beta,
@@ -619,10 +651,6 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
   auto wrapx = [&](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
   auto wrapy = [&](int y) { return y < 0 ? y + ny : (y >= ny ? y - ny : y); };
   auto cidx = [&](int x, int y) { return (long long)wrapy(y) * nx + wrapx(x); };
-  auto put5 = [](double* a0, int stride, int i, const D5& w) {
-    a0[i] = w.r; a0[stride + i] = w.vx; a0[2 * stride + i] = w.vy; a0[3 * stride + i] = w.vz; a0[4 * stride + i] = w.p;
-  };
-  (void)put5;
   // ---- load: own cell, then the cross-shaped halos of W (3) and λ (2) ----
   const long long c = (long long)(y0 + ty) * nx + (x0 + tx);
   const Cons<double> Uc = load_cons(U, n, c);
@@ -634,12 +662,10 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
     S.Lx[0][ty][tx + 2] = l.d; S.Lx[1][ty][tx + 2] = l.mx; S.Lx[2][ty][tx + 2] = l.my;
     S.Lx[3][ty][tx + 2] = l.mz; S.Lx[4][ty][tx + 2] = l.e;
   }
-  // Loops whose item count is not a multiple of 256 start at a wave that
-  // depends on the workgroup, so the partial last pass lands on different SIMDs
-  // for the workgroups that share a compute unit.
-  const int lt = (tid + 64 * (int)((blockIdx.x + blockIdx.y * gridDim.x) & 3)) & 255;
   constexpr int NWX = 6 * JTY, NLX = 4 * JTY, NWY = 6 * JTX, NLY = 4 * JTX;
-  for (int h = lt; h < NWX + NLX + NWY + NLY; h += 256) {
+  static_assert(NWX + NLX + NWY + NLY <= JNT, "one halo item per lane");
+  {
+    const int h = tid;
     if (h < NWX) {
       const int r = h / 6, q = h % 6;
       const int lc = q < 3 ? q : JTX + q;   // 0,1,2 | JTX+3..JTX+5
@@ -657,7 +683,7 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
       const int y = i < 3 ? y0 - 3 + i : y0 + JTY + (i - 3);
       const D5 w = cons2prim<double, false>(load_cons(U, n, cidx(x0 + cc, y)), g, g);
       S.Wy[0][i][cc] = w.r; S.Wy[1][i][cc] = w.vx; S.Wy[2][i][cc] = w.vy; S.Wy[3][i][cc] = w.vz; S.Wy[4][i][cc] = w.p;
-    } else {
+    } else if (h < NWX + NLX + NWY + NLY) {
       const int hh = h - NWX - NLX - NWY;
       const int i = hh / JTX, cc = hh % JTX;
       const int y = i < 2 ? y0 - 2 + i : y0 + JTY + (i - 2);
@@ -667,53 +693,63 @@ __global__ void __launch_bounds__(256, 2) adj_jat2d_periodic_kernel(const double
   }
   __syncthreads();
 
-  D5 Wb = d5_zero();
-  // ---- x faces: ql = 0..JTX+2 (face between columns x0−2+ql and x0−1+ql) ----
-  for (int h = lt; h < JTY * (JTX + 3); h += 256) {
-    const int r = h / (JTX + 3), ql = h % (JTX + 3);
-    const D5 w0 = jat_get(S.Wx, r, ql), w1 = jat_get(S.Wx, r, ql + 1), w2 = jat_get(S.Wx, r, ql + 2),
-             w3 = jat_get(S.Wx, r, ql + 3);
-    const D5 WL = face_value(w1, slope(w0, w1, w2), 0.5);
-    const D5 WR = face_value(w2, slope(w1, w2, w3), -0.5);
-    const Cons<double> mu = mu_of(G.area[0], jat_lx(S, r, ql), jat_lx(S, r, ql + 1));
+  // ---- face VJPs, x faces then y faces in one item list ----
+  //   x face (r, ql), ql = 0..JTX+2: between columns ql−2 and ql−1 of row r;
+  //   y face (ql, cc), ql = 0..JTY+2: between rows ql−2 and ql−1 of column cc.
+  for (int h = tid; h < JNXF + JNYF; h += JNT) {
+    const bool isy = h >= JNXF;
+    const int q = isy ? h - JNXF : h;
+    int row, col, dr, dc;
+    if (!isy) {
+      row = q / (JTX + 3);
+      col = q % (JTX + 3) - 3;   // cell ql−3 (the face's w0)
+      dr = 0;
+      dc = 1;
+    } else {
+      row = q / JTX - 3;
+      col = q % JTX;
+      dr = 1;
+      dc = 0;
+    }
+    const D5 w0 = jat_w(S, row, col), w1 = jat_w(S, row + dr, col + dc), w2 = jat_w(S, row + 2 * dr, col + 2 * dc),
+             w3 = jat_w(S, row + 3 * dr, col + 3 * dc);
+    const D5 WL = d5_swap_xy(face_value(w1, slope(w0, w1, w2), 0.5), isy);
+    const D5 WR = d5_swap_xy(face_value(w2, slope(w1, w2, w3), -0.5), isy);
+    const Cons<double> mu =
+        c_swap_xy(mu_of(isy ? G.area[1] : G.area[0], jat_l(S, row + dr, col + dc), jat_l(S, row + 2 * dr, col + 2 * dc)),
+                  isy);
     D5 bL = d5_zero(), bR = d5_zero();
     hllc_vjp<0>(WL, WR, g, mu, bL, bR);
-    jat_put_g(S, r * (JTX + 3) + ql, bL, bR);
+    bL = d5_swap_xy(bL, isy);
+    bR = d5_swap_xy(bR, isy);
+    double* gb = isy ? &S.Gy[0][q] : &S.Gx[0][q];
+    const int gp = isy ? JNYF : JNXF;
+    gb[0] = bL.r; gb[gp] = bL.vx; gb[2 * gp] = bL.vy; gb[3 * gp] = bL.vz; gb[4 * gp] = bL.p;
+    gb[5 * gp] = bR.r; gb[6 * gp] = bR.vx; gb[7 * gp] = bR.vy; gb[8 * gp] = bR.vz; gb[9 * gp] = bR.p;
   }
   __syncthreads();
+
+  // ---- gather both axes per cell ----
+  D5 Wb = d5_zero();
   {
     D5 w[5], gl[4], gr[4];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) w[k] = jat_get(S.Wx, ty, tx + 1 + k);
+    for (int k = 0; k < 5; ++k) w[k] = jat_w(S, ty, tx - 2 + k);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      gl[k] = jat_g(S, 0, ty * (JTX + 3) + tx + k);
-      gr[k] = jat_g(S, 1, ty * (JTX + 3) + tx + k);
+      gl[k] = jat_gx(S, 0, ty * (JTX + 3) + tx + k);
+      gr[k] = jat_gx(S, 1, ty * (JTX + 3) + tx + k);
     }
     gather_line(w, gl, gr, Wb);
   }
-  __syncthreads();
-  // ---- y faces: ql = 0..JTY+2 (face between rows y0−2+ql and y0−1+ql) ----
-  for (int h = (lt + 128) & 255; h < (JTY + 3) * JTX; h += 256) {
-    const int ql = h / JTX, cc = h % JTX;
-    const D5 w0 = jat_wy(S, ql - 3, cc), w1 = jat_wy(S, ql - 2, cc), w2 = jat_wy(S, ql - 1, cc),
-             w3 = jat_wy(S, ql, cc);
-    const D5 WL = face_value(w1, slope(w0, w1, w2), 0.5);
-    const D5 WR = face_value(w2, slope(w1, w2, w3), -0.5);
-    const Cons<double> mu = mu_of(G.area[1], jat_ly(S, ql - 2, cc), jat_ly(S, ql - 1, cc));
-    D5 bL = d5_zero(), bR = d5_zero();
-    hllc_vjp<1>(WL, WR, g, mu, bL, bR);
-    jat_put_g(S, ql * JTX + cc, bL, bR);
-  }
-  __syncthreads();
   {
     D5 w[5], gl[4], gr[4];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) w[k] = jat_wy(S, ty - 2 + k, tx);
+    for (int k = 0; k < 5; ++k) w[k] = jat_w(S, ty - 2 + k, tx);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      gl[k] = jat_g(S, 0, (ty + k) * JTX + tx);
-      gr[k] = jat_g(S, 1, (ty + k) * JTX + tx);
+      gl[k] = jat_gy(S, 0, (ty + k) * JTX + tx);
+      gr[k] = jat_gy(S, 1, (ty + k) * JTX + tx);
     }
     gather_line(w, gl, gr, Wb);
   }

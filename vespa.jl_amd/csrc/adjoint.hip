@@ -56,8 +56,8 @@ bool adj_launch_jat_fused(hipStream_t s, const double* U, const double* lam, con
   const bool periodic = G.bc[0] == BC_PERIODIC && G.bc[2] == BC_PERIODIC;
   if (G.rank != 2 || !periodic || G.n[0] % VH_PROFILE_NS::JTX || G.n[1] % VH_PROFILE_NS::JTY) return false;
   const dim3 grid(G.n[0] / VH_PROFILE_NS::JTX, G.n[1] / VH_PROFILE_NS::JTY);
-  hipLaunchKernelGGL(VH_PROFILE_NS::adj_jat2d_periodic_kernel, grid, dim3(256), 0, s, U, lam, base1, a1, base2, a2, beta,
-                     dtp, G, out);
+  hipLaunchKernelGGL(VH_PROFILE_NS::adj_jat2d_periodic_kernel, grid, dim3(VH_PROFILE_NS::JNT), 0, s, U, lam, base1, a1,
+                     base2, a2, beta, dtp, G, out);
   return true;
 }
 

@@ -29,6 +29,14 @@
 namespace vh {
 namespace VH_PROFILE_NS {
 
+// Waves per SIMD the block stage kernel is compiled for (Float64 / Float32 state).
+#ifndef VH_BLOCK_MIN_WAVES
+#define VH_BLOCK_MIN_WAVES 3
+#endif
+#ifndef VH_BLOCK_MIN_WAVES_F32
+#define VH_BLOCK_MIN_WAVES_F32 3
+#endif
+
 constexpr int BNT = 256;     // lanes per workgroup
 constexpr int BW = 432;      // max (TY+4)*(TX+4) over the tile shapes used
 constexpr int BFS = 592;     // max TY*(TX+2) + (TY+2)*TX (TX, TY <= 32, TX*TY <= 256)
@@ -149,7 +157,7 @@ __device__ __forceinline__ void add_link_group(Cons<T>& acc, const BlocksDev& B,
 // w + gridDim.x, ...). STAGE2 as in rk_stage_kernel; the CFL partial of the
 // new state covers the workgroup's tiles.
 template <typename T, int RANK, bool STAGE2, bool FLOOR64>
-__global__ void __launch_bounds__(BNT, VH_MIN_WAVES) block_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
+__global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 : VH_BLOCK_MIN_WAVES) block_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
                                                                        const double* __restrict__ dtp,
                                                                        double* __restrict__ partials, BlocksDev B) {
   __shared__ BlockLDS<T> S;
