@@ -304,6 +304,26 @@ int vh_blocks_set_tiling(vh_blocks* m, int32_t max_tile_x);
 int vh_blocks_info(const vh_blocks* m, int64_t* counts6, int32_t* zchunk, int32_t* workgroups);
 int vh_blocks_sync(vh_blocks* m);
 
+/* Regrid support (regrid!, src/driver.jl:440-462).
+ *
+ * vh_blocks_indicator: density_gradient_indicator (src/driver.jl:400-414) of
+ * every leaf on the device, the largest |ρn − ρc| / max(ρc, ρn, eps) over the
+ * representative neighbours across the 2·rank faces that are interior (domain
+ * boundaries are skipped, periodic sides wrap inside the block; the difference
+ * at the state's precision, the ratio in Float64). block_max (host, nblocks
+ * doubles) receives the per-block maximum; cell_g (host, ncell doubles, may be
+ * NULL) the per-leaf values. Returns after the values are on the host. */
+int vh_blocks_indicator(vh_blocks* m, double* block_max, double* cell_g);
+
+/* vh_blocks_remap: conservative transfer of src's state onto dst's mesh, device
+ * to device (the remap refine! triggers, prolong! by injection,
+ * lib/RefMesh/src/RefMesh.jl:284-302). src_block[b] names the block of src that
+ * contains block b of dst, at the same or a coarser level; every cell of b takes
+ * the conserved values of the src cell containing it, so Σ U·V is unchanged.
+ * Both contexts need the same dtype, device, rank, maxlevel and h0. dst's state
+ * is replaced; returns after the copy has completed. */
+int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_block);
+
 /* Time nsteps single-rate steps (device dt) with HIP events after two untimed
  * steps. method 0: one persistent launch per stage, eager; 1: one launch per
  * block per stage (the comparison loop); 2: persistent launches replayed from a

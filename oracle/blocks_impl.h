@@ -186,6 +186,34 @@ void BN(bo_totals)(const bo_mesh* m, const BT* U, double out[5]) {
     }
 }
 
+/* density_gradient_indicator (src/driver.jl:400-414): per leaf, the largest
+ * |ρn − ρc| / max(ρc, ρn, eps()) over the representative neighbours across the
+ * 2·rank faces that are Interior (a domain boundary is skipped; a periodic side
+ * wraps inside the block). The difference is taken in BT and the ratio in double,
+ * as Julia promotes a Float32 difference against eps()::Float64. block_max[b] is
+ * the largest value over the leaves of block b; cell_g may be NULL. */
+void BN(bo_indicator)(const bo_mesh* m, const BT* U, double* cell_g, double* block_max) {
+    for (int b = 0; b < m->nblocks; ++b) {
+        double gb = 0.0;
+        for (int64_t c = m->offset[b]; c < m->offset[b + 1]; ++c) {
+            BT rc = U[c];
+            double g = 0.0;
+            for (int d = 0; d < m->rank; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    int64_t rep;
+                    if (bo_neighbor(m, c, d, side, &rep, NULL, NULL) == NB_BOUNDARY) continue;
+                    BT rn = U[rep];
+                    double num = (double)BFABS(rn - rc);
+                    double den = vo_jmax_f64_(vo_jmax_f64_((double)rc, (double)rn), DBL_EPSILON);
+                    g = vo_jmax_f64_(g, num / den);
+                }
+            if (cell_g) cell_g[c] = g;
+            gb = vo_jmax_f64_(gb, g);
+        }
+        block_max[b] = gb;
+    }
+}
+
 static int BN(bo_max_level)(const bo_mesh* m) {
     int l = 0;
     for (int b = 0; b < m->nblocks; ++b)

@@ -24,6 +24,7 @@
  * across a coarse↔fine face; this restatement uses the fine cell that contains
  * the lo corner of the coarse face (parity with HG on that choice is unpinned).
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>

@@ -361,6 +361,7 @@ class BlockOracle:
                 getattr(L, f"bo_evolve_{suf}").argtypes = [vp, vp, C.c_double, C.c_double, C.c_int64,
                                                            C.c_int, C.c_int, C.POINTER(C.c_double)]
                 getattr(L, f"bo_evolve_{suf}").restype = C.c_int64
+                getattr(L, f"bo_indicator_{suf}").argtypes = [vp, vp, vp, vp]
             L._bo_typed = True
         self.L = L
         self.mesh = mesh
@@ -406,12 +407,57 @@ class BlockOracle:
         getattr(self.L, f"bo_totals_{_suf(U.dtype)}")(self.h, U.ctypes.data, out.ctypes.data)
         return out
 
+    def indicator(self, U):
+        """density_gradient_indicator per leaf and its per-block maximum
+        (blocks_impl.h bo_indicator) → (block_max, cell_g)."""
+        U = np.ascontiguousarray(U)
+        cell = np.empty(self.ncell)
+        bmax = np.empty(len(self.mesh.level))
+        getattr(self.L, f"bo_indicator_{_suf(U.dtype)}")(self.h, U.ctypes.data, cell.ctypes.data, bmax.ctypes.data)
+        return bmax, cell
+
     def evolve(self, U, tfinal, cfl, max_steps=0, subcycle=False, floor64=False, t0=0.0):
         assert U.flags.c_contiguous
         t = C.c_double(t0)
         k = getattr(self.L, f"bo_evolve_{_suf(U.dtype)}")(self.h, U.ctypes.data, tfinal, cfl, int(max_steps),
                                                          int(subcycle), int(floor64), C.byref(t))
         return t.value, int(k)
+
+
+def blocks_remap(dst, src, src_block, U_src: np.ndarray) -> np.ndarray:
+    """Injection remap between two block meshes (prolong!,
+    lib/RefMesh/src/RefMesh.jl:284-302): every cell of block b of `dst` takes
+    the fields of the cell of block src_block[b] of `src` that contains it.
+    `dst`/`src` are BlockMesh-like (rank, level, dims, origin, maxlevel)."""
+    def offsets(m):
+        return np.concatenate([[0], np.cumsum(np.prod(np.asarray(m.dims, dtype=np.int64), axis=1))])
+    so, do = offsets(src), offsets(dst)
+    nf = U_src.shape[0]
+    out = np.empty((nf, int(do[-1])), dtype=U_src.dtype)
+    for b in range(len(dst.level)):
+        p = int(src_block[b])
+        shift = int(dst.level[b]) - int(src.level[p])
+        if shift < 0:
+            raise ValueError("destination block coarser than its source")
+        sspan = 1 << (src.maxlevel - int(src.level[p]))
+        nd = [int(v) for v in dst.dims[b]]
+        ns = [int(v) for v in src.dims[p]]
+        idx = []
+        for d in range(3):
+            if d >= dst.rank:
+                idx.append(np.zeros(nd[d], dtype=np.int64))
+                continue
+            rel = int(dst.origin[b][d]) - int(src.origin[p][d])
+            if rel < 0 or rel % sspan:
+                raise ValueError("destination block not aligned to its source")
+            ii = rel // sspan + (np.arange(nd[d]) >> shift)
+            if ii[-1] >= ns[d]:
+                raise ValueError("destination block not inside its source")
+            idx.append(ii)
+        K, J, I = np.meshgrid(idx[2], idx[1], idx[0], indexing="ij")
+        sc = so[p] + (K * ns[1] + J) * ns[0] + I
+        out[:, do[b]:do[b + 1]] = U_src[:, sc.ravel()]
+    return out
 
 
 # ---------------- self-gravity (src/gravity.jl) ----------------

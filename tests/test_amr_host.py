@@ -1,6 +1,8 @@
-"""Host logic of the dynamic block AMR driver (no device): tagging, refinement
-with 2:1 balance, conservative remap, and the block mesh it hands to the
-library's topology builder."""
+"""Host logic of the dynamic block AMR driver (no device): tagging from the
+per-block indicator maxima, refinement with 2:1 balance, the source-block table
+of the remap, and the block mesh it hands to the library's topology builder.
+The indicator and the remap themselves come from the oracle here (the device
+computes them in a run)."""
 import numpy as np
 
 from conftest import load_vespa
@@ -16,14 +18,19 @@ def _bump(qb, width):
     return U
 
 
-def test_refine_balance_and_conservative_remap():
+def _regrid(O, qb, U, thr):
+    old, old_mesh = list(qb.blocks), qb.mesh()
+    bmax, _ = O.BlockOracle(old_mesh).indicator(U)
+    qb.refine(qb.tags_from_indicator(bmax, thr, symmetric=True))
+    return O.blocks_remap(qb.mesh(), old_mesh, qb.source_blocks(old), U)
+
+
+def test_refine_balance_and_conservative_remap(O):
     qb = amr.QuadBlocks(6, 8, 3)
     U = _bump(qb, 0.01)
     tot0 = (U * qb.volumes()).sum(axis=1)
     for _ in range(4):
-        old = list(qb.blocks)
-        qb.refine(qb.tag(U[0], 0.02))
-        U = qb.remap(old, U)
+        U = _regrid(O, qb, U, 0.02)
         # injection keeps every conserved total (to round-off of the sums)
         np.testing.assert_allclose((U * qb.volumes()).sum(axis=1), tot0, rtol=1e-14, atol=1e-16)
         m = qb.mesh()
@@ -37,19 +44,33 @@ def test_refine_balance_and_conservative_remap():
     assert abs(area - 1.0) < 1e-12
 
 
-def test_mesh_is_four_fold_symmetric():
+def test_mesh_is_four_fold_symmetric(O):
     qb = amr.QuadBlocks(6, 8, 3)
     U = _bump(qb, 0.02)
     for _ in range(3):
-        old = list(qb.blocks)
-        qb.refine(qb.tag(U[0], 0.02))
-        U = qb.remap(old, U)
+        U = _regrid(O, qb, U, 0.02)
     n = {}
     for (l, ix, iy) in qb.blocks:
         nl = 6 << l
         n[(l, ix, iy)] = 1
         for img in ((l, nl - 1 - ix, iy), (l, ix, nl - 1 - iy), (l, iy, ix)):
             assert img in set(qb.blocks), img
+
+
+def test_tags_and_source_blocks():
+    """Only blocks below maxlevel above the threshold are tagged; mirror images
+    join with symmetric=True; each child's source is its parent."""
+    qb = amr.QuadBlocks(2, 4, 1)
+    bm = np.zeros(4)
+    bm[0] = 1.0                                   # block (0, 0, 0)
+    assert qb.tags_from_indicator(bm, 0.5) == {(0, 0, 0)}
+    assert qb.tags_from_indicator(bm, 0.5, symmetric=True) == set(qb.blocks)
+    old = list(qb.blocks)
+    qb.refine({(0, 0, 0)})
+    src = qb.source_blocks(old)
+    for q, (l, ix, iy) in enumerate(qb.blocks):
+        assert old[src[q]] == ((l, ix, iy) if l == 0 else (0, ix >> 1, iy >> 1))
+    assert qb.tags_from_indicator(np.ones(len(qb.blocks)), 0.5) == {b for b in qb.blocks if b[0] == 0}
 
 
 def test_sedov_initial_state_deposits_the_energy():

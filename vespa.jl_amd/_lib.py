@@ -30,6 +30,7 @@ EXPORTED = (
     "vh_blocks_set_mode", "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt",
     "vh_blocks_step", "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch",
     "vh_blocks_info", "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling",
+    "vh_blocks_indicator", "vh_blocks_remap",
     "vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
     "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form",
     "vh_grid_step_bflux", "vh_grid_bflux_count",
@@ -121,10 +122,13 @@ def load_library(path: str | None = None):
     L.vh_blocks_set_tiling.argtypes = [vp, C.c_int32]
     L.vh_blocks_sync.argtypes = [vp]
     L.vh_blocks_time_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, dp, dp]
+    L.vh_blocks_indicator.argtypes = [vp, dp, dp]
+    L.vh_blocks_remap.argtypes = [vp, vp, i32p]
     for n in ("vh_blocks_describe", "vh_blocks_create", "vh_blocks_destroy", "vh_blocks_set_mode",
               "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt", "vh_blocks_step",
               "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch", "vh_blocks_info",
-              "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling"):
+              "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling", "vh_blocks_indicator",
+              "vh_blocks_remap"):
         getattr(L, n).restype = C.c_int
     L.vh_grid_step_vjp.argtypes = [vp, vp, C.c_double, vp, vp]
     L.vh_adjoint_create.argtypes = [C.POINTER(vp), vp, C.c_int64, C.c_int32]

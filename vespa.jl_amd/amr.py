@@ -5,18 +5,24 @@ The reference refines leaves with a density-gradient indicator every `every`
 steps and once before the first step, refine-only, with a conservative remap
 (src/driver.jl:392-462; HGBackend refine + remap, lib/HGBackend/src/HGBackend.jl;
 injection / volume mean, lib/RefMesh/src/RefMesh.jl:250-302). Here the unit of
-refinement is a block of B×B cells: a block is refined (split into four
-children of B×B cells at the next level) when any of its cells has
-``max |Δρ| / max(ρ_c, ρ_n)`` over its face neighbours above `refine_above`
-(density_gradient_indicator, src/driver.jl:400-413), then 2:1 balance is
-restored across block faces. The remap injects each parent cell's conserved
-values into its 2×2 children, so Σ U·V is unchanged. Between regrids the leaves
-are advanced on the GPU by the batched kernel (one persistent launch per RK
-stage).
+refinement is a block of B×B cells: a block below maxlevel is refined (split
+into four children of B×B cells at the next level) when any of its leaves has
+a density_gradient_indicator (src/driver.jl:400-414) above `refine_above`, then
+2:1 balance is restored across block faces.
+
+The state never leaves the device during a run: the indicator is evaluated by
+a kernel that returns one maximum per block, the host turns those into tags
+and a new block list (the tree is a few thousand entries), the library builds
+the new mesh's tables, and a remap kernel injects each parent cell's conserved
+values into its children device to device, so Σ U·V is unchanged. Between
+regrids the leaves are advanced by the batched kernel (one persistent launch
+per RK stage).
 """
 from __future__ import annotations
 
 import numpy as np
+
+import time
 
 from .blockmesh import BlockMesh, BlockSimulation, REF
 
@@ -79,39 +85,18 @@ class QuadBlocks:
             out[:, y0:y0 + self.B * r, x0:x0 + self.B * r] = fine
         return out
 
-    # -- indicator / tagging
-    def tag(self, rho: np.ndarray, refine_above: float, symmetric: bool = False) -> set:
-        """Blocks below maxlevel with a cell whose density jump to a face
-        neighbour exceeds refine_above (jumps measured on the finest raster).
-        symmetric: also tag the mirror images of every tagged block (x, y and
-        diagonal mirrors of the square), so that round-off differences between
-        symmetric cells cannot make the mesh asymmetric."""
-        R = self.raster(rho[None])[0]
-        g = np.zeros_like(R)
-        for ax in (0, 1):
-            for sh in (1, -1):
-                Rn = np.roll(R, sh, axis=ax)
-                valid = np.ones_like(R, dtype=bool)     # no wrap across the domain edge
-                if ax == 0:
-                    if sh == 1:
-                        valid[0, :] = False
-                    else:
-                        valid[-1, :] = False
-                else:
-                    if sh == 1:
-                        valid[:, 0] = False
-                    else:
-                        valid[:, -1] = False
-                jump = np.abs(Rn - R) / np.maximum(np.maximum(R, Rn), np.finfo(float).eps)
-                g = np.maximum(g, np.where(valid, jump, 0.0))
-        tags = set()
-        for (l, ix, iy) in self.blocks:
-            if l >= self.maxlevel:
-                continue
-            r = 1 << (self.maxlevel - l)
-            y0, x0 = iy * self.B * r, ix * self.B * r
-            if g[y0:y0 + self.B * r, x0:x0 + self.B * r].max() > refine_above:
-                tags.add((l, ix, iy))
+    # -- tagging (regrid!, src/driver.jl:440-452)
+    def tags_from_indicator(self, block_max, refine_above: float, symmetric: bool = False) -> set:
+        """Blocks below maxlevel whose largest leaf indicator exceeds refine_above.
+        block_max: one value per block in the current order (the per-block
+        maximum of density_gradient_indicator). symmetric: also tag the mirror
+        images of every tagged block (x, y and diagonal mirrors of the square),
+        so that round-off differences between symmetric cells cannot make the
+        mesh asymmetric."""
+        block_max = np.asarray(block_max)
+        if block_max.shape != (len(self.blocks),):
+            raise ValueError("one indicator value per block")
+        tags = {b for b, g in zip(self.blocks, block_max) if b[0] < self.maxlevel and g > refine_above}
         if symmetric:
             leaves = set(self.blocks)
             for (l, ix, iy) in list(tags):
@@ -183,33 +168,19 @@ class QuadBlocks:
         self.blocks = sorted(leaves, key=lambda b: (b[0], b[2], b[1]))
         return nsplit
 
-    def remap(self, old_blocks: list, U_old: np.ndarray) -> np.ndarray:
-        """Conservative remap (injection) of the fields from old_blocks to the
-        current blocks: unchanged blocks are copied, a refined block's cells
-        inject their values into their 2×2 (or 4×4, ...) descendants."""
-        B2 = self.B * self.B
+    def source_blocks(self, old_blocks: list) -> np.ndarray:
+        """For every current block, the index in old_blocks of the same block or
+        of the ancestor it was split from: the table the remap injects by."""
         where = {b: q for q, b in enumerate(old_blocks)}
-        out = np.empty((U_old.shape[0], len(self.blocks) * B2), dtype=U_old.dtype)
-        for q, b in enumerate(self.blocks):
-            l, ix, iy = b
-            if b in where:
-                p = where[b]
-                out[:, q * B2:(q + 1) * B2] = U_old[:, p * B2:(p + 1) * B2]
-                continue
-            # find the old ancestor
-            for up in range(1, l + 1):
+        out = np.empty(len(self.blocks), dtype=np.int32)
+        for q, (l, ix, iy) in enumerate(self.blocks):
+            for up in range(0, l + 1):
                 a = (l - up, ix >> up, iy >> up)
                 if a in where:
+                    out[q] = where[a]
                     break
             else:
-                raise ValueError(f"block {b} has no ancestor in the old mesh")
-            p = where[a]
-            r = 1 << up
-            parent = U_old[:, p * B2:(p + 1) * B2].reshape(-1, self.B, self.B)
-            fine = np.repeat(np.repeat(parent, r, axis=1), r, axis=2)       # (nf, B·r, B·r)
-            oy = (iy - (a[2] << up)) * self.B
-            ox = (ix - (a[1] << up)) * self.B
-            out[:, q * B2:(q + 1) * B2] = fine[:, oy:oy + self.B, ox:ox + self.B].reshape(-1, B2)
+                raise ValueError(f"block {(l, ix, iy)} has no ancestor in the old mesh")
         return out
 
 
@@ -236,42 +207,72 @@ def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.0
               snapshots=(), max_steps: int = 100000, symmetric: bool = True):
     """Sedov blast on [-0.5, 0.5]² with base (nroot·B)² cells and up to maxlevel
     levels of refinement, reflecting walls. Regrid before the first step and
-    every `every` steps. Returns a dict with the final blocks/state, step count,
-    leaf counts and the raster density at the requested snapshot times."""
+    every `every` steps (evolve! with a RefinementPolicy, src/driver.jl:483-508);
+    the state stays on the device from the first upload to the final download.
+    Returns a dict with the final blocks/state, step count, leaf counts, the
+    raster density at the requested snapshot times and the wall time spent in
+    the regrid phases (ms): indicator kernel + per-block maxima to the host,
+    host tagging and refinement, the new mesh's context (tables), and the remap."""
     qb = QuadBlocks(nroot, B, maxlevel)
-    U = sedov_quadblocks_state(qb, dtype=np.float64).astype(dtype)
-    vol0 = qb.volumes()
-    totals0 = (U.astype(np.float64) * vol0).sum(axis=1)
+    U0 = sedov_quadblocks_state(qb, dtype=np.float64).astype(dtype)
+    totals0 = (U0.astype(np.float64) * qb.volumes()).sum(axis=1)
+    sim = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
+    sim.set_mode(mode)
+    sim.upload(U0)
     t, steps = 0.0, 0
     snaps = {}
     pending = sorted(snapshots)
     leaves_hist = []
-    while t < tfinal * (1 - 1e-12) and steps < max_steps:
-        old = list(qb.blocks)
-        tags = qb.tag(U[0].astype(np.float64), refine_above, symmetric=symmetric)
-        if tags:
-            qb.refine(tags)
-            U = qb.remap(old, U)
-        sim = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
-        sim.set_mode(mode)
-        sim.upload(U)
-        sim.t = t
-        stop = tfinal
-        if pending and pending[0] < tfinal:
-            stop = pending[0]
-        tt, k = sim.evolve(tfinal=stop, max_steps=every)
-        t = tt
-        steps += k
+    phase = {"indicator": 0.0, "refine": 0.0, "tables": 0.0, "remap": 0.0}
+    nregrid = ncheck = 0
+    try:
+        while t < tfinal * (1 - 1e-12) and steps < max_steps:
+            t0 = time.perf_counter()
+            bmax = sim.indicator()
+            t1 = time.perf_counter()
+            tags = qb.tags_from_indicator(bmax, refine_above, symmetric=symmetric)
+            phase["indicator"] += t1 - t0
+            ncheck += 1
+            if tags:
+                old = list(qb.blocks)
+                qb.refine(tags)
+                src = qb.source_blocks(old)
+                t2 = time.perf_counter()
+                new = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
+                new.set_mode(mode)
+                t3 = time.perf_counter()
+                try:
+                    new.remap_from(sim, src)
+                except Exception:
+                    new.close()
+                    raise
+                t4 = time.perf_counter()
+                sim.close()
+                sim = new
+                phase["refine"] += t2 - t1
+                phase["tables"] += t3 - t2
+                phase["remap"] += t4 - t3
+                nregrid += 1
+            sim.t = t
+            stop = tfinal
+            if pending and pending[0] < tfinal:
+                stop = pending[0]
+            tt, k = sim.evolve(tfinal=stop, max_steps=every)
+            t = tt
+            steps += k
+            leaves_hist.append(len(qb.blocks) * B * B)
+            while pending and t >= pending[0] * (1 - 1e-12):
+                rho = sim.state()[:1].astype(np.float64)
+                snaps[pending.pop(0)] = (t, qb.raster(rho)[0])
         U = sim.state()
+    finally:
         sim.close()
-        leaves_hist.append(len(qb.blocks) * B * B)
-        while pending and t >= pending[0] * (1 - 1e-12):
-            snaps[pending.pop(0)] = (t, qb.raster(U[:1].astype(np.float64))[0])
     vol = qb.volumes()
     totals = (U.astype(np.float64) * vol).sum(axis=1)
     return {"blocks": list(qb.blocks), "qb": qb, "U": U, "t": t, "steps": steps, "leaves": len(qb.blocks) * B * B,
             "max_leaves": max(leaves_hist) if leaves_hist else 0, "totals0": totals0, "totals": totals,
-            "snapshots": snaps}
+            "snapshots": snaps, "indicator_calls": ncheck, "regrids": nregrid,
+            "regrid_ms": {k: 1e3 * v for k, v in phase.items()}}
 
 
 def shock_radius(rho_raster: np.ndarray, x0: float = -0.5, L: float = 1.0) -> float:

@@ -378,6 +378,31 @@ class BlockSimulation:
     def sync(self):
         self._check(self._lib.vh_blocks_sync(self._h))
 
+    def indicator(self, cells: bool = False):
+        """density_gradient_indicator (src/driver.jl:400-414) of every leaf, on the
+        device. Returns the per-block maximum (nblocks,), and with cells=True also
+        the per-leaf values (ncell,)."""
+        C = self._C
+        dp = C.POINTER(C.c_double)
+        bmax = np.empty(self.mesh.nblocks)
+        cell = np.empty(self.ncell) if cells else None
+        self._check(self._lib.vh_blocks_indicator(self._h, bmax.ctypes.data_as(dp),
+                                                  cell.ctypes.data_as(dp) if cells else None))
+        return (bmax, cell) if cells else bmax
+
+    def remap_from(self, src: "BlockSimulation", src_block):
+        """Replace this simulation's state by `src`'s, transferred device to device
+        onto this mesh: block b takes the cells of src block src_block[b] (the same
+        block or its ancestor) by injection (prolong!, lib/RefMesh/src/RefMesh.jl:284-302).
+        The clock is taken over from src."""
+        C = self._C
+        sb = np.ascontiguousarray(src_block, dtype=np.int32)
+        if sb.shape != (self.mesh.nblocks,):
+            raise ValueError(f"src_block must hold one source block per block ({self.mesh.nblocks})")
+        self._check(self._lib.vh_blocks_remap(self._h, src._h, sb.ctypes.data_as(C.POINTER(C.c_int32))))
+        self.t = src.t
+        self.step_count = src.step_count
+
     def time_steps(self, nsteps: int, method: str = "graph", stage2: bool = True) -> dict:
         """Time nsteps single-rate steps: method 'persistent' (eager), 'per_block'
         (one launch per block per stage) or 'graph' (persistent, HIP graph)."""

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -89,6 +90,11 @@ struct Topo {
   int neighbor(long long g, int d, int side, long long* rep, long long* fine, int* nfine, int* nbblock) const {
     long long ijk[3];
     const int b = locate(g, ijk);
+    return neighbor_at(b, ijk, d, side, rep, fine, nfine, nbblock);
+  }
+  // The same for cell ijk of block b (no search for the block).
+  int neighbor_at(int b, const long long ijk[3], int d, int side, long long* rep, long long* fine, int* nfine,
+                  int* nbblock) const {
     const int* n = &dims[3 * b];
     const long long j = ijk[d] + (side ? 1 : -1);
     if (j >= 0 && j < n[d]) {
@@ -217,6 +223,14 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   // ---- blocks, representative maps, link faces ----
   H.blocks.assign(nb, BlockDev{});
   std::unordered_map<long long, int> facemap;   // emitting cell * 6 + slot -> face
+  {
+    long long cells_on_linked_planes = 0;
+    for (int b = 0; b < nb; ++b)
+      for (int d = 0; d < rank; ++d)
+        for (int side = 0; side < 2; ++side)
+          if (!T.cand[6 * b + 2 * d + side].empty()) cells_on_linked_planes += plane_size(&T.dims[3 * b], d);
+    facemap.reserve((size_t)cells_on_linked_planes + 16);
+  }
   long long fine[4];
   int nfine = 0;
   for (int b = 0; b < nb; ++b) {
@@ -248,7 +262,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             ijk[e2] = v;
             const long long g = T.index(b, ijk);
             long long rep = -1;
-            const int kind = T.neighbor(g, d, side, &rep, fine, &nfine, nullptr);
+            const int kind = T.neighbor_at(b, ijk, d, side, &rep, fine, &nfine, nullptr);
             if (kind == -1) return fail(-1, "block %s%lld touches a block more than one level apart", "", b);
             if (kind < 0)
               return fail(-1, "block %s%lld: a coarse face is not covered by one finer block (fine blocks must be "
@@ -277,7 +291,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
       int* nbr = which ? f.nR : f.nL;
       for (int s = 0; s < 2; ++s) {
         long long rep = -1;
-        const int kind = T.neighbor(c, d, s, &rep, nullptr, nullptr, nullptr);
+        const int kind = T.neighbor_at(b, ijk, d, s, &rep, nullptr, nullptr, nullptr);
         if (kind < 0) return fail(-1, "invalid neighbour across a link face%s", "");
         nbr[s] = kind == NB_BOUNDARY ? -1 - T.bc[6 * b + 2 * d + s] : (int)rep;
       }
@@ -285,21 +299,26 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   }
 
   // ---- link-cell records ----
+  // A slot lists at most 2^(rank-1) <= 4 faces: [0] count, [1..4] faces.
+  using SlotList = std::array<int, 5>;
   std::unordered_map<long long, int> recof;   // cell -> record
+  recof.reserve(2 * H.faces.size() + 16);
   std::vector<int> recblock;
   std::vector<int> recgrpblock;               // record*6 + slot -> emitting block of a type-2 slot
   std::vector<LinkCellDev> recs;
-  std::vector<std::vector<int>> slotlists;    // record*6 + slot
+  std::vector<SlotList> slotlists;            // record*6 + slot
   auto record_for = [&](long long g, int b) -> int {
     auto it = recof.find(g);
     if (it != recof.end()) return it->second;
     const int r = (int)recs.size();
-    recof[g] = r;
+    recof.emplace(g, r);
     LinkCellDev R{};
     recs.push_back(R);
     recblock.push_back(b);
-    slotlists.resize(6 * (r + 1));
-    recgrpblock.resize(6 * (r + 1), -1);
+    for (int q = 0; q < 6; ++q) {
+      slotlists.push_back(SlotList{0, -1, -1, -1, -1});
+      recgrpblock.push_back(-1);
+    }
     return r;
   };
   for (int b = 0; b < nb; ++b) {
@@ -318,35 +337,36 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             const long long g = T.index(b, ijk);
             long long rep = -1;
             int nbb = -1;
-            const int kind = T.neighbor(g, d, side, &rep, fine, &nfine, &nbb);
+            const int kind = T.neighbor_at(b, ijk, d, side, &rep, fine, &nfine, &nbb);
             if (kind == NB_BOUNDARY) continue;
             int type = 0, emitter_block = -1;
-            std::vector<int> fl;
+            SlotList fl{0, -1, -1, -1, -1};
             auto face_of = [&](long long cell, int sl) -> int {
               auto it = facemap.find(cell * 6 + sl);
               return it == facemap.end() ? -1 : it->second;
             };
             if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
               type = 1;
-              fl.push_back(face_of(g, slot));
+              fl[++fl[0]] = face_of(g, slot);
             } else if (side == 0 && kind == NB_SAME) {
               type = 2;
               emitter_block = nbb;
-              fl.push_back(face_of(rep, 2 * d + 1));
+              fl[++fl[0]] = face_of(rep, 2 * d + 1);
             } else if (kind == NB_FINER) {
               type = 2;
               emitter_block = nbb;
-              for (int i = 0; i < nfine; ++i) fl.push_back(face_of(fine[i], side ? 2 * d : 2 * d + 1));
+              for (int i = 0; i < nfine; ++i) fl[++fl[0]] = face_of(fine[i], side ? 2 * d : 2 * d + 1);
             }
-            for (int f : fl)
-              if (f < 0) return fail(-1, "block %s%lld: a link face has no emitting cell (mesh not 2:1 aligned)", "", b);
+            for (int i = 1; i <= fl[0]; ++i)
+              if (fl[i] < 0)
+                return fail(-1, "block %s%lld: a link face has no emitting cell (mesh not 2:1 aligned)", "", b);
             const int r = record_for(g, b);
             recs[r].type[slot] = type;
             slotlists[6 * r + slot] = fl;
             recgrpblock[6 * r + slot] = emitter_block;
             if (kind == NB_FINER) recs[r].finer |= 1 << slot;
-            for (int f : fl) {   // each listed face must touch this cell
-              const LinkFaceDev& F = H.faces[f];
+            for (int i = 1; i <= fl[0]; ++i) {   // each listed face must touch this cell
+              const LinkFaceDev& F = H.faces[fl[i]];
               if (F.iL != g && F.iR != g) return fail(-1, "link face bookkeeping error at block %s%lld", "", b);
             }
           }
@@ -357,9 +377,10 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
     LinkCellDev R = recs[r];
     std::vector<std::pair<int, int>> grp;   // (emitting block, slot)
     for (int s = 0; s < 6; ++s) {
+      const SlotList& sl = slotlists[6 * r + s];
       R.start[s] = (int)H.slot_faces.size();
-      R.count[s] = (int)slotlists[6 * r + s].size();
-      for (int f : slotlists[6 * r + s]) H.slot_faces.push_back(f);
+      R.count[s] = sl[0];
+      for (int i = 1; i <= sl[0]; ++i) H.slot_faces.push_back(sl[i]);
       if (R.type[s] == 2) grp.push_back({recgrpblock[6 * r + s], s});
     }
     std::sort(grp.begin(), grp.end());
@@ -688,6 +709,7 @@ struct vh_blocks {
   void* d_flux = nullptr;
   int* d_cfl_blocks = nullptr;
   long long* d_cfl_prefix = nullptr;
+  double* d_ind = nullptr;   // per-block refinement indicator maximum
   std::vector<DevReg> regs;
   // state
   void* U = nullptr;
@@ -954,6 +976,7 @@ static void free_tables(vh_blocks* m) {
   (void)hipFree(m->d_flux);
   (void)hipFree(m->d_cfl_blocks);
   (void)hipFree(m->d_cfl_prefix);
+  (void)hipFree(m->d_ind);
   for (DevReg& r : m->regs) {
     (void)hipFree(r.cells);
     (void)hipFree(r.lcs);
@@ -968,6 +991,7 @@ static void free_tables(vh_blocks* m) {
   m->d_faces = nullptr;
   m->d_flux = nullptr;
   m->d_cfl_prefix = nullptr;
+  m->d_ind = nullptr;
 }
 
 static int upload_tables(vh_blocks* m) {
@@ -983,6 +1007,7 @@ static int upload_tables(vh_blocks* m) {
   if ((rc = upload_vec(&m->d_cfl_blocks, H.cfl_blocks))) return rc;
   if ((rc = upload_vec(&m->d_cfl_prefix, H.cfl_prefix))) return rc;
   VH_HIP(hipMalloc(&m->d_flux, m->esize * 5 * std::max<size_t>(1, H.faces.size())));
+  VH_HIP(hipMalloc(&m->d_ind, sizeof(double) * m->T.nb));
   m->regs.assign(H.regs.size(), DevReg{});
   for (size_t l = 0; l < H.regs.size(); ++l) {
     DevReg& R = m->regs[l];
@@ -1249,6 +1274,72 @@ int vh_blocks_info(const vh_blocks* m, int64_t* counts6, int32_t* zchunk, int32_
 int vh_blocks_sync(vh_blocks* m) {
   if (!m) return fail(-1, "null argument%s", "");
   VH_HIP(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int vh_blocks_indicator(vh_blocks* m, double* block_max, double* cell_g) {
+  if (!m || !block_max) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(m->device));
+  const long long n = m->T.off[m->T.nb];
+  double* dcell = nullptr;
+  if (cell_g) VH_HIP(hipMalloc(&dcell, sizeof(double) * n));
+  regrid_indicator(m->stream, m->dtype, m->T.nb, m->d_blocks, m->d_repmap, m->T.rank, m->U, dcell, m->d_ind);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(block_max, m->d_ind, sizeof(double) * m->T.nb, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess && cell_g)
+    e = hipMemcpyAsync(cell_g, dcell, sizeof(double) * n, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (dcell) (void)hipFree(dcell);
+  if (e != hipSuccess) return fail(-2, "HIP error %s in the refinement indicator%s", hipGetErrorString(e));
+  return 0;
+}
+
+int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_block) {
+  if (!dst || !src || !src_block) return fail(-1, "null argument%s", "");
+  if (dst == src) return fail(-1, "remap needs two different contexts%s", "");
+  const Topo& D = dst->T;
+  const Topo& S = src->T;
+  if (dst->dtype != src->dtype) return fail(-1, "remap between contexts of different dtypes%s", "");
+  if (dst->device != src->device) return fail(-1, "remap between contexts on different devices%s", "");
+  if (D.rank != S.rank || D.maxlevel != S.maxlevel) return fail(-1, "remap between meshes of different rank or maxlevel%s", "");
+  for (int d = 0; d < D.rank; ++d)
+    if (D.h0[d] != S.h0[d]) return fail(-1, "remap between meshes of different base cell widths%s (axis %lld)", "", d);
+  std::vector<RemapDev> map(D.nb);
+  for (int b = 0; b < D.nb; ++b) {
+    const int p = src_block[b];
+    if (p < 0 || p >= S.nb) return fail(-1, "block %s%lld: source block out of range", "", b);
+    const int shift = D.level[b] - S.level[p];
+    if (shift < 0)
+      return fail(-1, "block %s%lld is coarser than its source block (the remap prolongs by injection only)", "", b);
+    map[b].src = p;
+    map[b].shift = shift;
+    for (int d = 0; d < 3; ++d) {
+      map[b].o[d] = 0;
+      if (d >= D.rank) continue;
+      const long long ss = S.span(p);
+      const long long rel = D.origin[3 * b + d] - S.origin[3 * p + d];
+      if (rel < 0 || rel % ss) return fail(-1, "block %s%lld does not start on a cell of its source block", "", b);
+      const long long o = rel / ss;
+      if (o + ((long long)(D.dims[3 * b + d] - 1) >> shift) >= S.dims[3 * p + d])
+        return fail(-1, "block %s%lld is not contained in its source block", "", b);
+      map[b].o[d] = (int)o;
+    }
+  }
+  VH_HIP(hipSetDevice(dst->device));
+  VH_HIP(hipStreamSynchronize(src->stream));
+  RemapDev* dmap = nullptr;
+  int rc = upload_vec(&dmap, map);
+  if (rc) {
+    (void)hipFree(dmap);
+    return rc;
+  }
+  regrid_remap(dst->stream, dst->dtype, D.nb, dst->d_blocks, src->d_blocks, dmap, src->U, S.off[S.nb], dst->U,
+               D.off[D.nb]);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
+  (void)hipFree(dmap);
+  if (e != hipSuccess) return fail(-2, "HIP error %s in the remap%s", hipGetErrorString(e));
   return 0;
 }
 

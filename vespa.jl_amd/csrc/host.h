@@ -77,3 +77,19 @@ int grav_solve(hipStream_t s, int dtype, const GridDev& G, const GravParams& P, 
                double* relres);
 
 }  // namespace vh
+
+namespace vh {
+
+// Block-mesh regrid (regrid.hip). RemapDev: per destination block, the source
+// block that contains it, the level difference and the position of its first
+// cell in the source block's cells.
+struct RemapDev {
+  int src, shift;
+  int o[3];
+};
+void regrid_indicator(hipStream_t s, int dtype, int nblocks, const BlockDev* blocks, const int* repmap, int rank,
+                      const void* rho, double* cell_g, double* block_max);
+void regrid_remap(hipStream_t s, int dtype, int ndst, const BlockDev* dst_blocks, const BlockDev* src_blocks,
+                  const RemapDev* map, const void* Us, long long ns, void* Ud, long long nd);
+
+}  // namespace vh
