@@ -214,6 +214,32 @@ function hip_evolve!(m::HIPBlocks, cfl::Real, tfinal::Real; t0::Real = 0.0, subc
     return t[], k[]
 end
 
+"""
+    hip_indicator(m::HIPBlocks, nblocks) -> Vector{Float64}
+
+density_gradient_indicator (src/driver.jl:400-414) of every leaf, evaluated on the
+device; returns its maximum per block, all `regrid!`'s tagging needs.
+"""
+function hip_indicator(m::HIPBlocks, nblocks::Integer)
+    bmax = Vector{Float64}(undef, nblocks)
+    _check(ccall((:vh_blocks_indicator, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                 m.ptr, bmax, C_NULL))
+    return bmax
+end
+
+"""
+    hip_remap!(dst::HIPBlocks, src::HIPBlocks, src_block)
+
+Conservative transfer of `src`'s state onto `dst`'s refined mesh on the device:
+block b of dst takes the cells of block src_block[b] (1-based: the same block or
+its ancestor) by injection (lib/RefMesh/src/RefMesh.jl:284-302).
+"""
+function hip_remap!(dst::HIPBlocks, src::HIPBlocks, src_block::AbstractVector{<:Integer})
+    sb = Int32.(src_block) .- Int32(1)
+    _check(ccall((:vh_blocks_remap, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}), dst.ptr, src.ptr, sb))
+    return dst
+end
+
 # ------------------------------------------------------------------ reverse mode
 
 """
@@ -288,6 +314,7 @@ end
 #   end
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
-       HIPBlocks, hip_download, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!, hip_phi
+       HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
+       hip_phi
 
 end # module
