@@ -520,3 +520,60 @@ def compute_dt_grav(g: Grid, U: np.ndarray, cfl: float, G: float, floor64=False)
     U = np.ascontiguousarray(U)
     return float(getattr(lib(), f"vo_compute_dt_grav_{_suf(U.dtype)}")(C.byref(g), _ptr(U), float(cfl),
                                                                        int(floor64), float(G)))
+
+
+# ---------------- PPMKernels unsplit RK2 MUSCL (muscl_oracle.c) ----------------
+
+def _muscl_typed():
+    L = lib()
+    if not getattr(L, "_mu_typed", False):
+        vp = C.c_void_p
+        i64p = C.POINTER(C.c_int64)
+        for suf in ("f64", "f32"):
+            getattr(L, f"vo_muscl_flux_line_{suf}").argtypes = [vp] * 5 + [C.c_int64] * 3 + [C.c_double] * 3 + [vp] * 5
+            getattr(L, f"vo_muscl_step_3d_{suf}").argtypes = [vp, i64p, C.c_int64] + [C.c_double] * 5 + [vp]
+            getattr(L, f"vo_muscl_fill_periodic_{suf}").argtypes = [vp, i64p, C.c_int64]
+            getattr(L, f"vo_muscl_max_wavespeed_{suf}").argtypes = [vp, C.c_int64, C.c_double]
+            getattr(L, f"vo_muscl_max_wavespeed_{suf}").restype = C.c_double
+        L._mu_typed = True
+    return L
+
+
+def muscl_flux_line(rho, eint, vx, vy, vz, ncells, nghost, jdim=1, gamma=1.4, theta=1.5, small_rho=1e-10):
+    """muscl_flux_line! (lib/PPMKernels/src/muscl.jl:223-239) → (fd, fs1, fs2, fs3, fe),
+    each of (ncells − 2·nghost + 1)·jdim interfaces."""
+    L = _muscl_typed()
+    arrs = [np.ascontiguousarray(a) for a in (rho, eint, vx, vy, vz)]
+    dt = arrs[0].dtype
+    nfi = ncells - 2 * nghost + 1
+    out = [np.empty(nfi * jdim, dtype=dt) for _ in range(5)]
+    getattr(L, f"vo_muscl_flux_line_{_suf(dt)}")(*[a.ctypes.data for a in arrs], int(ncells), int(nghost), int(jdim),
+                                                  float(gamma), float(theta), float(small_rho),
+                                                  *[o.ctypes.data for o in out])
+    return tuple(out)
+
+
+def muscl_step_3d(U, dims, ng, dt, gamma=1.4, theta=1.5, dx=1.0, small_rho=1e-10, record=False):
+    """muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:314-370) on U (5, N) in place;
+    with record=True returns the recorded fluxes (3, 5, N) (½F per stage)."""
+    L = _muscl_typed()
+    assert U.flags.c_contiguous and U.shape[0] == 5
+    d = (C.c_int64 * 3)(*dims)
+    frec = np.empty((3, 5, U.shape[1]), dtype=U.dtype) if record else None
+    getattr(L, f"vo_muscl_step_3d_{_suf(U.dtype)}")(U.ctypes.data, d, int(ng), float(dt), float(gamma), float(theta),
+                                                     float(dx), float(small_rho), frec.ctypes.data if record else None)
+    return frec
+
+
+def muscl_fill_periodic(f, dims, ng):
+    """fill_periodic! of one field (lib/PPMKernels/src/muscl_grid.jl:644-670), in place."""
+    L = _muscl_typed()
+    assert f.flags.c_contiguous
+    getattr(L, f"vo_muscl_fill_periodic_{_suf(f.dtype)}")(f.ctypes.data, (C.c_int64 * 3)(*dims), int(ng))
+
+
+def muscl_max_wavespeed(U, gamma):
+    """max_wavespeed (lib/PPMKernels/src/muscl_grid.jl:672-695)."""
+    L = _muscl_typed()
+    U = np.ascontiguousarray(U)
+    return getattr(L, f"vo_muscl_max_wavespeed_{_suf(U.dtype)}")(U.ctypes.data, U.shape[1], float(gamma))

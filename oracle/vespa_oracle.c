@@ -256,3 +256,4 @@ int vo_abi_version(void) { return 1; }
 /* Block-structured AMR restatement (shares the scalar kernels above). */
 #include "blocks_oracle.c"
 #include "adjoint_oracle.c"
+#include "muscl_oracle.c"

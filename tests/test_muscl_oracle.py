@@ -1,0 +1,285 @@
+"""Pins of the MUSCL oracle (CPU): PPMKernels' unsplit SSP-RK2 MUSCL step
+(lib/PPMKernels/src/muscl.jl, lib/PPMKernels/src/muscl_grid.jl).
+
+The reference certifies its flux line against the live Enzo solver (EnzoLib,
+absent here). The restatement is pinned instead by:
+  * known answers of the flux line: linear data are reconstructed exactly and
+    the HLL flux of equal states is the Euler flux (closed form); a local
+    extremum gets a zero slope; the θ-limited slope picks θ·(smaller jump);
+    supersonic flow takes the upwind flux;
+  * the reference's own driver checks (test/test_muscl_grid.jl): a 3D step on a
+    state varying along one axis equals a 1D RK2 built from the flux line
+    (x, y and z, exercising the cyclic momentum roles), conservation to
+    round-off, and recorded fluxes that reproduce the update;
+  * Sod against the exact Riemann solution, with first-order L1 convergence;
+  * fill_periodic! against numpy's wrap padding.
+"""
+import numpy as np
+import pytest
+
+G14 = 1.4
+
+
+def euler_flux(r, e, u, v, w, g=G14):
+    """Closed-form Euler flux along the normal u (ρ, eint, u, v, w)."""
+    p = (g - 1) * r * e
+    E = r * (e + 0.5 * (u * u + v * v + w * w))
+    return np.array([r * u, r * u * u + p, r * u * v, r * u * w, (E + p) * u])
+
+
+def test_linear_data_reconstructed_exactly_and_hll_of_equal_states(O):
+    """Linear profiles: θ ≥ 1 keeps the full slope, so L = R = the face value
+    and HLL returns the Euler flux of that state."""
+    n, ng = 16, 3
+    x = np.arange(n, dtype=float)
+    r, e, u, v, w = 1.0 + 0.01 * x, 2.0 + 0.03 * x, 0.1 + 0.002 * x, -0.05 + 0.001 * x, 0.02 - 0.003 * x
+    F = np.array(O.muscl_flux_line(r, e, u, v, w, n, ng))
+    xf = np.arange(ng, n - ng + 1) - 0.5                   # face positions
+    lin = lambda a0, a1: a0 + a1 * xf
+    ref = euler_flux(lin(1.0, 0.01), lin(2.0, 0.03), lin(0.1, 0.002), lin(-0.05, 0.001), lin(0.02, -0.003))
+    np.testing.assert_allclose(F, ref, rtol=2e-15, atol=1e-17)
+
+
+def _supersonic(rho, e=1.0, u=5.0):
+    n = rho.size
+    return rho, np.full(n, e), np.full(n, u), np.zeros(n), np.zeros(n)
+
+
+def test_extremum_and_theta_limited_slope_upwind(O):
+    """Supersonic to the right (am = 0): the flux is the Euler flux of the L state,
+    the right-face value of the upwind cell. At a local maximum the slope is 0;
+    with jumps 1 and 3 the slope is min(θ·1, θ·3, 2) = 1.5 (θ = 1.5), or 1 (θ = 1)."""
+    n, ng = 12, 3
+    rho = np.full(n, 10.0)
+    rho[5:] = [11, 14, 20, 20, 20, 20, 20]                  # cell 5: jumps 1 (left) and 3 (right)
+    rho[8] = 25.0                                           # cell 8: local maximum
+    for th, s5 in ((1.5, 1.5), (1.0, 1.0)):
+        fd = O.muscl_flux_line(*_supersonic(rho), n, ng, theta=th)[0]
+        # face index f (0-based) sits between cells ng-1+f and ng+f; L = right face of ng-1+f
+        np.testing.assert_allclose(fd[5 - ng + 1], (11 + 0.5 * s5) * 5.0, rtol=2e-16)
+        np.testing.assert_allclose(fd[8 - ng + 1], 25.0 * 5.0, rtol=2e-16)
+    # leftward supersonic: the R state (left face of the downwind cell) carries the flux
+    r, e, u, v, w = _supersonic(rho, u=-5.0)
+    fd = O.muscl_flux_line(r, e, u, v, w, n, ng)[0]
+    np.testing.assert_allclose(fd[5 - ng], (11 - 0.5 * 1.5) * -5.0, rtol=2e-16)
+
+
+def test_enzo_sign_of_zero_and_density_floor(O):
+    """sign(0) = −1 makes the slope of an all-zero difference −0, so a −0 velocity
+    keeps its sign through reconstruction and the transverse momentum flux is −0;
+    reconstructed densities are floored at small_rho."""
+    n, ng = 10, 3
+    r, e, u, v, w = _supersonic(np.linspace(1.0, 2.0, n))
+    w = np.full(n, -0.0)
+    fs3 = O.muscl_flux_line(r, e, u, v, w, n, ng)[3]
+    assert np.all(fs3 == 0) and np.all(np.signbit(fs3))
+    fd = O.muscl_flux_line(*_supersonic(np.full(n, 1e-12)), n, ng, small_rho=1e-10)[0]
+    np.testing.assert_allclose(fd, 1e-10 * 5.0, rtol=2e-16)
+
+
+def _rk2_line_ref(O, D, Sn, St1, St2, Tau, na, ng, dt, dx, gamma, theta, grid_order=None):
+    """1D SSP-RK2 from the flux line (test/test_muscl_grid.jl:20-44). grid_order:
+    the grid axis of each velocity slot (normal, t1, t2), to sum |v|² in grid
+    order as the 3D driver's cons2prim does (then the result is bit-exact)."""
+    active = na - 2 * ng
+    dtdx = dt / dx
+
+    def Lop(D, Sn, St1, St2, Tau):
+        vx, vy, vz = Sn / D, St1 / D, St2 / D
+        if grid_order is None:
+            eint = Tau / D - 0.5 * (vx ** 2 + vy ** 2 + vz ** 2)
+        else:
+            g = [None] * 3
+            for q, a in enumerate(grid_order):
+                g[a] = (vx, vy, vz)[q]
+            eint = Tau / D - 0.5 * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
+        F = O.muscl_flux_line(D.copy(), eint, vx, vy, vz, na, ng, gamma=gamma, theta=theta)
+        d = [np.zeros(na) for _ in range(5)]
+        for m in range(active):
+            c = ng + m
+            for f in range(5):
+                d[f][c] = -(F[f][m + 1] - F[f][m]) * dtdx
+        return d
+
+    U0 = [a.copy() for a in (D, Sn, St1, St2, Tau)]
+    d1 = Lop(*U0)
+    U1 = [U0[i] + d1[i] for i in range(5)]
+    d2 = Lop(*U1)
+    return [0.5 * (U0[i] + U1[i] + d2[i]) for i in range(5)]
+
+
+@pytest.mark.parametrize("varying_transverse", [False, True])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_3d_step_reduces_to_1d_rk2(O, axis, varying_transverse):
+    """test/test_muscl_grid.jl:46-115, same profiles, tolerance of its layer A;
+    then a cold, fast variant whose transverse velocities vary along the line
+    (still uniform across it): there eint is small beside ½|v|², so the order in
+    which |v|² is summed shows in the result and pins the (normal, t1, t2) roles."""
+    ng, gamma, theta, dt, dx = 3, 1.4, 1.5, 0.02, 1.0
+    na = 22
+    odims = [(na, 8, 7), (7, na, 8), (8, 7, na)][axis]
+    m = np.arange(1, na + 1)
+    t = np.tanh((m - 11.0) * 0.8)
+    rho1, pr1, vn1 = 0.6 - 0.35 * t, 0.6 - 0.45 * t, 0.15 - 0.10 * t
+    vt1, vt2 = np.full(na, 0.05), np.full(na, -0.03)
+    if varying_transverse:
+        pr1 = 1e-3 * pr1
+        vn1 = 4.0 * vn1
+        vt1 = 0.3 + 0.13 * np.sin(m)
+        vt2 = -0.2 + 0.07 * np.cos(1.7 * m)
+    eint1 = pr1 / ((gamma - 1) * rho1)
+    etot1 = eint1 + 0.5 * (vn1 ** 2 + vt1 ** 2 + vt2 ** 2)
+    line = [rho1, rho1 * vn1, rho1 * vt1, rho1 * vt2, rho1 * etot1]
+    ref = _rk2_line_ref(O, *line, na=na, ng=ng, dt=dt, dx=dx, gamma=gamma, theta=theta)
+    nx, ny, nz = odims
+    shape = (nz, ny, nx)
+    roles = [(1, 2, 3), (2, 3, 1), (3, 1, 2)][axis]        # grid slots of (normal, t1, t2)
+    U = np.zeros((5, nx * ny * nz))
+    bshape = [1, 1, 1]
+    bshape[2 - axis] = na
+
+    def bcast(a):
+        return np.broadcast_to(a.reshape(bshape), shape).ravel()
+    U[0] = bcast(line[0])
+    U[4] = bcast(line[4])
+    for q, slot in enumerate(roles):
+        U[slot] = bcast(line[1 + q])
+    O.muscl_step_3d(U, odims, ng, dt, gamma, theta, dx)
+    got = U.reshape((5,) + shape)
+    sel = [4, 4, 4]
+    sel[2 - axis] = slice(None)
+    exact = _rk2_line_ref(O, *line, na=na, ng=ng, dt=dt, dx=dx, gamma=gamma, theta=theta,
+                          grid_order=[axis, (axis + 1) % 3, (axis + 2) % 3])
+    for name, f in (("D", 0), ("Sn", roles[0]), ("St1", roles[1]), ("St2", roles[2]), ("Tau", 4)):
+        q = {"D": 0, "Sn": 1, "St1": 2, "St2": 3, "Tau": 4}[name]
+        g = got[f][tuple(sel)]
+        np.testing.assert_allclose(g, ref[q], rtol=1e-12, atol=1e-14, err_msg=name)
+        # with |v|² summed in grid order the transverse sweeps are exact no-ops:
+        # bit-exact, which also fixes the cyclic (normal, t1, t2) roles
+        assert np.array_equal(g, exact[q]), name
+
+
+def _blob(m, ng_center, scale, amp_r=0.4, amp_p=0.3):
+    c = (np.arange(1, m + 1) - ng_center) / scale
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    b = np.exp(-(X ** 2 + Y ** 2 + Z ** 2))
+    rho = 1.0 + amp_r * b
+    pr = 0.6 + amp_p * b
+    vx, vy, vz = 0.10 * X * b, 0.08 * Y * b, -0.06 * Z * b
+    etot = pr / ((G14 - 1) * rho) + 0.5 * (vx ** 2 + vy ** 2 + vz ** 2)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, rho * etot]).reshape(5, -1).copy()
+
+
+def _total(f, dims, ng, dx):
+    nx, ny, nz = dims
+    a = f.reshape(nz, ny, nx)[ng:nz - ng, ng:ny - ng, ng:nx - ng]
+    return a.sum() * dx ** 3
+
+
+def test_conservation(O):
+    """test/test_muscl_grid.jl:117-147: three steps of a contained blob."""
+    m, ng, dt, dx = 26, 3, 0.02, 1.0
+    dims = (m, m, m)
+    U = _blob(m, 13.5, 2.2)
+    t0 = [_total(U[f], dims, ng, dx) for f in range(5)]
+    for _ in range(3):
+        O.muscl_step_3d(U, dims, ng, dt, G14, 1.5, dx)
+    t1 = [_total(U[f], dims, ng, dx) for f in range(5)]
+    assert abs(t1[0] - t0[0]) / abs(t0[0]) < 1e-12
+    for f in (1, 2, 3):
+        assert abs(t1[f] - t0[f]) < 1e-11
+    assert abs(t1[4] - t0[4]) / abs(t0[4]) < 1e-12
+
+
+def _rec_state(n=16, ng=3):
+    nb = n + 2 * ng
+    c = (np.arange(1, nb + 1) - ng - 0.5) / n
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    rho = 1.0 + 0.3 * np.sin(np.pi * 2 * X) * np.cos(np.pi * 2 * Y)
+    pr = 0.6 + 0.2 * np.cos(np.pi * 2 * Z)
+    vx, vy, vz = 0.2 * np.sin(np.pi * 2 * Y), 0.15 * np.cos(np.pi * 2 * Z), -0.1 * np.sin(np.pi * 2 * X)
+    etot = pr / ((G14 - 1) * rho) + 0.5 * (vx ** 2 + vy ** 2 + vz ** 2)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, rho * etot]).reshape(5, -1).copy(), (nb, nb, nb)
+
+
+def test_flux_recording_reproduces_the_update(O):
+    """test/test_muscl_grid.jl:663-696: ΔU = −Σ_a (frec[a][c+ê_a] − frec[a][c])·dt/dx
+    to round-off on the active cells, and recording leaves the step bit-identical."""
+    n, ng, dt = 16, 3, 0.008
+    dx = 1.0 / n
+    U, dims = _rec_state(n, ng)
+    U0 = U.copy()
+    frec = O.muscl_step_3d(U, dims, ng, dt, G14, 1.5, dx, record=True)
+    nx, ny, nz = dims
+    ea = (1, nx, nx * ny)
+    idx = np.arange(U.shape[1]).reshape(nz, ny, nx)[ng:nz - ng, ng:ny - ng, ng:nx - ng].ravel()
+    for f in range(5):
+        div = np.zeros(idx.size)
+        for a in range(3):
+            div += (frec[a, f, idx + ea[a]] - frec[a, f, idx]) * (dt / dx)
+        assert np.abs((U[f, idx] - U0[f, idx]) + div).max() < 1e-13
+    U2 = U0.copy()
+    O.muscl_step_3d(U2, dims, ng, dt, G14, 1.5, dx)
+    assert np.array_equal(U2, U)
+
+
+def _sod_l1(O, n):
+    ng = 3
+    dims = (n + 2 * ng, 2 * ng + 1, 2 * ng + 1)
+    dx = 1.0 / n
+    x = (np.arange(dims[0]) - ng + 0.5) * dx
+    left = x < 0.5
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    e = p / ((G14 - 1) * rho)
+    U = np.zeros((5,) + dims[::-1])
+    U[0] = rho
+    U[4] = rho * e
+    U = U.reshape(5, -1).copy()
+    t = 0.0
+    while t < 0.2 - 1e-12:
+        dt = min(0.4 * dx / O.muscl_max_wavespeed(U, G14), 0.2 - t)
+        O.muscl_step_3d(U, dims, ng, dt, G14, 1.5, dx)
+        t += dt
+    r = U[0].reshape(dims[::-1])[ng, ng, ng:-ng]
+    xs = x[ng:-ng]
+    ex = np.array([O.exact_riemann_sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), G14, (xi - 0.5) / 0.2)[0]
+                   for xi in xs])
+    return np.abs(r - ex).mean()
+
+
+def test_sod_against_exact_solution(O):
+    e200, e400 = _sod_l1(O, 200), _sod_l1(O, 400)
+    assert e200 < 1.2e-2
+    assert e400 / e200 < 0.75
+
+
+def test_internal_energy_floor(O):
+    """eint = τ/D − ½|v|² is floored at small_rho before reconstruction: a state
+    whose eint is negative everywhere still steps to finite values."""
+    n, ng = 12, 3
+    dims = (n, 2 * ng + 1, 2 * ng + 1)
+    x = np.arange(n)
+    rho = np.broadcast_to(1.0 + 0.5 * (x > 5), dims[::-1]).ravel()
+    U = np.zeros((5, rho.size))
+    U[0] = rho
+    U[1] = rho * 2.0
+    U[4] = rho * (0.5 * 4.0 - 1e-3)                     # eint = −1e-3
+    O.muscl_step_3d(U, dims, ng, 1e-3, G14, 1.5, 1.0)
+    assert np.isfinite(U).all()
+
+
+def test_fill_periodic_and_max_wavespeed(O):
+    rng = np.random.default_rng(0)
+    dims, ng = (9, 8, 10), 2
+    f = rng.normal(size=dims[::-1]).copy()
+    inner = f[ng:-ng, ng:-ng, ng:-ng].copy()
+    O.muscl_fill_periodic(f.reshape(-1), dims, ng)
+    assert np.array_equal(f, np.pad(inner, ng, mode="wrap"))
+    U = np.stack([rng.uniform(0.5, 2, 50), rng.normal(size=50), rng.normal(size=50), rng.normal(size=50),
+                  rng.uniform(-0.5, 3, 50)])
+    u, v, w = U[1] / U[0], U[2] / U[0], U[3] / U[0]
+    eint = U[4] / U[0] - 0.5 * (u * u + v * v + w * w)
+    ref = (np.sqrt(np.maximum(1.4 * 0.4 * eint, 0.0)) + np.maximum(np.abs(u), np.maximum(np.abs(v), np.abs(w)))).max()
+    assert (eint < 0).any()
+    assert O.muscl_max_wavespeed(U, 1.4) == pytest.approx(ref, rel=1e-15)
