@@ -283,6 +283,83 @@ def sedov_leg(V, args):
     return out
 
 
+def muscl_turbulence_state(n, ng=4, nmodes=24, seed=1234, mach=0.3, gamma=1.4):
+    """PPMKernels' turbulence benchmark state (lib/PPMKernels/bench/bench_turbulence.jl:31-57)
+    on an (n + 2·ng)³ grid: ρ = 1, P = 1/γ (c_s = 1), a solenoidal sum of random
+    Fourier modes |k_i| ≤ 3 with amplitude 1/|k| scaled to RMS Mach `mach`; our own
+    seeded generator (numpy), the same construction."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    N = n + 2 * ng
+    i = np.arange(N, dtype=np.float64)
+    v = [np.zeros((N, N, N)) for _ in range(3)]
+    for _ in range(nmodes):
+        k = rng.integers(-3, 4, 3)
+        if not k.any():
+            continue
+        k2 = float(k @ k)
+        a = rng.standard_normal(3)
+        a = a - (a @ k) / k2 * k
+        ph = 2 * np.pi * rng.random()
+        c = np.cos(2 * np.pi * (k[0] * i[None, None, :] + k[1] * i[None, :, None] + k[2] * i[:, None, None]) / N + ph)
+        c *= 1.0 / np.sqrt(k2)
+        for q in range(3):
+            v[q] += a[q] * c
+    vrms = np.sqrt(sum((x * x).mean() for x in v))
+    v = [x * (mach / vrms) for x in v]
+    rho = np.ones((N, N, N))
+    ge = (1.0 / gamma) / (gamma - 1.0)
+    etot = ge + 0.5 * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
+    U = np.stack([rho, rho * v[0], rho * v[1], rho * v[2], rho * etot]).reshape(5, -1)
+    return U, (N, N, N)
+
+
+def muscl_leg(V, args):
+    """PPMKernels unsplit RK2 MUSCL (PLM minmod-θ + HLL) on the reference's turbulence
+    benchmark: 256³ active cells + 4 ghost zones, γ = 1.4, θ = 1.5, dx = 1/256,
+    dt = 0.2·dx, Float32 and Float64 (fused stage kernels, exact arithmetic).
+    Mcell/s counts active cells per second, as the reference's table does; the
+    CPU baseline is the oracle (one thread) on a 32³ + ghost sample."""
+    import numpy as np
+    from vespa_jl_amd import muscl as M
+    n, ng = args.muscl_n, 4
+    U64, dims = muscl_turbulence_state(n, ng)
+    dx = 1.0 / n
+    dt = 0.2 * dx
+    ncell = int(np.prod(dims))
+    out = {"workload": f"{n}^3 active + {ng} ghost zones per side, decaying-turbulence state, Mach 0.3",
+           "published_reference": {"value_Mcell_s": 96.51, "hardware": "Apple Silicon, Metal, Float32",
+                                   "source": "lib/PPMKernels/bench/RESULTS_turbulence.md:24"}}
+    for dtype, key in ((np.float32, "f32"), (np.float64, "f64")):
+        g = M.MusclGrid(dims, ng, 1.4, 1.5, dx, eltype=dtype)
+        g.upload(U64.astype(dtype))
+        r = g.time_steps(dt, args.muscl_steps)
+        esz = np.dtype(dtype).itemsize
+        out[key] = {"Mcell_s": n ** 3 / (r["step_ms"] * 1e-3) / 1e6, "ms_per_step": r["step_ms"],
+                    "stage2_ms": r["stage2_ms"],
+                    "stage2_GBps": 15 * esz * ncell / (r["stage2_ms"] * 1e-3) / 1e9,
+                    "speedup_vs_published": n ** 3 / (r["step_ms"] * 1e-3) / 1e6 / 96.51,
+                    "finite": bool(np.isfinite(g.state()).all())}
+        g.close()
+        log(f"[bench] muscl {key}: {out[key]['Mcell_s']:.1f} Mcell/s ({r['step_ms']:.3f} ms/step, stage 2 "
+            f"{out[key]['stage2_GBps']:.0f} GB/s)")
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        m = 32
+        Uc, dc = muscl_turbulence_state(m, ng)
+        O.muscl_step_3d(Uc, dc, ng, 0.2 / m, 1.4, 1.5, 1.0 / m)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 5.0:
+            O.muscl_step_3d(Uc, dc, ng, 0.2 / m, 1.4, 1.5, 1.0 / m)
+            k += 1
+        el = (time.perf_counter() - t0) / k
+        out["cpu_baseline"] = {"value": m ** 3 / el / 1e6, "unit": "Mcell/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} Float64 steps of the oracle at {m}^3 + {ng} ghost zones"}
+        log(f"[bench] muscl CPU oracle {out['cpu_baseline']['value']:.2f} Mcell/s")
+    return out
+
+
 def gravity_leg(V, args):
     """Self-gravity on a 3D periodic box (n^3, gamma=5/3, G=1): a Jeans-unstable
     density ripple rho = 1 + 1e-3 (cos 2πx + cos 2πy + cos 2πz) plus 1e-4 random
@@ -376,6 +453,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint rollout leg")
     ap.add_argument("--no-sedov", action="store_true", help="skip the dynamic-AMR Sedov leg")
     ap.add_argument("--no-gravity", action="store_true", help="skip the self-gravity leg")
+    ap.add_argument("--no-muscl", action="store_true", help="skip the PPMKernels MUSCL leg")
+    ap.add_argument("--muscl-n", type=int, default=256)
+    ap.add_argument("--muscl-steps", type=int, default=20)
     ap.add_argument("--gravity-n", type=int, default=256)
     ap.add_argument("--gravity-steps", type=int, default=20)
     ap.add_argument("--adjoint-n", type=int, default=512)
@@ -528,6 +608,8 @@ def main():
         out["sedov_amr"] = sedov_leg(V, args)
     if not args.no_gravity:
         out["self_gravity"] = gravity_leg(V, args)
+    if not args.no_muscl:
+        out["muscl_rk2"] = muscl_leg(V, args)
 
     print(json.dumps(out), flush=True)
 

@@ -332,6 +332,45 @@ int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_blo
 int vh_blocks_time_steps(vh_blocks* m, double cfl, int64_t nsteps, int32_t method, double* ms_step,
                          double* ms_stage2);
 
+/*
+ * PPMKernels' unsplit SSP-RK2 MUSCL step (muscl_step_3d!, lib/PPMKernels/src/muscl_grid.jl:314-370:
+ * PLM minmod-θ reconstruction + HLL flux, lib/PPMKernels/src/muscl.jl:13-58,181-206) on a
+ * ghosted 3D grid: dims = cells per axis including ng ghost zones on each side (the
+ * active region is ng .. dims-ng-1 per axis), column-major (x fastest), state SoA
+ * U[f*ncell + cell] with f = D, S1, S2, S3, τ (τ = ρ·etot). Ghost zones are not
+ * refilled by a step (the reference's bc! = nothing); vh_muscl_fill_periodic is
+ * fill_periodic!. Arithmetic at the state's precision, bit-identical to the reference's
+ * operation order. No dual-energy field.
+ */
+typedef struct vh_muscl vh_muscl;
+
+/* gamma, theta (limiter, reference default 1.5), small_rho (density and eint floor,
+ * default 1e-10) and dx are fixed per context; ng >= 2, dims[d] >= 2*ng+1. */
+int vh_muscl_create(vh_muscl** out, const int32_t* dims, int32_t ng, double gamma, double theta,
+                    double small_rho, double dx, int32_t dtype);
+int vh_muscl_destroy(vh_muscl* m);
+int64_t vh_muscl_ncell(const vh_muscl* m);
+int vh_muscl_upload(vh_muscl* m, const void* U);
+int vh_muscl_download(vh_muscl* m, void* U);
+int vh_muscl_state_device_ptr(vh_muscl* m, void** ptr);
+/* Tiles: variant 0 = 16x16 cells, 1 = 32x8; zchunk = z planes per workgroup. */
+int vh_muscl_set_tiles(vh_muscl* m, int32_t variant, int32_t zchunk);
+/* One muscl_step_3d!(…; dt): two launches, stream-ordered. record != 0 is
+ * fluxrec: the recorded fluxes (zeroed, then ½F per stage at the lo face of each
+ * cell, lib/PPMKernels/src/muscl_grid.jl:278-291) are read with
+ * vh_muscl_fluxrec_download, 15·ncell values [axis][field][cell]. */
+int vh_muscl_step(vh_muscl* m, double dt, int32_t record);
+int vh_muscl_fluxrec_download(vh_muscl* m, void* frec);
+/* fill_periodic! (lib/PPMKernels/src/muscl_grid.jl:644-670) on the five fields. */
+int vh_muscl_fill_periodic(vh_muscl* m);
+/* max_wavespeed (lib/PPMKernels/src/muscl_grid.jl:672-695) over every cell. */
+int vh_muscl_max_wavespeed(vh_muscl* m, double* out);
+int vh_muscl_sync(vh_muscl* m);
+/* Time nsteps steps with HIP events after one untimed step; ms_stage2: mean of up
+ * to 20 stage-2 launches timed one by one. */
+int vh_muscl_time_steps(vh_muscl* m, double dt, int64_t nsteps, int32_t record, double* ms_step,
+                        double* ms_stage2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,141 @@
+"""GPU parity of the fused unsplit RK2 MUSCL step (PPMKernels muscl_step_3d!,
+lib/PPMKernels/src/muscl_grid.jl:314-370) against the oracle, through the C ABI:
+bit-identical states and recorded fluxes in Float64 and Float32 on grids whose
+sizes are not tile multiples, both tile shapes, ng 2..4; fill_periodic! and
+max_wavespeed; and the published benchmark workload (256³ active cells + 4 ghost
+zones, Float32) stepped once against the oracle."""
+import numpy as np
+import pytest
+
+from conftest import load_vespa
+
+V = load_vespa()
+from vespa_jl_amd import muscl as M  # noqa: E402
+from vespa_jl_amd._lib import VespaHIPError  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+G14 = 1.4
+
+
+def smooth_state(dims, seed, dtype, mach=0.5):
+    """Smooth seeded state: ρ ∈ [0.6, 1.4], p ∈ [0.5, 0.9], |v| ~ mach·c."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    Z, Y, X = np.meshgrid(np.arange(nz) / nz, np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+
+    def field(a):
+        k = rng.integers(1, 4, 3)
+        ph = rng.uniform(0, 2 * np.pi, 2)
+        return a * (np.sin(2 * np.pi * (k[0] * X + k[1] * Y) + ph[0]) * np.cos(2 * np.pi * k[2] * Z + ph[1]))
+    rho = 1.0 + field(0.4)
+    p = 0.7 + field(0.2)
+    vx, vy, vz = field(mach), field(mach), field(mach)
+    etot = p / ((G14 - 1) * rho) + 0.5 * (vx * vx + vy * vy + vz * vz)
+    return M.prim_to_cons(rho, vx, vy, vz, etot).reshape(5, -1).astype(dtype)
+
+
+CASES = [((22, 19, 13), 3), ((26, 26, 26), 3), ((9, 12, 40), 2), ((21, 21, 21), 4), ((40, 7, 9), 3)]
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dims,ng", CASES)
+def test_step_bitwise(O, dims, ng, dtype, variant):
+    print(f"muscl step {dims} ng={ng} {np.dtype(dtype).name} variant {variant}", flush=True)
+    U = smooth_state(dims, seed=sum(dims), dtype=dtype)
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10, eltype=dtype)
+    try:
+        g.set_tiles(variant, 8 if variant else 16)
+        g.upload(U)
+        for s in range(2):
+            rec = s == 1
+            frec_o = O.muscl_step_3d(U, dims, ng, 0.004, G14, 1.5, 0.05, 1e-10, record=rec)
+            g.step(0.004, fluxrec=rec)
+            np.testing.assert_array_equal(g.state(), U, err_msg=f"step {s}")
+        np.testing.assert_array_equal(g.fluxrec(), frec_o)
+    finally:
+        g.close()
+
+
+def test_reference_driver_cases_bitwise(O):
+    """The reference's conservation case (26³ blob, dt 0.02, dx 1, 3 steps) and a
+    cold supersonic line (eint floor, zero-sign paths)."""
+    from test_muscl_oracle import _blob
+    for dtype in (np.float64, np.float32):
+        U = _blob(26, 13.5, 2.2).astype(dtype)
+        g = M.MusclGrid((26, 26, 26), 3, G14, 1.5, 1.0, eltype=dtype)
+        g.upload(U)
+        for _ in range(3):
+            O.muscl_step_3d(U, (26, 26, 26), 3, 0.02, G14, 1.5, 1.0)
+            g.step(0.02)
+        np.testing.assert_array_equal(g.state(), U)
+        g.close()
+    n, ng = 16, 3
+    dims = (n, 2 * ng + 1, 2 * ng + 1)
+    x = np.arange(n)
+    rho = np.broadcast_to(1.0 + 0.5 * (x > 5), dims[::-1]).ravel()
+    U = np.zeros((5, rho.size))
+    U[0] = rho
+    U[1] = rho * 2.0
+    U[3] = -0.0
+    U[4] = rho * (0.5 * 4.0 - 1e-3)
+    g = M.MusclGrid(dims, ng, G14)
+    g.upload(U)
+    O.muscl_step_3d(U, dims, ng, 1e-3, G14)
+    g.step(1e-3)
+    got = g.state()
+    np.testing.assert_array_equal(got, U)
+    assert np.array_equal(np.signbit(got), np.signbit(U))
+    g.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fill_periodic_and_wavespeed(O, dtype):
+    dims, ng = (20, 17, 23), 3
+    U = smooth_state(dims, seed=4, dtype=dtype, mach=2.0)
+    U[4, ::7] *= -1.0                                      # negative eint in some cells
+    g = M.MusclGrid(dims, ng, G14, eltype=dtype)
+    try:
+        g.upload(U)
+        assert g.max_wavespeed() == O.muscl_max_wavespeed(U, G14)
+        g.fill_periodic()
+        for f in range(5):
+            O.muscl_fill_periodic(U[f], dims, ng)
+        np.testing.assert_array_equal(g.state(), U)
+    finally:
+        g.close()
+
+
+def test_array_form_and_errors(O):
+    dims, ng = (12, 11, 10), 2
+    U = smooth_state(dims, seed=9, dtype=np.float64)
+    fields = [U[f].copy() for f in range(5)]
+    fr = np.empty((3, 5, U.shape[1]))
+    M.muscl_step_3d(*fields, dims, ng, dt=0.01, gamma=G14, dx=0.1, fluxrec=fr)
+    fo = O.muscl_step_3d(U, dims, ng, 0.01, G14, 1.5, 0.1, record=True)
+    np.testing.assert_array_equal(np.stack(fields), U)
+    np.testing.assert_array_equal(fr, fo)
+    with pytest.raises(VespaHIPError, match="ghost"):
+        M.MusclGrid(dims, 1, G14)
+    with pytest.raises(VespaHIPError, match="2\\*ng\\+1"):
+        M.MusclGrid((4, 12, 12), 2, G14)
+    g = M.MusclGrid(dims, ng, G14)
+    with pytest.raises(VespaHIPError, match="recorded"):
+        g.fluxrec()
+    g.close()
+
+
+def test_benchmark_workload_one_step_bitwise(O):
+    """256³ active + 4 ghost zones per side, Float32, dt = 0.2·dx, dx = 1/256."""
+    print("muscl 264^3 f32 vs oracle", flush=True)
+    dims, ng = (264, 264, 264), 4
+    U = smooth_state(dims, seed=1234, dtype=np.float32, mach=0.3)
+    g = M.MusclGrid(dims, ng, G14, 1.5, 1.0 / 256, eltype=np.float32)
+    try:
+        g.upload(U)
+        g.step(0.2 / 256)
+        O.muscl_step_3d(U, dims, ng, 0.2 / 256, G14, 1.5, 1.0 / 256)
+        got = g.state()
+        assert np.array_equal(got, U)
+    finally:
+        g.close()

@@ -13,6 +13,7 @@ from .hydro import (BoundaryFluxRegister, HIPFlux, IdealHydro, Outflow, Periodic
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
                     dump_fields, evolve, prim2cons_array, run_steps, sod_problem, step)
 from .gravity import GravityField, enable_gravity, solve_poisson
+from .muscl import MusclGrid, cons_to_prim, muscl_step_3d, prim_to_cons, total_field
 
 __all__ = [
     "DeviceBuffer",
@@ -23,4 +24,5 @@ __all__ = [
     "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",
     "dump_fields", "evolve", "prim2cons_array", "run_steps", "sod_problem", "step",
     "GravityField", "enable_gravity", "solve_poisson",
+    "MusclGrid", "cons_to_prim", "muscl_step_3d", "prim_to_cons", "total_field",
 ]

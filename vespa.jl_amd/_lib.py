@@ -36,6 +36,9 @@ EXPORTED = (
     "vh_grid_step_bflux", "vh_grid_bflux_count",
     "vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
     "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel",
+    "vh_muscl_create", "vh_muscl_destroy", "vh_muscl_ncell", "vh_muscl_upload", "vh_muscl_download",
+    "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
+    "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps",
 )
 
 
@@ -148,6 +151,25 @@ def load_library(path: str | None = None):
     L.vh_grid_phi_download.argtypes = [vp, vp]
     L.vh_grid_apply_laplacian.argtypes = [vp, vp, vp]
     L.vh_grid_gravity_accel.argtypes = [vp, vp]
+    L.vh_muscl_create.argtypes = [C.POINTER(vp), i32p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_int32]
+    L.vh_muscl_destroy.argtypes = [vp]
+    L.vh_muscl_ncell.argtypes = [vp]
+    L.vh_muscl_ncell.restype = C.c_int64
+    L.vh_muscl_upload.argtypes = [vp, vp]
+    L.vh_muscl_download.argtypes = [vp, vp]
+    L.vh_muscl_state_device_ptr.argtypes = [vp, C.POINTER(vp)]
+    L.vh_muscl_set_tiles.argtypes = [vp, C.c_int32, C.c_int32]
+    L.vh_muscl_step.argtypes = [vp, C.c_double, C.c_int32]
+    L.vh_muscl_fluxrec_download.argtypes = [vp, vp]
+    L.vh_muscl_fill_periodic.argtypes = [vp]
+    L.vh_muscl_max_wavespeed.argtypes = [vp, dp]
+    L.vh_muscl_sync.argtypes = [vp]
+    L.vh_muscl_time_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, dp, dp]
+    for n in ("vh_muscl_create", "vh_muscl_destroy", "vh_muscl_upload", "vh_muscl_download",
+              "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
+              "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps"):
+        getattr(L, n).restype = C.c_int
     for n in ("vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
               "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel"):
         getattr(L, n).restype = C.c_int
