@@ -483,25 +483,24 @@ def main():
     from vespa_jl_amd import problems as P
 
     out = {}
+    calib = None
+    if args.calib_copy:
+        nb = 1 << 30
+        a, b = V.DeviceBuffer(nb), V.DeviceBuffer(nb)
+        b.copy_from(a)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            b.copy_from(a)
+        el = (time.perf_counter() - t0) / 3
+        calib = {"copy_bytes": nb, "copy_GBps_read_plus_write": 2 * nb / el / 1e9}
+        a.free()
+        b.free()
+        log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
     if not args.skip_uniform:
         n = args.n
         ncell = n ** 3
         log(f"[bench] building {n}^3 turbulence-like initial state")
         U64 = P.turbulence_state(n)
-
-        calib = None
-        if args.calib_copy:
-            nb = 1 << 30
-            a, b = V.DeviceBuffer(nb), V.DeviceBuffer(nb)
-            b.copy_from(a)
-            t0 = time.perf_counter()
-            for _ in range(3):
-                b.copy_from(a)
-            el = (time.perf_counter() - t0) / 3
-            calib = {"copy_bytes": nb, "copy_GBps_read_plus_write": 2 * nb / el / 1e9}
-            a.free()
-            b.free()
-            log(f"[bench] D2D copy 1 GiB: {calib['copy_GBps_read_plus_write']:.0f} GB/s (read+write)")
 
         log(f"[bench] FP64 run ({args.mode} profile)")
         sim = make_sim(V, n, np.float64, U64, args.tile, args.zchunk, args.mode)

@@ -40,22 +40,14 @@ struct MuW {
   T r, e, vx, vy, vz;
 };
 
-// Julia's max / min: NaN wins, then the larger / smaller value, ±0 ordered.
+// Julia's max / min on floats are IEEE 754-2019 maximum / minimum (NaN wins, −0 < +0).
 template <typename T>
 __device__ __forceinline__ T jmax(T x, T y) {
-  if (x != x) return x;
-  if (y != y) return y;
-  if (y > x) return y;
-  if (x > y) return x;
-  return __builtin_signbit(y) ? x : y;
+  return __builtin_elementwise_maximum(x, y);
 }
 template <typename T>
 __device__ __forceinline__ T jmin(T x, T y) {
-  if (x != x) return x;
-  if (y != y) return y;
-  if (y < x) return y;
-  if (x < y) return x;
-  return __builtin_signbit(y) ? y : x;
+  return __builtin_elementwise_minimum(x, y);
 }
 __device__ __forceinline__ double absT(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float absT(float x) { return __builtin_fabsf(x); }
@@ -70,6 +62,39 @@ __device__ __forceinline__ T plm_pt(T vm1, T v, T vp1, T th) {
   const T sa = a > T(0) ? T(1) : T(-1), sb = b > T(0) ? T(1) : T(-1), sc = c > T(0) ? T(1) : T(-1);
   const T s = T(0.25) * (sa + sb) * absT(sa + sc) * jmin(absT(a), jmin(absT(b), absT(c)));
   return v + T(0.5) * s;
+}
+// Both face values of the middle cell at once. The lo-face value is _plm_pt with
+// the neighbours exchanged (the reference's R state); its minmod arguments are the
+// exact negatives of the hi face's in swapped order, so it reuses their magnitudes
+// and minimum and only recomputes the signs: bit-identical to evaluating it anew,
+// zero signs included.
+// The leading factor of minmod3, 0.25·(sa+sb)·|sa+sd| with s = +1 for a positive
+// argument and −1 otherwise, is exactly ±1 when the three signs agree and a zero
+// otherwise: −0 when sa = sb = −1 ≠ sd, +0 in the other cases (the signs the
+// products give). Selected from the comparisons instead of multiplied out.
+template <typename T>
+__device__ __forceinline__ T minmod_factor(bool pa, bool pb, bool pd) {
+  const T full = pa ? T(1) : T(-1);
+  const T zero = pa ? T(0) : T(-0.0);
+  return pa == pb ? (pa == pd ? full : zero) : T(0);
+}
+template <typename T>
+__device__ __forceinline__ void plm_pair(T m, T c, T p, T th, T& hi, T& lo) {
+  const T a = (c - m) * th, b = (p - c) * th, d = T(0.5) * (p - m);
+  const T mn = jmin(absT(a), jmin(absT(b), absT(d)));
+  hi = c + T(0.5) * (minmod_factor<T>(a > T(0), b > T(0), d > T(0)) * mn);
+  lo = c + T(0.5) * (minmod_factor<T>(b < T(0), a < T(0), d < T(0)) * mn);
+}
+template <typename T>
+__device__ __forceinline__ void face_pair(const MuW<T>& m, const MuW<T>& c, const MuW<T>& p, T th, T sr, MuW<T>& hi,
+                                          MuW<T>& lo) {
+  plm_pair(m.r, c.r, p.r, th, hi.r, lo.r);
+  hi.r = jmax(hi.r, sr);
+  lo.r = jmax(lo.r, sr);
+  plm_pair(m.e, c.e, p.e, th, hi.e, lo.e);
+  plm_pair(m.vx, c.vx, p.vx, th, hi.vx, lo.vx);
+  plm_pair(m.vy, c.vy, p.vy, th, hi.vy, lo.vy);
+  plm_pair(m.vz, c.vz, p.vz, th, hi.vz, lo.vz);
 }
 // Hi-face state of cell c from its neighbours m (lo side) and p (hi side); the
 // lo-face state is the same with m and p exchanged (the reference's R state).
@@ -187,8 +212,16 @@ __device__ __forceinline__ void put_f(T* base, int stride, int i, const T F[5]) 
 
 __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
 
+// Waves per SIMD the stage kernel is compiled for (Float64 / Float32 state).
+#ifndef VH_MUSCL_WAVES
+#define VH_MUSCL_WAVES 2
+#endif
+#ifndef VH_MUSCL_WAVES_F32
+#define VH_MUSCL_WAVES_F32 4
+#endif
+
 template <typename T, int TX, int TY, bool STAGE2, bool FREC>
-__global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
+__global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : VH_MUSCL_WAVES) muscl_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
                                                                T* __restrict__ frec, MusclDev G) {
   constexpr int NT = TX * TY;
   __shared__ MusLDS<T, TX, TY> S;
@@ -197,8 +230,13 @@ __global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restr
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int nblk = G.tiles_x * G.tiles_y * G.chunks_z;
+  // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of tiles so
+  // tiles that share halos share an L2 (any count of workgroups).
   int b = blockIdx.x;
-  if (G.xcd_remap && (nblk % 8) == 0) b = (b % 8) * (nblk / 8) + b / 8;
+  if (G.xcd_remap) {
+    const int q = nblk / 8, r = nblk % 8, x = b % 8;
+    b = x * q + min(x, r) + b / 8;
+  }
   const int tile = b % (G.tiles_x * G.tiles_y);
   const int chunk = b / (G.tiles_x * G.tiles_y);
   const int x0 = (tile % G.tiles_x) * TX, y0 = (tile / G.tiles_x) * TY;
@@ -213,16 +251,17 @@ __global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restr
   auto zsrc = [&](int kk) { return col + (long long)clampi(kk, nz) * plane; };
 
   // ---- z pipeline prologue: planes kz0-2 .. kz0+2, the face below kz0 ----
-  MuW<T> wm1, w0, w1, w2;
+  MuW<T> w0, w1, w2, zhi;   // planes k, k+1, k+2 of the column; hi z state of plane k
   T fzlo[5];
   {
     const MuW<T> wm2 = to_prim(Us, n, zsrc(kz0 - 2), sr);
-    wm1 = to_prim(Us, n, zsrc(kz0 - 1), sr);
+    const MuW<T> wm1 = to_prim(Us, n, zsrc(kz0 - 1), sr);
     w0 = to_prim(Us, n, zsrc(kz0), sr);
     w1 = to_prim(Us, n, zsrc(kz0 + 1), sr);
     w2 = to_prim(Us, n, zsrc(kz0 + 2), sr);
-    if (kz0 >= ng && kz0 <= nz - ng)
-      hll_axis(face_state(wm2, wm1, w0, th, sr), face_state(w1, w0, wm1, th, sr), 2, g, gm1, fzlo);
+    MuW<T> zlo;
+    face_pair(wm1, w0, w1, th, sr, zhi, zlo);
+    if (kz0 >= ng && kz0 <= nz - ng) hll_axis(face_state(wm2, wm1, w0, th, sr), zlo, 2, g, gm1, fzlo);
   }
 
   // Halo cells of a plane (two deep, cross-shaped): NHX in x, NHY in y.
@@ -261,12 +300,11 @@ __global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restr
     MuW<T> xlo, ylo;
     {
       const int ic = (ty + 2) * L_::WX + tx + 2;
-      const MuW<T> wl = get_w(wf, L_::WS, ic - 1), wr = get_w(wf, L_::WS, ic + 1);
-      put_w(ff, L_::FS, L_::xs(ty, tx + 1), face_state(wl, w0, wr, th, sr));
-      xlo = face_state(wr, w0, wl, th, sr);
-      const MuW<T> wd = get_w(wf, L_::WS, ic - L_::WX), wu = get_w(wf, L_::WS, ic + L_::WX);
-      put_w(ff, L_::FS, L_::ys(ty + 1, tx), face_state(wd, w0, wu, th, sr));
-      ylo = face_state(wu, w0, wd, th, sr);
+      MuW<T> hi;
+      face_pair(get_w(wf, L_::WS, ic - 1), w0, get_w(wf, L_::WS, ic + 1), th, sr, hi, xlo);
+      put_w(ff, L_::FS, L_::xs(ty, tx + 1), hi);
+      face_pair(get_w(wf, L_::WS, ic - L_::WX), w0, get_w(wf, L_::WS, ic + L_::WX), th, sr, hi, ylo);
+      put_w(ff, L_::FS, L_::ys(ty + 1, tx), hi);
     }
     // Rim: hi state of the cell before the tile and lo state of the cell after
     // it, per row (x) and per column (y).
@@ -318,10 +356,11 @@ __global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restr
         put_f(ff, L_::FS, sl, F);
       }
     }
-    // z face k+½ of the lane's column
+    // z face k+½ of the lane's column: hi state of plane k, lo state of plane k+1
     T fzhi[5];
-    if (k + 1 >= ng && k + 1 <= nz - ng)
-      hll_axis(face_state(wm1, w0, w1, th, sr), face_state(w2, w1, w0, th, sr), 2, g, gm1, fzhi);
+    MuW<T> zhi1, zlo1;
+    face_pair(w0, w1, w2, th, sr, zhi1, zlo1);
+    if (k + 1 >= ng && k + 1 <= nz - ng) hll_axis(zhi, zlo1, 2, g, gm1, fzhi);
     __syncthreads();
 
     // ---- PC: divergence in the reference's order, RK combine, flux record ----
@@ -382,7 +421,7 @@ __global__ void __launch_bounds__(TX* TY, 2) muscl_stage_kernel(const T* __restr
 
     // advance the z pipeline
     for (int q = 0; q < 5; ++q) fzlo[q] = fzhi[q];
-    wm1 = w0;
+    zhi = zhi1;
     w0 = w1;
     w1 = w2;
     if (k + 1 < kz1) w2 = to_prim(Us, n, zsrc(k + 3), sr);
@@ -517,7 +556,7 @@ int vh_muscl_create(vh_muscl** out, const int32_t* dims, int32_t ng, double gamm
   vh_muscl* m = new vh_muscl();
   for (int d = 0; d < 3; ++d) m->G.n[d] = dims[d];
   m->G.ng = ng;
-  m->G.zchunk = 16;
+  m->G.zchunk = 8;
   m->G.xcd_remap = 1;
   m->G.ncell = n;
   m->G.gamma = gamma;
@@ -526,7 +565,7 @@ int vh_muscl_create(vh_muscl** out, const int32_t* dims, int32_t ng, double gamm
   m->dx = dx;
   m->dtype = dtype;
   m->esize = dtype == VH_F64 ? 8 : 4;
-  m->variant = dtype == VH_F64 ? 0 : 1;
+  m->variant = 0;
   set_tiles(m);
   if (hipGetDevice(&m->device) != hipSuccess) {
     delete m;

@@ -60,7 +60,7 @@ class MusclGrid:
         check(self._lib.vh_muscl_download(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
-    def set_tiles(self, variant: int = 0, zchunk: int = 16):
+    def set_tiles(self, variant: int = 0, zchunk: int = 8):
         """Tile shape: 0 = 16x16 cells, 1 = 32x8; zchunk z planes per workgroup."""
         check(self._lib.vh_muscl_set_tiles(self._h, int(variant), int(zchunk)))
 
