@@ -240,6 +240,58 @@ function hip_remap!(dst::HIPBlocks, src::HIPBlocks, src_block::AbstractVector{<:
     return dst
 end
 
+# ------------------------------------------------------------------ PPMKernels MUSCL
+
+"""
+    HIPMuscl(dims, ng; gamma, theta=1.5, dx=1.0, small_rho=1e-10, eltype=Float64)
+
+Device context for PPMKernels' `muscl_step_3d!` on a ghosted 3D grid
+(lib/PPMKernels/src/muscl_grid.jl:314-370): state SoA (D, S1, S2, S3, τ), x fastest.
+"""
+mutable struct HIPMuscl
+    ptr::Ptr{Cvoid}
+    ncell::Int
+    eltype::DataType
+end
+
+function HIPMuscl(dims::NTuple{3,Int}, ng::Integer; gamma::Real, theta::Real = 1.5, dx::Real = 1.0,
+                  small_rho::Real = 1e-10, eltype = Float64)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:vh_muscl_create, libvespahip), Cint,
+                 (Ref{Ptr{Cvoid}}, Ptr{Int32}, Int32, Float64, Float64, Float64, Float64, Int32),
+                 h, Int32[dims...], Int32(ng), Float64(gamma), Float64(theta), Float64(small_rho), Float64(dx),
+                 eltype === Float64 ? VH_F64 : VH_F32))
+    m = HIPMuscl(h[], prod(dims), eltype)
+    finalizer(m) do x
+        x.ptr == C_NULL || ccall((:vh_muscl_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    return m
+end
+
+hip_upload!(m::HIPMuscl, U::Vector) = (GC.@preserve U _check(ccall((:vh_muscl_upload, libvespahip), Cint,
+    (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U)); m)
+
+function hip_download(m::HIPMuscl)
+    U = Vector{m.eltype}(undef, 5 * m.ncell)
+    GC.@preserve U _check(ccall((:vh_muscl_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U))
+    return U
+end
+
+"muscl_step_3d!(…; dt, fluxrec) on the device state; record = fluxrec !== nothing."
+hip_muscl_step!(m::HIPMuscl, dt::Real; record::Bool = false) =
+    (_check(ccall((:vh_muscl_step, libvespahip), Cint, (Ptr{Cvoid}, Float64, Int32), m.ptr, Float64(dt),
+                  Int32(record))); m)
+
+hip_fill_periodic!(m::HIPMuscl) = (_check(ccall((:vh_muscl_fill_periodic, libvespahip), Cint, (Ptr{Cvoid},),
+                                                m.ptr)); m)
+
+function hip_max_wavespeed(m::HIPMuscl)
+    s = Ref{Float64}(0.0)
+    _check(ccall((:vh_muscl_max_wavespeed, libvespahip), Cint, (Ptr{Cvoid}, Ref{Float64}), m.ptr, s))
+    return s[]
+end
+
 # ------------------------------------------------------------------ reverse mode
 
 """
@@ -315,6 +367,6 @@ end
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
        HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
-       hip_phi
+       hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_max_wavespeed
 
 end # module

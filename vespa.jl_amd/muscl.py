@@ -127,7 +127,7 @@ def muscl_step_3d(D, S1, S2, S3, Tau, dims, ng, *, dt, gamma, theta=1.5, dx=1.0,
         g.step(dt, fluxrec is not None)
         U = g.state()
         for f, u in zip(fields, U):
-            f.reshape(-1)[:] = u
+            np.copyto(f, u.reshape(np.shape(f)))
         if fluxrec is not None:
             fluxrec[...] = g.fluxrec()
     finally:
