@@ -361,6 +361,14 @@ int vh_muscl_set_tiles(vh_muscl* m, int32_t variant, int32_t zchunk);
  * vh_muscl_fluxrec_download, 15·ncell values [axis][field][cell]. */
 int vh_muscl_step(vh_muscl* m, double dt, int32_t record);
 int vh_muscl_fluxrec_download(vh_muscl* m, void* frec);
+/* boundary_flux_register (lib/PPMKernels/src/reflux_capture.jl:86-131) of the last
+ * recording step: face fluxes as ∫F·area·dt = F·dt·dx² in Float64. act = dims − 2·ng;
+ * for each axis a (x, y, z) in turn, with b1 < b2 the other two axes and u1, u2 their
+ * active cells: planes receives the lo boundary plane (the lo face of the first active
+ * cell) then the hi plane (the hi face of the last), each [field][u2][u1] (10·act_b1·act_b2
+ * values per axis); interior (may be NULL) the hi faces of active cells 1 .. act_a−1,
+ * [field][u2][u1][c] (5·(act_a−1)·act_b1·act_b2 values per axis). */
+int vh_muscl_flux_register(vh_muscl* m, double dt, double* planes, double* interior);
 /* fill_periodic! (lib/PPMKernels/src/muscl_grid.jl:644-670) on the five fields. */
 int vh_muscl_fill_periodic(vh_muscl* m);
 /* max_wavespeed (lib/PPMKernels/src/muscl_grid.jl:672-695) over every cell. */

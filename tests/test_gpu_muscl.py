@@ -139,3 +139,38 @@ def test_benchmark_workload_one_step_bitwise(O):
         assert np.array_equal(got, U)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_flux_register_gather(O, dtype):
+    """boundary_flux_register on the device equals the host-loop reference over the
+    same recorded fluxes, key for key and value for value (the reference's index-
+    mapping lock, test/test_reflux_capture.jl:36-51)."""
+    from test_muscl_oracle import register_ref, _rec_state
+    n, ng, dt = 12, 3, 0.008
+    dx = 1.0 / n
+    U, dims = _rec_state(n, ng)
+    g = M.MusclGrid(dims, ng, G14, 1.5, dx, eltype=dtype)
+    try:
+        g.upload(U.astype(dtype))
+        g.step(dt, fluxrec=True)
+        frec = g.fluxrec()
+        reg = M.boundary_flux_register(g, dt)
+        flux, interior, act = register_ref(frec, dims, ng, dt, dx)
+        assert reg.act == act
+        assert reg.flux == flux
+        assert reg.interior == interior
+    finally:
+        g.close()
+    # an anisotropic grid with one active cell along z (no interior z faces)
+    dims2 = (14, 11, 7)
+    U2 = smooth_state(dims2, seed=3, dtype=dtype)
+    g = M.MusclGrid(dims2, 3, G14, 1.5, 0.1, eltype=dtype)
+    try:
+        g.upload(U2)
+        g.step(0.004, fluxrec=True)
+        reg = M.boundary_flux_register(g, 0.004)
+        flux, interior, act = register_ref(g.fluxrec(), dims2, 3, 0.004, 0.1)
+        assert reg.flux == flux and reg.interior == interior and act[2] == 1
+    finally:
+        g.close()

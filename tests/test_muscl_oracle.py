@@ -283,3 +283,54 @@ def test_fill_periodic_and_max_wavespeed(O):
     ref = (np.sqrt(np.maximum(1.4 * 0.4 * eint, 0.0)) + np.maximum(np.abs(u), np.maximum(np.abs(v), np.abs(w)))).max()
     assert (eint < 0).any()
     assert O.muscl_max_wavespeed(U, 1.4) == pytest.approx(ref, rel=1e-15)
+
+
+def register_ref(frec, dims, ng, dt, dx):
+    """boundary_flux_register_ref (lib/PPMKernels/src/reflux_capture.jl:140-165):
+    host loops over the full recorded fluxes, 1-based keys."""
+    act = tuple(d - 2 * ng for d in dims)
+    sc = float(dt) * float(dx) ** 2
+    nx, ny, nz = dims
+    fr = frec.reshape(3, 5, nz, ny, nx)
+
+    def F(a, p, b1, b2, u1, u2):
+        c = [0, 0, 0]
+        c[a - 1], c[b1 - 1], c[b2 - 1] = p - 1, ng + u1 - 1, ng + u2 - 1
+        return tuple(float(fr[a - 1, f, c[2], c[1], c[0]]) * sc for f in range(5))
+
+    def key(a, k, b1, b2, u1, u2):
+        return tuple(k if d == a else (u1 if d == b1 else u2) for d in (1, 2, 3))
+    flux, interior = {}, {}
+    for a in (1, 2, 3):
+        b1, b2 = (2, 3) if a == 1 else ((1, 3) if a == 2 else (1, 2))
+        for u2 in range(1, act[b2 - 1] + 1):
+            for u1 in range(1, act[b1 - 1] + 1):
+                for c in range(1, act[a - 1]):
+                    interior[(a, key(a, c, b1, b2, u1, u2))] = F(a, ng + c + 1, b1, b2, u1, u2)
+                flux[(a, "lo", key(a, 1, b1, b2, u1, u2))] = F(a, ng + 1, b1, b2, u1, u2)
+                flux[(a, "hi", key(a, act[a - 1], b1, b2, u1, u2))] = F(a, ng + act[a - 1] + 1, b1, b2, u1, u2)
+    return flux, interior, act
+
+
+def test_boundary_flux_register_conservation_identity(O):
+    """test/test_reflux_capture.jl:54-70 with the unsplit driver: the extensive change
+    of each field over the active cells equals Σ lo-plane − Σ hi-plane fluxes; the
+    interior register holds every active cell's hi face but the last."""
+    n, ng, dt = 12, 3, 0.008
+    dx = 1.0 / n
+    U, dims = _rec_state(n, ng)
+    U0 = U.copy()
+    frec = O.muscl_step_3d(U, dims, ng, dt, G14, 1.5, dx, record=True)
+    flux, interior, act = register_ref(frec, dims, ng, dt, dx)
+    assert len(flux) == 3 * 2 * n * n and len(interior) == 3 * (n - 1) * n * n
+    nx, ny, nz = dims
+    sl = (slice(ng, nz - ng), slice(ng, ny - ng), slice(ng, nx - ng))
+    for f in range(5):
+        dQ = ((U[f] - U0[f]).reshape(nz, ny, nx)[sl]).sum() * dx ** 3
+        slo = sum(v[f] for (a, side, _), v in flux.items() if side == "lo")
+        shi = sum(v[f] for (a, side, _), v in flux.items() if side == "hi")
+        assert dQ == pytest.approx(slo - shi, rel=1e-11, abs=1e-13)
+    # an interior face is the lo face of the next cell: the x-face between active cells (1,1,1) and (2,1,1)
+    i0 = ng + 1
+    lin = (ng * ny + ng) * nx + i0
+    assert interior[(1, (1, 1, 1))] == tuple(float(frec[0, f, lin]) * (dt * (dx * dx)) for f in range(5))

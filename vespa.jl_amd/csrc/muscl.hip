@@ -474,6 +474,48 @@ __global__ void __launch_bounds__(256) wavespeed_kernel(const T* __restrict__ U,
   if (threadIdx.x == 0) partials[blockIdx.x] = jmax(jmax(red[0], red[1]), jmax(red[2], red[3]));
 }
 
+// boundary_flux_register's gathers (lib/PPMKernels/src/reflux_capture.jl:52-77,86-131) for
+// one axis a: the lo and hi boundary planes ([side][field][u2][u1]) and the interior
+// hi faces of active cells 1..act-1 ([field][u2][u1][c]), each value Float64(F)·sc.
+// b1 < b2 are the other two axes; u1, u2 run over their active cells.
+template <typename T>
+__global__ void __launch_bounds__(256) flux_register_kernel(const T* __restrict__ frec, MusclDev G, int a, double sc,
+                                                            double* __restrict__ planes, double* __restrict__ inner) {
+  const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
+  const int ng = G.ng;
+  const int act[3] = {G.n[0] - 2 * ng, G.n[1] - 2 * ng, G.n[2] - 2 * ng};
+  const long long n = G.ncell;
+  const long long str[3] = {1, G.n[0], (long long)G.n[0] * G.n[1]};
+  const long long m1 = act[b1], m2 = act[b2];
+  const long long nplane = 2 * 5 * m1 * m2;
+  const long long ninner = inner && act[a] > 1 ? 5 * (long long)(act[a] - 1) * m1 * m2 : 0;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < nplane + ninner; t += (long long)gridDim.x * 256) {
+    long long q = t < nplane ? t : t - nplane;
+    int p, comp;
+    long long u1, u2;
+    if (t < nplane) {
+      u1 = q % m1;
+      q /= m1;
+      u2 = q % m2;
+      q /= m2;
+      comp = (int)(q % 5);
+      p = q / 5 ? ng + act[a] : ng;
+    } else {
+      const long long c = q % (act[a] - 1);
+      q /= act[a] - 1;
+      u1 = q % m1;
+      q /= m1;
+      u2 = q % m2;
+      comp = (int)(q / m2);
+      p = ng + (int)c + 1;
+    }
+    const long long cell = p * str[a] + (ng + u1) * str[b1] + (ng + u2) * str[b2];
+    const double v = (double)frec[((long long)a * 5 + comp) * n + cell] * sc;
+    if (t < nplane) planes[t] = v;
+    else inner[t - nplane] = v;
+  }
+}
+
 constexpr int kWsWgs = 1024;
 
 }  // namespace
@@ -672,6 +714,46 @@ int vh_muscl_max_wavespeed(vh_muscl* m, double* out) {
   double r = p[0];
   for (int i = 1; i < nb; ++i) r = std::isnan(r) ? r : (std::isnan(p[i]) || p[i] > r ? p[i] : r);
   *out = r;
+  return 0;
+}
+
+int vh_muscl_flux_register(vh_muscl* m, double dt, double* planes, double* interior) {
+  if (!m || !planes) return fail(-1, "null argument%s", "");
+  if (!m->frec) return fail(-1, "no step has recorded fluxes yet%s", "");
+  VH_HIP(hipSetDevice(m->device));
+  const int ng = m->G.ng;
+  const long long act[3] = {m->G.n[0] - 2LL * ng, m->G.n[1] - 2LL * ng, m->G.n[2] - 2LL * ng};
+  long long np = 0, ni = 0;
+  for (int a = 0; a < 3; ++a) {
+    const long long t = act[(a + 1) % 3] * act[(a + 2) % 3];
+    np += 10 * t;
+    ni += 5 * (act[a] - 1) * t;
+  }
+  double* d = nullptr;
+  VH_HIP(hipMalloc(&d, sizeof(double) * (np + (interior ? ni : 0))));
+  const double sc = dt * (m->dx * m->dx);   // ∫F·area·dt = F·dt·dx²
+  long long op = 0, oi = 0;
+  hipError_t e = hipSuccess;
+  for (int a = 0; a < 3 && e == hipSuccess; ++a) {
+    const long long t = act[(a + 1) % 3] * act[(a + 2) % 3];
+    const long long work = 10 * t + (interior ? 5 * (act[a] - 1) * t : 0);
+    const unsigned nb = (unsigned)std::min<long long>(4096, (work + 255) / 256);
+    double* pl = d + op;
+    double* in = interior ? d + np + oi : nullptr;
+    if (m->dtype == VH_F64)
+      flux_register_kernel<double><<<nb, 256, 0, m->stream>>>((const double*)m->frec, m->G, a, sc, pl, in);
+    else
+      flux_register_kernel<float><<<nb, 256, 0, m->stream>>>((const float*)m->frec, m->G, a, sc, pl, in);
+    e = hipGetLastError();
+    op += 10 * t;
+    oi += 5 * (act[a] - 1) * t;
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(planes, d, sizeof(double) * np, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess && interior)
+    e = hipMemcpyAsync(interior, d + np, sizeof(double) * ni, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(-2, "HIP error %s in the flux register gather%s", hipGetErrorString(e));
   return 0;
 }
 

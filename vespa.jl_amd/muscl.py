@@ -95,6 +95,70 @@ class MusclGrid:
         return {"step_ms": a.value, "stage2_ms": b.value}
 
 
+class BoundaryFluxSet:
+    """The reflux fluxes of one grid (lib/PPMKernels/src/reflux_capture.jl:38-50):
+    ``flux[(axis, side, cell)]`` on the outer boundary faces, side "lo"/"hi", and
+    ``interior[(axis, cell)]`` on the hi face of each active cell but the last along
+    the axis; values are 5-tuples of ∫F·area·dt. Axes and cell indices are 1-based
+    active indices (the reference's keys); ``act`` the active dims."""
+
+    def __init__(self, flux: dict, interior: dict, act: tuple):
+        self.flux, self.interior, self.act = flux, interior, act
+
+
+def _trans3(a):
+    return (2, 3) if a == 1 else ((1, 3) if a == 2 else (1, 2))
+
+
+def _key(a, k, b1, b2, u1, u2):
+    return tuple(k if d == a else (u1 if d == b1 else u2) for d in (1, 2, 3))
+
+
+def flux_register_arrays(g: MusclGrid, dt: float, interior: bool = True):
+    """vh_muscl_flux_register: per axis (planes (2, 5, act_b2, act_b1), interior
+    (5, act_b2, act_b1, act_a − 1) or None), Float64 ∫F·area·dt."""
+    act = [d - 2 * g.ng for d in g.dims]
+    sizes = []
+    for a in range(3):
+        b1, b2 = [d for d in range(3) if d != a]
+        sizes.append((act[b1], act[b2], act[a]))
+    npl = sum(10 * m1 * m2 for m1, m2, _ in sizes)
+    nin = sum(5 * (ma - 1) * m1 * m2 for m1, m2, ma in sizes)
+    pl = np.empty(npl)
+    inn = np.empty(max(nin, 1)) if interior else None
+    dp = C.POINTER(C.c_double)
+    check(g._lib.vh_muscl_flux_register(g._h, float(dt), pl.ctypes.data_as(dp),
+                                        inn.ctypes.data_as(dp) if interior else None))
+    out, op, oi = [], 0, 0
+    for m1, m2, ma in sizes:
+        p = pl[op:op + 10 * m1 * m2].reshape(2, 5, m2, m1)
+        op += 10 * m1 * m2
+        i = None
+        if interior:
+            i = inn[oi:oi + 5 * (ma - 1) * m1 * m2].reshape(5, m2, m1, ma - 1)
+            oi += 5 * (ma - 1) * m1 * m2
+        out.append((p, i))
+    return out
+
+
+def boundary_flux_register(g: MusclGrid, dt: float) -> BoundaryFluxSet:
+    """boundary_flux_register(frec, gd, ng, dt, dx) (lib/PPMKernels/src/reflux_capture.jl:86-131)
+    for the fluxes recorded by the grid's last ``step(dt, fluxrec=True)``: the device
+    gathers the face planes, the host keys them."""
+    act = tuple(d - 2 * g.ng for d in g.dims)
+    flux, interior = {}, {}
+    for a, (p, inn) in enumerate(flux_register_arrays(g, dt), start=1):
+        b1, b2 = _trans3(a)
+        ma, m1, m2 = act[a - 1], act[b1 - 1], act[b2 - 1]
+        for u2 in range(m2):
+            for u1 in range(m1):
+                flux[(a, "lo", _key(a, 1, b1, b2, u1 + 1, u2 + 1))] = tuple(float(v) for v in p[0, :, u2, u1])
+                flux[(a, "hi", _key(a, ma, b1, b2, u1 + 1, u2 + 1))] = tuple(float(v) for v in p[1, :, u2, u1])
+                for c in range(ma - 1):
+                    interior[(a, _key(a, c + 1, b1, b2, u1 + 1, u2 + 1))] = tuple(float(v) for v in inn[:, u2, u1, c])
+    return BoundaryFluxSet(flux, interior, act)
+
+
 def prim_to_cons(rho, vx, vy, vz, etot):
     """prim_to_cons! (lib/PPMKernels/src/muscl_grid.jl:612-619) → (5, N)."""
     return np.stack([rho, rho * vx, rho * vy, rho * vz, rho * etot])
