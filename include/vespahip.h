@@ -340,7 +340,7 @@ int vh_blocks_time_steps(vh_blocks* m, double cfl, int64_t nsteps, int32_t metho
  * U[f*ncell + cell] with f = D, S1, S2, S3, τ (τ = ρ·etot). Ghost zones are not
  * refilled by a step (the reference's bc! = nothing); vh_muscl_fill_periodic is
  * fill_periodic!. Arithmetic at the state's precision, bit-identical to the reference's
- * operation order. No dual-energy field.
+ * operation order. The dual-energy field is optional (vh_muscl_enable_dual_energy).
  */
 typedef struct vh_muscl vh_muscl;
 
@@ -350,6 +350,13 @@ int vh_muscl_create(vh_muscl** out, const int32_t* dims, int32_t ng, double gamm
                     double small_rho, double dx, int32_t dtype);
 int vh_muscl_destroy(vh_muscl* m);
 int64_t vh_muscl_ncell(const vh_muscl* m);
+/* The dual-energy formalism (muscl_step_3d!(…; ge, eta1)): the state gains the gas
+ * energy ρe as a sixth field (initialised to 0; upload the six fields after this call),
+ * eint is selected by _dual_eint, ρe is advected with the HLL flux ρe·u and synced with
+ * τ at the end of each step (lib/PPMKernels/src/muscl_grid.jl:34-115); recorded
+ * fluxes then hold 6 fields per axis. Idempotent; eta1 may be changed later. */
+int vh_muscl_enable_dual_energy(vh_muscl* m, double eta1);
+/* 5 (or 6) fields × ncell values, field-major. */
 int vh_muscl_upload(vh_muscl* m, const void* U);
 int vh_muscl_download(vh_muscl* m, void* U);
 int vh_muscl_state_device_ptr(vh_muscl* m, void** ptr);
@@ -358,7 +365,7 @@ int vh_muscl_set_tiles(vh_muscl* m, int32_t variant, int32_t zchunk);
 /* One muscl_step_3d!(…; dt): two launches, stream-ordered. record != 0 is
  * fluxrec: the recorded fluxes (zeroed, then ½F per stage at the lo face of each
  * cell, lib/PPMKernels/src/muscl_grid.jl:278-291) are read with
- * vh_muscl_fluxrec_download, 15·ncell values [axis][field][cell]. */
+ * vh_muscl_fluxrec_download, 3·nfields·ncell values [axis][field][cell]. */
 int vh_muscl_step(vh_muscl* m, double dt, int32_t record);
 int vh_muscl_fluxrec_download(vh_muscl* m, void* frec);
 /* boundary_flux_register (lib/PPMKernels/src/reflux_capture.jl:86-131) of the last

@@ -530,8 +530,9 @@ def _muscl_typed():
         vp = C.c_void_p
         i64p = C.POINTER(C.c_int64)
         for suf in ("f64", "f32"):
-            getattr(L, f"vo_muscl_flux_line_{suf}").argtypes = [vp] * 5 + [C.c_int64] * 3 + [C.c_double] * 3 + [vp] * 5
-            getattr(L, f"vo_muscl_step_3d_{suf}").argtypes = [vp, i64p, C.c_int64] + [C.c_double] * 5 + [vp]
+            getattr(L, f"vo_muscl_flux_line_{suf}").argtypes = [vp] * 5 + [C.c_int64] * 3 + [C.c_double] * 3 + [vp] * 6
+            getattr(L, f"vo_muscl_step_3d_{suf}").argtypes = [vp, i64p, C.c_int64] + [C.c_double] * 5 + \
+                [C.c_int, C.c_double, vp]
             getattr(L, f"vo_muscl_fill_periodic_{suf}").argtypes = [vp, i64p, C.c_int64]
             getattr(L, f"vo_muscl_max_wavespeed_{suf}").argtypes = [vp, C.c_int64, C.c_double]
             getattr(L, f"vo_muscl_max_wavespeed_{suf}").restype = C.c_double
@@ -539,29 +540,33 @@ def _muscl_typed():
     return L
 
 
-def muscl_flux_line(rho, eint, vx, vy, vz, ncells, nghost, jdim=1, gamma=1.4, theta=1.5, small_rho=1e-10):
-    """muscl_flux_line! (lib/PPMKernels/src/muscl.jl:223-239) → (fd, fs1, fs2, fs3, fe),
+def muscl_flux_line(rho, eint, vx, vy, vz, ncells, nghost, jdim=1, gamma=1.4, theta=1.5, small_rho=1e-10,
+                    ge_flux=False):
+    """muscl_flux_line! (lib/PPMKernels/src/muscl.jl:223-239) → (fd, fs1, fs2, fs3, fe[, fge]),
     each of (ncells − 2·nghost + 1)·jdim interfaces."""
     L = _muscl_typed()
     arrs = [np.ascontiguousarray(a) for a in (rho, eint, vx, vy, vz)]
     dt = arrs[0].dtype
     nfi = ncells - 2 * nghost + 1
-    out = [np.empty(nfi * jdim, dtype=dt) for _ in range(5)]
+    out = [np.empty(nfi * jdim, dtype=dt) for _ in range(6 if ge_flux else 5)]
     getattr(L, f"vo_muscl_flux_line_{_suf(dt)}")(*[a.ctypes.data for a in arrs], int(ncells), int(nghost), int(jdim),
                                                   float(gamma), float(theta), float(small_rho),
-                                                  *[o.ctypes.data for o in out])
+                                                  *[o.ctypes.data for o in out], out[5].ctypes.data if ge_flux else None)
     return tuple(out)
 
 
-def muscl_step_3d(U, dims, ng, dt, gamma=1.4, theta=1.5, dx=1.0, small_rho=1e-10, record=False):
-    """muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:314-370) on U (5, N) in place;
-    with record=True returns the recorded fluxes (3, 5, N) (½F per stage)."""
+def muscl_step_3d(U, dims, ng, dt, gamma=1.4, theta=1.5, dx=1.0, small_rho=1e-10, record=False, eta1=1e-3):
+    """muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:314-370) on U in place: (5, N), or
+    (6, N) with the gas energy ρe as the sixth field (dual-energy formalism, eta1);
+    with record=True returns the recorded fluxes (3, nf, N) (½F per stage)."""
     L = _muscl_typed()
-    assert U.flags.c_contiguous and U.shape[0] == 5
+    assert U.flags.c_contiguous and U.shape[0] in (5, 6)
+    nf = U.shape[0]
     d = (C.c_int64 * 3)(*dims)
-    frec = np.empty((3, 5, U.shape[1]), dtype=U.dtype) if record else None
+    frec = np.empty((3, nf, U.shape[1]), dtype=U.dtype) if record else None
     getattr(L, f"vo_muscl_step_3d_{_suf(U.dtype)}")(U.ctypes.data, d, int(ng), float(dt), float(gamma), float(theta),
-                                                     float(dx), float(small_rho), frec.ctypes.data if record else None)
+                                                     float(dx), float(small_rho), nf, float(eta1),
+                                                     frec.ctypes.data if record else None)
     return frec
 
 

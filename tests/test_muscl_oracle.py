@@ -334,3 +334,90 @@ def test_boundary_flux_register_conservation_identity(O):
     i0 = ng + 1
     lin = (ng * ny + ng) * nx + i0
     assert interior[(1, (1, 1, 1))] == tuple(float(frec[0, f, lin]) * (dt * (dx * dx)) for f in range(5))
+
+
+def _with_ge(U):
+    rho = U[0]
+    u, v, w = U[1] / rho, U[2] / rho, U[3] / rho
+    eint = U[4] / rho - 0.5 * (u * u + v * v + w * w)
+    return np.vstack([U, rho * eint]).copy()
+
+
+def test_gas_energy_flux_upwinds(O):
+    """The sixth flux is ρe·u of the upwind state: rightward supersonic takes the L
+    state, leftward the R state (a velocity jump makes them differ)."""
+    n, ng = 12, 3
+    rho, e = np.full(n, 2.0), np.full(n, 0.5)
+    z = np.zeros(n)
+    for sgn in (1.0, -1.0):
+        u = np.where(np.arange(n) < 6, 5.0, 6.0) * sgn     # jump at the face between cells 5 and 6
+        F = O.muscl_flux_line(rho, e, u, z, z, n, ng, ge_flux=True)
+        f = 6 - ng                                          # that face's index
+        up = 5.0 if sgn > 0 else 6.0
+        np.testing.assert_allclose(F[5][f], 2.0 * 0.5 * up * sgn, rtol=2e-16)
+        np.testing.assert_allclose(F[0][f], 2.0 * up * sgn, rtol=2e-16)
+
+
+def test_dual_energy_subsonic_is_inert(O):
+    """test/test_muscl_grid.jl:550-593 (rk2 driver): with the gas energy on, a subsonic
+    blob conserves mass, keeps ρe > 0, and its density is bit-identical to the run
+    without it (the η₁ selection picks the total-energy eint)."""
+    m, ng, dims = 26, 3, (26, 26, 26)
+    A = _with_ge(_blob(m, 13.5, 2.2))
+    B = _blob(m, 13.5, 2.2)
+    m0 = _total(A[0], dims, ng, 1.0)
+    for _ in range(3):
+        O.muscl_step_3d(A, dims, ng, 0.02, G14, 1.5, 1.0)
+        O.muscl_step_3d(B, dims, ng, 0.02, G14, 1.5, 1.0)
+    assert abs(_total(A[0], dims, ng, 1.0) - m0) / m0 < 1e-11
+    assert A[5].min() > 0
+    assert np.array_equal(A[0], B[0])
+
+
+def test_dual_energy_selection_and_sync(O):
+    """A uniform state has no flux divergence, so one step only applies the end-of-step
+    sync: a cold fast flow (γ(γ−1)·eint_tot ≤ η₁|v|²) takes eint from ρe, a warm slow
+    one from τ; ρe and τ are then rewritten from the chosen eint."""
+    ng, dims = 2, (5, 5, 5)
+    n = 125
+
+    def run(rho, vel, e_tot, e_dual):
+        v2 = vel * vel
+        U = np.stack([np.full(n, rho), np.full(n, rho * vel), np.zeros(n), np.zeros(n),
+                      np.full(n, rho * (e_tot + 0.5 * v2)), np.full(n, rho * e_dual)])
+        O.muscl_step_3d(U, dims, ng, 1e-3, G14, 1.5, 1.0)
+        return U
+    cold = run(2.0, 10.0, 1e-6, 5e-6)
+    np.testing.assert_allclose(cold[5], 2.0 * 5e-6, rtol=1e-12)
+    np.testing.assert_allclose(cold[4], 2.0 * (5e-6 + 50.0), rtol=1e-14)
+    warm = run(2.0, 0.1, 2.0, 1.0)
+    np.testing.assert_allclose(warm[5], 2.0 * 2.0, rtol=1e-12)
+    np.testing.assert_allclose(warm[4], 2.0 * (2.0 + 0.005), rtol=1e-14)
+    low = run(2.0, 0.1, 2.0, 5.0)                     # eint_tot ≤ ½·eint_dual: the advected one wins
+    np.testing.assert_allclose(low[5], 2.0 * 5.0, rtol=1e-12)
+
+
+def test_dual_energy_gas_energy_is_advected_conservatively(O):
+    """A cold blob in a fast bulk flow takes eint from ρe everywhere (γ(γ−1)eint ≪
+    η₁|v|²), so the sync keeps ρe and its change is the divergence of the recorded
+    sixth flux ρe·u, cell by cell, to round-off."""
+    m, ng, dims = 20, 3, (20, 20, 20)
+    U = _blob(m, 10.5, 2.0, amp_r=0.5, amp_p=0.0)
+    rho = U[0]
+    vel = (1.0, 0.5, -0.25)
+    eint = np.full(rho.size, 1e-5)
+    for q in range(3):
+        U[1 + q] = rho * vel[q]
+    U[4] = rho * (eint + 0.5 * sum(v * v for v in vel))
+    A = np.vstack([U, rho * eint]).copy()
+    A0 = A.copy()
+    dt, dx = 0.01, 1.0
+    frec = O.muscl_step_3d(A, dims, ng, dt, G14, 1.5, dx, record=True)
+    assert frec.shape == (3, 6, A.shape[1])
+    nx, ny, nz = dims
+    ea = (1, nx, nx * ny)
+    idx = np.arange(A.shape[1]).reshape(nz, ny, nx)[ng:nz - ng, ng:ny - ng, ng:nx - ng].ravel()
+    for f in (0, 5):
+        div = sum((frec[a, f, idx + ea[a]] - frec[a, f, idx]) * (dt / dx) for a in range(3))
+        assert np.abs((A[f, idx] - A0[f, idx]) + div).max() < 1e-13 * np.abs(A0[f]).max()
+    assert A[5].min() > 0

@@ -30,9 +30,10 @@ struct MusclDev {
   int n[3];
   int ng;
   int tiles_x, tiles_y, chunks_z, zchunk;
-  int xcd_remap, pad;
+  int xcd_remap, nf;                  // nf: 5 fields, or 6 with the gas energy ρe
   long long ncell;
   double gamma, theta, small, dtdx;   // dtdx = T(dt)/T(dx), exact in T
+  double eta1;                        // dual-energy selection threshold
 };
 
 template <typename T>
@@ -117,8 +118,8 @@ __device__ __forceinline__ T hll1(T ap, T am, T fl, T fr, T ul, T ur) {
 // _hll6's hydro fluxes (lib/PPMKernels/src/muscl.jl:37-58) for states whose
 // velocities are already in (normal, t1, t2) order in (vx, vy, vz). F: D, normal,
 // t1, t2 momentum, energy.
-template <typename T>
-__device__ __forceinline__ void hll_rot(const MuW<T>& L, const MuW<T>& R, T g, T gm1, T F[5]) {
+template <typename T, int NF>
+__device__ __forceinline__ void hll_rot(const MuW<T>& L, const MuW<T>& R, T g, T gm1, T F[NF]) {
   const T h = T(0.5);
   const T rl = L.r, el = L.e, ul = L.vx, vl = L.vy, wl = L.vz;
   const T rr = R.r, er = R.e, ur = R.vx, vr = R.vy, wr = R.vz;
@@ -137,6 +138,10 @@ __device__ __forceinline__ void hll_rot(const MuW<T>& L, const MuW<T>& R, T g, T
   F[2] = hll1(ap, am, FlS2, FrS2, UlS2, UrS2);
   F[3] = hll1(ap, am, FlS3, FrS3, UlS3, UrS3);
   F[4] = hll1(ap, am, FlE, FrE, UlE, UrE);
+  if constexpr (NF == 6) {   // _hll6's gas-energy flux: U = ρe, F = ρe·u
+    const T UlG = rl * el, UrG = rr * er;
+    F[5] = hll1(ap, am, UlG * ul, UrG * ur, UlG, UrG);
+  }
 }
 
 // Velocities of a state in the cyclic roles of axis a: (normal, t1, t2) =
@@ -151,13 +156,14 @@ __device__ __forceinline__ MuW<T> roles(const MuW<T>& w, int a) {
   }
   return o;
 }
-// HLL flux through a face normal to axis a, returned in grid order (D, Sx, Sy, Sz, E).
-template <typename T>
-__device__ __forceinline__ void hll_axis(const MuW<T>& L, const MuW<T>& R, int a, T g, T gm1, T out[5]) {
-  T F[5];
-  hll_rot(roles(L, a), roles(R, a), g, gm1, F);
+// HLL flux through a face normal to axis a, returned in grid order (D, Sx, Sy, Sz, E[, ρe]).
+template <typename T, int NF>
+__device__ __forceinline__ void hll_axis(const MuW<T>& L, const MuW<T>& R, int a, T g, T gm1, T out[NF]) {
+  T F[NF];
+  hll_rot<T, NF>(roles(L, a), roles(R, a), g, gm1, F);
   out[0] = F[0];
   out[4] = F[4];
+  if constexpr (NF == 6) out[5] = F[5];
   if (a == 0) {
     out[1] = F[1]; out[2] = F[2]; out[3] = F[3];
   } else if (a == 1) {
@@ -167,9 +173,22 @@ __device__ __forceinline__ void hll_axis(const MuW<T>& L, const MuW<T>& R, int a
   }
 }
 
-// cons → prim (_cons2prim_k!, lib/PPMKernels/src/muscl_grid.jl:23-33), eint floored.
+// _dual_eint (lib/PPMKernels/src/muscl_grid.jl:41-47): the total-energy eint where the
+// pressure is dynamically significant, else the advected one; floored.
 template <typename T>
-__device__ __forceinline__ MuW<T> to_prim(const T* __restrict__ U, long long n, long long c, T sr) {
+__device__ __forceinline__ T dual_eint(T tau, T d, T ge, T v2, T g, T gm1, T eta1, T small) {
+  const T eint_tot = tau / d - T(0.5) * v2;
+  const T eint_dual = ge / d;
+  const T cs2 = eint_tot > T(0) ? g * gm1 * eint_tot : T(0);
+  const bool use_tot = (cs2 > eta1 * v2) && (eint_tot > T(0.5) * eint_dual);
+  return jmax(use_tot ? eint_tot : eint_dual, small);
+}
+
+// cons → prim (_cons2prim_k!, lib/PPMKernels/src/muscl_grid.jl:23-33, eint floored; with
+// the gas energy, _cons2prim_dual_k!, :49-60).
+template <typename T, int NF>
+__device__ __forceinline__ MuW<T> to_prim(const T* __restrict__ U, long long n, long long c, T sr, T g, T gm1,
+                                          T eta1) {
   MuW<T> w;
   const T d = U[c];
   const T u = U[n + c] / d, v = U[2 * n + c] / d, ww = U[3 * n + c] / d;
@@ -177,18 +196,19 @@ __device__ __forceinline__ MuW<T> to_prim(const T* __restrict__ U, long long n, 
   w.vx = u;
   w.vy = v;
   w.vz = ww;
-  w.e = jmax(U[4 * n + c] / d - T(0.5) * (u * u + v * v + ww * ww), sr);
+  if constexpr (NF == 6) w.e = dual_eint(U[4 * n + c], d, U[5 * n + c], u * u + v * v + ww * ww, g, gm1, eta1, sr);
+  else w.e = jmax(U[4 * n + c] / d - T(0.5) * (u * u + v * v + ww * ww), sr);
   return w;
 }
 
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, int NF>
 struct MusLDS {
   static constexpr int WX = TX + 4;
   static constexpr int WS = (TY + 4) * WX;                 // per-field stride of w
   static constexpr int NXS = TY * (TX + 2);
   static constexpr int FS = NXS + (TY + 2) * TX;           // per-field stride of f
   T w[5][TY + 4][TX + 4];
-  T f[5][FS];
+  T f[NF][FS];
   __device__ static int xs(int row, int i) { return row * (TX + 2) + i; }   // x face i of row (0..TX+1)
   __device__ static int ys(int j, int col) { return NXS + j * TX + col; }   // y face j of column (0..TY+1)
 };
@@ -205,9 +225,9 @@ __device__ __forceinline__ MuW<T> get_w(const T* base, int stride, int i) {
   w.vz = base[4 * stride + i];
   return w;
 }
-template <typename T>
-__device__ __forceinline__ void put_f(T* base, int stride, int i, const T F[5]) {
-  for (int q = 0; q < 5; ++q) base[q * stride + i] = F[q];
+template <typename T, int NF>
+__device__ __forceinline__ void put_f(T* base, int stride, int i, const T F[NF]) {
+  for (int q = 0; q < NF; ++q) base[q * stride + i] = F[q];
 }
 
 __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
@@ -220,13 +240,13 @@ __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n
 #define VH_MUSCL_WAVES_F32 4
 #endif
 
-template <typename T, int TX, int TY, bool STAGE2, bool FREC>
+template <typename T, int TX, int TY, bool STAGE2, bool FREC, int NF>
 __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : VH_MUSCL_WAVES) muscl_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
                                                                T* __restrict__ frec, MusclDev G) {
   constexpr int NT = TX * TY;
-  __shared__ MusLDS<T, TX, TY> S;
-  using L_ = MusLDS<T, TX, TY>;
-  const T g = T(G.gamma), gm1 = g - T(1), th = T(G.theta), sr = T(G.small), dtdx = T(G.dtdx);
+  __shared__ MusLDS<T, TX, TY, NF> S;
+  using L_ = MusLDS<T, TX, TY, NF>;
+  const T g = T(G.gamma), gm1 = g - T(1), th = T(G.theta), sr = T(G.small), dtdx = T(G.dtdx), e1 = T(G.eta1);
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int nblk = G.tiles_x * G.tiles_y * G.chunks_z;
@@ -252,16 +272,16 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
 
   // ---- z pipeline prologue: planes kz0-2 .. kz0+2, the face below kz0 ----
   MuW<T> w0, w1, w2, zhi;   // planes k, k+1, k+2 of the column; hi z state of plane k
-  T fzlo[5];
+  T fzlo[NF];
   {
-    const MuW<T> wm2 = to_prim(Us, n, zsrc(kz0 - 2), sr);
-    const MuW<T> wm1 = to_prim(Us, n, zsrc(kz0 - 1), sr);
-    w0 = to_prim(Us, n, zsrc(kz0), sr);
-    w1 = to_prim(Us, n, zsrc(kz0 + 1), sr);
-    w2 = to_prim(Us, n, zsrc(kz0 + 2), sr);
+    const MuW<T> wm2 = to_prim<T, NF>(Us, n, zsrc(kz0 - 2), sr, g, gm1, e1);
+    const MuW<T> wm1 = to_prim<T, NF>(Us, n, zsrc(kz0 - 1), sr, g, gm1, e1);
+    w0 = to_prim<T, NF>(Us, n, zsrc(kz0), sr, g, gm1, e1);
+    w1 = to_prim<T, NF>(Us, n, zsrc(kz0 + 1), sr, g, gm1, e1);
+    w2 = to_prim<T, NF>(Us, n, zsrc(kz0 + 2), sr, g, gm1, e1);
     MuW<T> zlo;
     face_pair(wm1, w0, w1, th, sr, zhi, zlo);
-    if (kz0 >= ng && kz0 <= nz - ng) hll_axis(face_state(wm2, wm1, w0, th, sr), zlo, 2, g, gm1, fzlo);
+    if (kz0 >= ng && kz0 <= nz - ng) hll_axis<T, NF>(face_state(wm2, wm1, w0, th, sr), zlo, 2, g, gm1, fzlo);
   }
 
   // Halo cells of a plane (two deep, cross-shaped): NHX in x, NHY in y.
@@ -292,7 +312,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
       int lx, ly;
       halo_pos(h, lx, ly);
       const long long src = (long long)k * plane + (long long)clampi(y0 + ly, ny) * nx + clampi(x0 + lx, nx);
-      put_w(wf, L_::WS, (ly + 2) * L_::WX + lx + 2, to_prim(Us, n, src, sr));
+      put_w(wf, L_::WS, (ly + 2) * L_::WX + lx + 2, to_prim<T, NF>(Us, n, src, sr, g, gm1, e1));
     }
     __syncthreads();
 
@@ -330,16 +350,16 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
 
     // ---- PB: HLL fluxes (each face once), written over the L state ----
     if (gy < ny && gx >= ng && gx <= nx - ng) {
-      T F[5];
+      T F[NF];
       const int sl = L_::xs(ty, tx);
-      hll_axis(get_w(ff, L_::FS, sl), xlo, 0, g, gm1, F);
-      put_f(ff, L_::FS, sl, F);
+      hll_axis<T, NF>(get_w(ff, L_::FS, sl), xlo, 0, g, gm1, F);
+      put_f<T, NF>(ff, L_::FS, sl, F);
     }
     if (gx < nx && gy >= ng && gy <= ny - ng) {
-      T F[5];
+      T F[NF];
       const int sl = L_::ys(ty, tx);
-      hll_axis(get_w(ff, L_::FS, sl), ylo, 1, g, gm1, F);
-      put_f(ff, L_::FS, sl, F);
+      hll_axis<T, NF>(get_w(ff, L_::FS, sl), ylo, 1, g, gm1, F);
+      put_f<T, NF>(ff, L_::FS, sl, F);
     }
     // tile hi-edge faces: x face TX of each row, y face TY of each column
     for (int h = lt; h < TY + TX; h += NT) {
@@ -351,30 +371,30 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
       if (ok) {
         const int sl = isy ? L_::ys(TY, line) : L_::xs(line, TX);
         const int srr = isy ? L_::ys(TY + 1, line) : L_::xs(line, TX + 1);
-        T F[5];
-        hll_axis(get_w(ff, L_::FS, sl), get_w(ff, L_::FS, srr), isy ? 1 : 0, g, gm1, F);
-        put_f(ff, L_::FS, sl, F);
+        T F[NF];
+        hll_axis<T, NF>(get_w(ff, L_::FS, sl), get_w(ff, L_::FS, srr), isy ? 1 : 0, g, gm1, F);
+        put_f<T, NF>(ff, L_::FS, sl, F);
       }
     }
     // z face k+½ of the lane's column: hi state of plane k, lo state of plane k+1
-    T fzhi[5];
+    T fzhi[NF];
     MuW<T> zhi1, zlo1;
     face_pair(w0, w1, w2, th, sr, zhi1, zlo1);
-    if (k + 1 >= ng && k + 1 <= nz - ng) hll_axis(zhi, zlo1, 2, g, gm1, fzhi);
+    if (k + 1 >= ng && k + 1 <= nz - ng) hll_axis<T, NF>(zhi, zlo1, 2, g, gm1, fzhi);
     __syncthreads();
 
     // ---- PC: divergence in the reference's order, RK combine, flux record ----
     if (inside) {
       const bool ax = gx >= ng && gx < nx - ng, ay = gy >= ng && gy < ny - ng, az = k >= ng && k < nz - ng;
-      T acc[5];
-      for (int q = 0; q < 5; ++q) acc[q] = T(0);
+      T acc[NF];
+      for (int q = 0; q < NF; ++q) acc[q] = T(0);
       if (ax) {
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < NF; ++q) {
           const T lo = ff[q * L_::FS + L_::xs(ty, tx)], hi = ff[q * L_::FS + L_::xs(ty, tx + 1)];
           acc[q] = acc[q] + (-(hi - lo) * dtdx);
         }
       }
-      for (int q = 0; q < 5; ++q) {
+      for (int q = 0; q < NF; ++q) {
         T t = T(0);
         if (ay) {
           const T lo = ff[q * L_::FS + L_::ys(ty, tx)], hi = ff[q * L_::FS + L_::ys(ty + 1, tx)];
@@ -382,30 +402,38 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
         }
         acc[q] = acc[q] + t;
       }
-      for (int q = 0; q < 5; ++q) {
+      for (int q = 0; q < NF; ++q) {
         T t = T(0);
         if (az) t = -(fzhi[q] - fzlo[q]) * dtdx;
         acc[q] = acc[q] + t;
       }
-      for (int q = 0; q < 5; ++q) {
+      T o[NF];
+      for (int q = 0; q < NF; ++q) {
         const T us = Us[q * n + c];
-        T o;
-        if constexpr (STAGE2) o = T(0.5) * (U0[q * n + c] + us + acc[q]);
-        else o = us + acc[q];
-        Uout[q * n + c] = o;
+        if constexpr (STAGE2) o[q] = T(0.5) * (U0[q * n + c] + us + acc[q]);
+        else o[q] = us + acc[q];
       }
+      if constexpr (STAGE2 && NF == 6) {
+        // dual_energy_sync! (lib/PPMKernels/src/muscl_grid.jl:97-107) on the new state
+        const T d = o[0], u = o[1] / d, v = o[2] / d, w = o[3] / d;
+        const T v2 = u * u + v * v + w * w;
+        const T e = dual_eint(o[4], d, o[5], v2, g, gm1, e1, sr);
+        o[5] = d * e;
+        o[4] = d * (e + T(0.5) * v2);
+      }
+      for (int q = 0; q < NF; ++q) Uout[q * n + c] = o[q];
       if constexpr (FREC) {
         // frec[a][q][c]: ½F through the lo face of c along a, where that face is
         // one of the axis' active faces; stage 1 stores 0 + s (every cell), stage 2 adds s.
         const bool fx = gx >= ng && gx <= nx - ng, fy = gy >= ng && gy <= ny - ng, fz = k >= ng && k <= nz - ng;
         const T w = T(0.5);
-        for (int q = 0; q < 5; ++q) {
+        for (int q = 0; q < NF; ++q) {
           const T sxv = fx ? w * ff[q * L_::FS + L_::xs(ty, tx)] : T(0);
           const T syv = fy ? w * ff[q * L_::FS + L_::ys(ty, tx)] : T(0);
           const T szv = fz ? w * fzlo[q] : T(0);
-          T* px = frec + (0 * 5 + q) * n + c;
-          T* py = frec + (1 * 5 + q) * n + c;
-          T* pz = frec + (2 * 5 + q) * n + c;
+          T* px = frec + (0 * NF + q) * n + c;
+          T* py = frec + (1 * NF + q) * n + c;
+          T* pz = frec + (2 * NF + q) * n + c;
           if constexpr (!STAGE2) {
             *px = T(0) + sxv;
             *py = T(0) + syv;
@@ -420,11 +448,11 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
     }
 
     // advance the z pipeline
-    for (int q = 0; q < 5; ++q) fzlo[q] = fzhi[q];
+    for (int q = 0; q < NF; ++q) fzlo[q] = fzhi[q];
     zhi = zhi1;
     w0 = w1;
     w1 = w2;
-    if (k + 1 < kz1) w2 = to_prim(Us, n, zsrc(k + 3), sr);
+    if (k + 1 < kz1) w2 = to_prim<T, NF>(Us, n, zsrc(k + 3), sr, g, gm1, e1);
     __syncthreads();
   }
 }
@@ -446,7 +474,7 @@ __global__ void __launch_bounds__(256) fill_periodic_kernel(T* __restrict__ U, M
     const int sj = j < ng ? j + niy : (j >= ny - ng ? j - niy : j);
     const int sk = k < ng ? k + niz : (k >= nz - ng ? k - niz : k);
     const long long s = ((long long)sk * ny + sj) * nx + si;
-    for (int q = 0; q < 5; ++q) U[q * n + c] = U[q * n + s];
+    for (int q = 0; q < G.nf; ++q) U[q * n + c] = U[q * n + s];
   }
 }
 
@@ -510,7 +538,7 @@ __global__ void __launch_bounds__(256) flux_register_kernel(const T* __restrict_
       p = ng + (int)c + 1;
     }
     const long long cell = p * str[a] + (ng + u1) * str[b1] + (ng + u2) * str[b2];
-    const double v = (double)frec[((long long)a * 5 + comp) * n + cell] * sc;
+    const double v = (double)frec[((long long)a * G.nf + comp) * n + cell] * sc;
     if (t < nplane) planes[t] = v;
     else inner[t - nplane] = v;
   }
@@ -548,17 +576,23 @@ void set_tiles(vh_muscl* m) {
   m->G.chunks_z = (m->G.n[2] + m->G.zchunk - 1) / m->G.zchunk;
 }
 
-template <typename T, int TX, int TY>
-void launch_stage_t(vh_muscl* m, bool stage2, bool rec, const void* Us, const void* U0, void* Uout) {
+template <typename T, int TX, int TY, int NF>
+void launch_stage_nf(vh_muscl* m, bool stage2, bool rec, const void* Us, const void* U0, void* Uout) {
   const unsigned nb = (unsigned)(m->G.tiles_x * m->G.tiles_y * m->G.chunks_z);
   T* fr = (T*)m->frec;
   if (!stage2) {
-    if (rec) muscl_stage_kernel<T, TX, TY, false, true><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, nullptr, (T*)Uout, fr, m->G);
-    else muscl_stage_kernel<T, TX, TY, false, false><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, nullptr, (T*)Uout, fr, m->G);
+    if (rec) muscl_stage_kernel<T, TX, TY, false, true, NF><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, nullptr, (T*)Uout, fr, m->G);
+    else muscl_stage_kernel<T, TX, TY, false, false, NF><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, nullptr, (T*)Uout, fr, m->G);
   } else {
-    if (rec) muscl_stage_kernel<T, TX, TY, true, true><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, m->G);
-    else muscl_stage_kernel<T, TX, TY, true, false><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, m->G);
+    if (rec) muscl_stage_kernel<T, TX, TY, true, true, NF><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, m->G);
+    else muscl_stage_kernel<T, TX, TY, true, false, NF><<<nb, TX * TY, 0, m->stream>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, m->G);
   }
+}
+
+template <typename T, int TX, int TY>
+void launch_stage_t(vh_muscl* m, bool stage2, bool rec, const void* Us, const void* U0, void* Uout) {
+  if (m->G.nf == 6) launch_stage_nf<T, TX, TY, 6>(m, stage2, rec, Us, U0, Uout);
+  else launch_stage_nf<T, TX, TY, 5>(m, stage2, rec, Us, U0, Uout);
 }
 
 void launch_stage(vh_muscl* m, bool stage2, bool rec, const void* Us, const void* U0, void* Uout) {
@@ -604,6 +638,8 @@ int vh_muscl_create(vh_muscl** out, const int32_t* dims, int32_t ng, double gamm
   m->G.gamma = gamma;
   m->G.theta = theta;
   m->G.small = small_rho;
+  m->G.nf = 5;
+  m->G.eta1 = 1e-3;
   m->dx = dx;
   m->dtype = dtype;
   m->esize = dtype == VH_F64 ? 8 : 4;
@@ -644,7 +680,7 @@ int64_t vh_muscl_ncell(const vh_muscl* m) { return m ? m->G.ncell : -1; }
 int vh_muscl_upload(vh_muscl* m, const void* U) {
   if (!m || !U) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
-  VH_HIP(hipMemcpyAsync(m->U, U, (size_t)5 * m->G.ncell * m->esize, hipMemcpyHostToDevice, m->stream));
+  VH_HIP(hipMemcpyAsync(m->U, U, (size_t)m->G.nf * m->G.ncell * m->esize, hipMemcpyHostToDevice, m->stream));
   VH_HIP(hipStreamSynchronize(m->stream));
   return 0;
 }
@@ -652,7 +688,7 @@ int vh_muscl_upload(vh_muscl* m, const void* U) {
 int vh_muscl_download(vh_muscl* m, void* U) {
   if (!m || !U) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
-  VH_HIP(hipMemcpyAsync(U, m->U, (size_t)5 * m->G.ncell * m->esize, hipMemcpyDeviceToHost, m->stream));
+  VH_HIP(hipMemcpyAsync(U, m->U, (size_t)m->G.nf * m->G.ncell * m->esize, hipMemcpyDeviceToHost, m->stream));
   VH_HIP(hipStreamSynchronize(m->stream));
   return 0;
 }
@@ -660,6 +696,31 @@ int vh_muscl_download(vh_muscl* m, void* U) {
 int vh_muscl_state_device_ptr(vh_muscl* m, void** ptr) {
   if (!m || !ptr) return fail(-1, "null argument%s", "");
   *ptr = m->U;
+  return 0;
+}
+
+int vh_muscl_enable_dual_energy(vh_muscl* m, double eta1) {
+  if (!m) return fail(-1, "null argument%s", "");
+  VH_HIP(hipSetDevice(m->device));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  m->G.eta1 = eta1;
+  if (m->G.nf == 6) return 0;
+  const size_t b5 = (size_t)5 * m->G.ncell * m->esize, b6 = (size_t)6 * m->G.ncell * m->esize;
+  void *U = nullptr, *U1 = nullptr;
+  if (hipMalloc(&U, b6) != hipSuccess || hipMalloc(&U1, b6) != hipSuccess) {
+    (void)hipFree(U);
+    return fail(-2, "device allocation of %s%lld bytes failed", "", (long long)(2 * b6));
+  }
+  VH_HIP(hipMemcpyAsync(U, m->U, b5, hipMemcpyDeviceToDevice, m->stream));
+  VH_HIP(hipMemsetAsync((char*)U + b5, 0, b6 - b5, m->stream));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  (void)hipFree(m->U);
+  (void)hipFree(m->U1);
+  (void)hipFree(m->frec);
+  m->U = U;
+  m->U1 = U1;
+  m->frec = nullptr;
+  m->G.nf = 6;
   return 0;
 }
 
@@ -677,7 +738,7 @@ int vh_muscl_set_tiles(vh_muscl* m, int32_t variant, int32_t zchunk) {
 int vh_muscl_step(vh_muscl* m, double dt, int32_t record) {
   if (!m) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
-  if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)15 * m->G.ncell * m->esize));
+  if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)3 * m->G.nf * m->G.ncell * m->esize));
   enqueue_step(m, dt, record != 0);
   VH_HIP(hipGetLastError());
   return 0;
@@ -686,7 +747,8 @@ int vh_muscl_step(vh_muscl* m, double dt, int32_t record) {
 int vh_muscl_fluxrec_download(vh_muscl* m, void* frec) {
   if (!m || !frec) return fail(-1, "null argument%s", "");
   if (!m->frec) return fail(-1, "no step has recorded fluxes yet%s", "");
-  VH_HIP(hipMemcpyAsync(frec, m->frec, (size_t)15 * m->G.ncell * m->esize, hipMemcpyDeviceToHost, m->stream));
+  VH_HIP(hipMemcpyAsync(frec, m->frec, (size_t)3 * m->G.nf * m->G.ncell * m->esize, hipMemcpyDeviceToHost,
+                        m->stream));
   VH_HIP(hipStreamSynchronize(m->stream));
   return 0;
 }
@@ -766,7 +828,7 @@ int vh_muscl_sync(vh_muscl* m) {
 int vh_muscl_time_steps(vh_muscl* m, double dt, int64_t nsteps, int32_t record, double* ms_step, double* ms_stage2) {
   if (!m || nsteps <= 0 || nsteps > 100000) return fail(-1, "bad argument%s (nsteps %lld)", "", nsteps);
   VH_HIP(hipSetDevice(m->device));
-  if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)15 * m->G.ncell * m->esize));
+  if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)3 * m->G.nf * m->G.ncell * m->esize));
   enqueue_step(m, dt, record != 0);   // untimed warm-up
   hipEvent_t e0, e1, s0, s1;
   VH_HIP(hipEventCreate(&e0));

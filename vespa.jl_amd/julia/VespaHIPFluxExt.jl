@@ -252,20 +252,23 @@ mutable struct HIPMuscl
     ptr::Ptr{Cvoid}
     ncell::Int
     eltype::DataType
+    nfields::Int        # 5, or 6 with the gas energy ρe (dual energy)
 end
 
 function HIPMuscl(dims::NTuple{3,Int}, ng::Integer; gamma::Real, theta::Real = 1.5, dx::Real = 1.0,
-                  small_rho::Real = 1e-10, eltype = Float64)
+                  small_rho::Real = 1e-10, eltype = Float64, dual::Bool = false, eta1::Real = 1e-3)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     _check(ccall((:vh_muscl_create, libvespahip), Cint,
                  (Ref{Ptr{Cvoid}}, Ptr{Int32}, Int32, Float64, Float64, Float64, Float64, Int32),
                  h, Int32[dims...], Int32(ng), Float64(gamma), Float64(theta), Float64(small_rho), Float64(dx),
                  eltype === Float64 ? VH_F64 : VH_F32))
-    m = HIPMuscl(h[], prod(dims), eltype)
+    m = HIPMuscl(h[], prod(dims), eltype, dual ? 6 : 5)
     finalizer(m) do x
         x.ptr == C_NULL || ccall((:vh_muscl_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
         x.ptr = C_NULL
     end
+    dual && _check(ccall((:vh_muscl_enable_dual_energy, libvespahip), Cint, (Ptr{Cvoid}, Float64), m.ptr,
+                         Float64(eta1)))
     return m
 end
 
@@ -273,7 +276,7 @@ hip_upload!(m::HIPMuscl, U::Vector) = (GC.@preserve U _check(ccall((:vh_muscl_up
     (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U)); m)
 
 function hip_download(m::HIPMuscl)
-    U = Vector{m.eltype}(undef, 5 * m.ncell)
+    U = Vector{m.eltype}(undef, m.nfields * m.ncell)
     GC.@preserve U _check(ccall((:vh_muscl_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, U))
     return U
 end

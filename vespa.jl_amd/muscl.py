@@ -6,7 +6,8 @@ A :class:`MusclGrid` holds the conserved state (D, S1, S2, S3, τ = ρ·etot) of
 ghosted 3D grid on the device: ``dims`` counts cells per axis including ``ng``
 ghost zones on each side, x fastest. ``step(dt)`` is ``muscl_step_3d!`` without a
 ``bc!`` callback (ghost zones are left as they are; call ``fill_periodic()``
-between steps for a periodic box) and without the dual-energy field.
+between steps for a periodic box). With ``dual=True`` the state carries the gas
+energy ρe as a sixth field (the ``ge``/``eta1`` dual-energy formalism).
 ``muscl_step_3d`` is the array-in/array-out form of the reference call.
 """
 from __future__ import annotations
@@ -21,7 +22,7 @@ from ._lib import check, load_library
 
 class MusclGrid:
     def __init__(self, dims, ng: int, gamma: float, theta: float = 1.5, dx: float = 1.0, small_rho: float = 1e-10,
-                 eltype=np.float64):
+                 eltype=np.float64, dual: bool = False, eta1: float = 1e-3):
         self.dims = tuple(int(d) for d in dims)
         if len(self.dims) != 3:
             raise ValueError("muscl_step_3d! works on 3D grids")
@@ -37,6 +38,10 @@ class MusclGrid:
                                         _lib.VH_F64 if self.eltype == np.float64 else _lib.VH_F32))
         self._h = h
         self.ncell = int(np.prod(self.dims))
+        self.nf = 5
+        if dual:
+            check(self._lib.vh_muscl_enable_dual_energy(self._h, float(eta1)))
+            self.nf = 6
 
     def close(self):
         if getattr(self, "_h", None):
@@ -51,12 +56,12 @@ class MusclGrid:
 
     def upload(self, U: np.ndarray):
         U = np.ascontiguousarray(U, dtype=self.eltype)
-        if U.size != 5 * self.ncell:
-            raise ValueError(f"state must hold 5*{self.ncell} values")
+        if U.size != self.nf * self.ncell:
+            raise ValueError(f"state must hold {self.nf}*{self.ncell} values")
         check(self._lib.vh_muscl_upload(self._h, U.ctypes.data_as(C.c_void_p)))
 
     def state(self) -> np.ndarray:
-        out = np.empty((5, self.ncell), dtype=self.eltype)
+        out = np.empty((self.nf, self.ncell), dtype=self.eltype)
         check(self._lib.vh_muscl_download(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
@@ -69,9 +74,9 @@ class MusclGrid:
         check(self._lib.vh_muscl_step(self._h, float(dt), int(bool(fluxrec))))
 
     def fluxrec(self) -> np.ndarray:
-        """Fluxes recorded by the last recording step: (3, 5, ncell), [axis][field][cell],
+        """Fluxes recorded by the last recording step: (3, nf, ncell), [axis][field][cell],
         ½(F₁+F₂) through the lo face of each cell (grid-order momenta)."""
-        out = np.empty((3, 5, self.ncell), dtype=self.eltype)
+        out = np.empty((3, self.nf, self.ncell), dtype=self.eltype)
         check(self._lib.vh_muscl_fluxrec_download(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
 
@@ -180,12 +185,13 @@ def total_field(f, dims, ng, dx):
 
 
 def muscl_step_3d(D, S1, S2, S3, Tau, dims, ng, *, dt, gamma, theta=1.5, dx=1.0, small_rho=1e-10,
-                  fluxrec=None):
-    """muscl_step_3d!(D, S1, S2, S3, Tau, dims, ng; dt, gamma, theta, dx, small_rho,
-    fluxrec) on host arrays, updated in place through a device context. fluxrec: None
-    or a (3, 5, N) array filled with the recorded fluxes."""
-    fields = (D, S1, S2, S3, Tau)
-    g = MusclGrid(dims, ng, gamma, theta, dx, small_rho, eltype=np.asarray(D).dtype)
+                  ge=None, eta1=1e-3, fluxrec=None):
+    """muscl_step_3d!(D, S1, S2, S3, Tau, dims, ng; dt, gamma, theta, dx, small_rho, ge,
+    eta1, fluxrec) on host arrays, updated in place through a device context. fluxrec:
+    None or a (3, nf, N) array filled with the recorded fluxes (nf = 6 with ge)."""
+    fields = (D, S1, S2, S3, Tau) + ((ge,) if ge is not None else ())
+    g = MusclGrid(dims, ng, gamma, theta, dx, small_rho, eltype=np.asarray(D).dtype, dual=ge is not None,
+                  eta1=eta1)
     try:
         g.upload(np.stack([np.ravel(f) for f in fields]))
         g.step(dt, fluxrec is not None)
