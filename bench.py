@@ -317,7 +317,8 @@ def muscl_turbulence_state(n, ng=4, nmodes=24, seed=1234, mach=0.3, gamma=1.4):
 def muscl_leg(V, args):
     """PPMKernels unsplit RK2 MUSCL (PLM minmod-θ + HLL) on the reference's turbulence
     benchmark: 256³ active cells + 4 ghost zones, γ = 1.4, θ = 1.5, dx = 1/256,
-    dt = 0.2·dx, Float32 and Float64 (fused stage kernels, exact arithmetic).
+    dt = 0.2·dx, Float32 and Float64, fused stage kernels in the exact (bit-identical)
+    and the fast arithmetic profile.
     Mcell/s counts active cells per second, as the reference's table does; the
     CPU baseline is the oracle (one thread) on a 32³ + ghost sample."""
     import numpy as np
@@ -331,18 +332,21 @@ def muscl_leg(V, args):
            "published_reference": {"value_Mcell_s": 96.51, "hardware": "Apple Silicon, Metal, Float32",
                                    "source": "lib/PPMKernels/bench/RESULTS_turbulence.md:24"}}
     for dtype, key in ((np.float32, "f32"), (np.float64, "f64")):
-        g = M.MusclGrid(dims, ng, 1.4, 1.5, dx, eltype=dtype)
-        g.upload(U64.astype(dtype))
-        r = g.time_steps(dt, args.muscl_steps)
-        esz = np.dtype(dtype).itemsize
-        out[key] = {"Mcell_s": n ** 3 / (r["step_ms"] * 1e-3) / 1e6, "ms_per_step": r["step_ms"],
-                    "stage2_ms": r["stage2_ms"],
-                    "stage2_GBps": 15 * esz * ncell / (r["stage2_ms"] * 1e-3) / 1e9,
-                    "speedup_vs_published": n ** 3 / (r["step_ms"] * 1e-3) / 1e6 / 96.51,
-                    "finite": bool(np.isfinite(g.state()).all())}
-        g.close()
-        log(f"[bench] muscl {key}: {out[key]['Mcell_s']:.1f} Mcell/s ({r['step_ms']:.3f} ms/step, stage 2 "
-            f"{out[key]['stage2_GBps']:.0f} GB/s)")
+        for mode in ("exact", "fast"):
+            g = M.MusclGrid(dims, ng, 1.4, 1.5, dx, eltype=dtype)
+            g.set_mode(mode)
+            g.upload(U64.astype(dtype))
+            r = g.time_steps(dt, args.muscl_steps)
+            esz = np.dtype(dtype).itemsize
+            k = key if mode == "exact" else key + "_fast"
+            out[k] = {"Mcell_s": n ** 3 / (r["step_ms"] * 1e-3) / 1e6, "ms_per_step": r["step_ms"],
+                      "stage2_ms": r["stage2_ms"],
+                      "stage2_GBps": 15 * esz * ncell / (r["stage2_ms"] * 1e-3) / 1e9,
+                      "speedup_vs_published": n ** 3 / (r["step_ms"] * 1e-3) / 1e6 / 96.51,
+                      "finite": bool(np.isfinite(g.state()).all())}
+            g.close()
+            log(f"[bench] muscl {k}: {out[k]['Mcell_s']:.1f} Mcell/s ({r['step_ms']:.3f} ms/step, stage 2 "
+                f"{out[k]['stage2_GBps']:.0f} GB/s)")
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         m = 32

@@ -360,6 +360,11 @@ int vh_muscl_enable_dual_energy(vh_muscl* m, double eta1);
 int vh_muscl_upload(vh_muscl* m, const void* U);
 int vh_muscl_download(vh_muscl* m, void* U);
 int vh_muscl_state_device_ptr(vh_muscl* m, void** ptr);
+/* Arithmetic profile of the stage kernels: VH_EXACT (default; the reference's operations
+ * and order, correctly rounded divisions and square roots: bit-identical) or VH_FAST
+ * (contracted a*b+c, one refined reciprocal per cell and per face, refined rsqrt sound
+ * speeds: within a few ulps per face). */
+int vh_muscl_set_mode(vh_muscl* m, int32_t mode);
 /* Tiles: variant 0 = 16x16 cells, 1 = 32x8; zchunk = z planes per workgroup. */
 int vh_muscl_set_tiles(vh_muscl* m, int32_t variant, int32_t zchunk);
 /* One muscl_step_3d!(…; dt): two launches, stream-ordered. record != 0 is

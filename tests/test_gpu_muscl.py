@@ -223,3 +223,71 @@ def test_dual_energy_array_form(O):
     fo = O.muscl_step_3d(U, dims, ng, 0.01, G14, 1.5, 0.1, record=True)
     np.testing.assert_array_equal(np.stack(fields), U)
     np.testing.assert_array_equal(fr, fo)
+
+
+# Fast profile: per field, max |fast − oracle| / max |oracle| after three steps.
+FAST_TOL = {np.float64: 1e-12, np.float32: 1e-5}
+
+
+def _rel_err(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return (np.abs(a - b).max(axis=-1) / np.maximum(np.abs(b).max(axis=-1), 1e-300)).max()
+
+
+@pytest.mark.parametrize("dual", [False, True])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fast_profile_within_tolerance(O, dtype, dual):
+    """The fast arithmetic profile (contracted a*b+c, refined reciprocals, rsqrt sound
+    speeds) against the oracle: states and recorded fluxes within FAST_TOL after
+    three steps, and not bit-identical to them (the fast kernels ran)."""
+    print(f"muscl fast {np.dtype(dtype).name} dual={dual}", flush=True)
+    dims, ng = (26, 23, 19), 3
+    U = smooth_state(dims, seed=77, dtype=dtype, mach=0.8)
+    if dual:
+        U = with_ge(U)
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10, eltype=dtype, dual=dual, eta1=1e-3)
+    try:
+        with pytest.raises(ValueError):
+            g.set_mode("approximate")
+        g.set_mode("fast")
+        g.upload(U)
+        for s in range(3):
+            rec = s == 2
+            frec_o = O.muscl_step_3d(U, dims, ng, 0.004, G14, 1.5, 0.05, 1e-10, record=rec)
+            g.step(0.004, fluxrec=rec)
+        got = g.state()
+        assert np.isfinite(got).all()
+        assert _rel_err(got, U) <= FAST_TOL[dtype]
+        assert not np.array_equal(got, U)
+        fr = g.fluxrec()
+        assert _rel_err(fr.reshape(3 * g.nf, -1), frec_o.reshape(3 * g.nf, -1)) <= FAST_TOL[dtype]
+        g.set_mode("exact")                      # back to the bit-identical kernels
+        g.upload(U)
+        frec_o = O.muscl_step_3d(U, dims, ng, 0.004, G14, 1.5, 0.05, 1e-10, record=True)
+        g.step(0.004, fluxrec=True)
+        np.testing.assert_array_equal(g.state(), U)
+        np.testing.assert_array_equal(g.fluxrec(), frec_o)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_fast_profile_benchmark_workload(dtype):
+    """256³ active + 4 ghost zones: the fast profile within FAST_TOL of the exact
+    profile (bit-identical to the oracle at this size) after two steps."""
+    print(f"muscl 264^3 fast vs exact {np.dtype(dtype).name}", flush=True)
+    dims, ng = (264, 264, 264), 4
+    U = smooth_state(dims, seed=99, dtype=dtype, mach=0.3)
+    res = {}
+    for mode in ("exact", "fast"):
+        g = M.MusclGrid(dims, ng, G14, 1.5, 1.0 / 256, eltype=dtype)
+        try:
+            g.set_mode(mode)
+            g.upload(U)
+            for _ in range(2):
+                g.step(0.2 / 256)
+            res[mode] = g.state()
+        finally:
+            g.close()
+    assert np.isfinite(res["fast"]).all()
+    assert _rel_err(res["fast"], res["exact"]) <= FAST_TOL[dtype]

@@ -39,7 +39,7 @@ EXPORTED = (
     "vh_muscl_create", "vh_muscl_destroy", "vh_muscl_ncell", "vh_muscl_upload", "vh_muscl_download",
     "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
     "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps",
-    "vh_muscl_flux_register", "vh_muscl_enable_dual_energy",
+    "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode",
 )
 
 
@@ -169,10 +169,11 @@ def load_library(path: str | None = None):
     L.vh_muscl_time_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, dp, dp]
     L.vh_muscl_flux_register.argtypes = [vp, C.c_double, dp, dp]
     L.vh_muscl_enable_dual_energy.argtypes = [vp, C.c_double]
+    L.vh_muscl_set_mode.argtypes = [vp, C.c_int32]
     for n in ("vh_muscl_create", "vh_muscl_destroy", "vh_muscl_upload", "vh_muscl_download",
               "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
               "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps",
-              "vh_muscl_flux_register", "vh_muscl_enable_dual_energy"):
+              "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode"):
         getattr(L, n).restype = C.c_int
     for n in ("vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
               "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel"):

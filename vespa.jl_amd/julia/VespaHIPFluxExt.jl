@@ -286,6 +286,10 @@ hip_muscl_step!(m::HIPMuscl, dt::Real; record::Bool = false) =
     (_check(ccall((:vh_muscl_step, libvespahip), Cint, (Ptr{Cvoid}, Float64, Int32), m.ptr, Float64(dt),
                   Int32(record))); m)
 
+"Exact (bit-identical, default) or fast (contracted, refined reciprocals) stage kernels."
+hip_set_mode!(m::HIPMuscl, fast::Bool) =
+    (_check(ccall((:vh_muscl_set_mode, libvespahip), Cint, (Ptr{Cvoid}, Int32), m.ptr, fast ? VH_FAST : VH_EXACT)); m)
+
 hip_fill_periodic!(m::HIPMuscl) = (_check(ccall((:vh_muscl_fill_periodic, libvespahip), Cint, (Ptr{Cvoid},),
                                                 m.ptr)); m)
 

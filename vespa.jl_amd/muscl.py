@@ -39,6 +39,7 @@ class MusclGrid:
         self._h = h
         self.ncell = int(np.prod(self.dims))
         self.nf = 5
+        self.mode = "exact"
         if dual:
             check(self._lib.vh_muscl_enable_dual_energy(self._h, float(eta1)))
             self.nf = 6
@@ -64,6 +65,15 @@ class MusclGrid:
         out = np.empty((self.nf, self.ncell), dtype=self.eltype)
         check(self._lib.vh_muscl_download(self._h, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def set_mode(self, mode: str = "exact"):
+        """Arithmetic profile of the stage kernels: "exact" (the reference's operations and
+        order, bit-identical) or "fast" (contracted a*b+c, refined reciprocals and rsqrt
+        sound speeds; within a few ulps per face)."""
+        if mode not in ("exact", "fast"):
+            raise ValueError(f"mode must be 'exact' or 'fast', got {mode!r}")
+        check(self._lib.vh_muscl_set_mode(self._h, _lib.VH_EXACT if mode == "exact" else _lib.VH_FAST))
+        self.mode = mode
 
     def set_tiles(self, variant: int = 0, zchunk: int = 8):
         """Tile shape: 0 = 16x16 cells, 1 = 32x8; zchunk z planes per workgroup."""
