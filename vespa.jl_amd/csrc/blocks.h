@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
       if (active) {
         const Cons<T> uown = load_cons(Us, n, c);
         Cons<T> u0own;
-        if constexpr (STAGE2) u0own = load_cons(U0, n, c);
+        if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
         const int ix = ty * (TX + 2) + tx;
         const int iy = NXS + ty * TX + tx;
         auto flux = [&](int s) -> Cons<T> {
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
           o.e = hh * u0own.e + hh * (uown.e - invV * acc.e);
           cfl.add(cons2prim<T, FLOOR64>(o, g, B.gamma), g);
         }
-        store_cons(Uout, n, c, o);
+        store_cons_stream(Uout, n, c, o);
       }
 
       if constexpr (RANK == 3) {

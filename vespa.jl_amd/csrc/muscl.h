@@ -427,6 +427,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
         o[5] = d * e;
         o[4] = d * (e + T(0.5) * v2);
       }
+      // plain stores: non-temporal ones measured slower for this kernel in Float32
       for (int q = 0; q < NF; ++q) Uout[q * n + c] = o[q];
       if constexpr (FREC) {
         // frec[a][q][c]: ½F through the lo face of c along a, where that face is

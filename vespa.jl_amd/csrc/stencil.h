@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
 #if VH_EARLY_OWN
     if (active) {
       uown = load_cons(Us, n, c);
-      if constexpr (STAGE2) u0own = load_cons(U0, n, c);
+      if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
     }
 #endif
 
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     if (active) {
 #if !VH_EARLY_OWN
       uown = load_cons(Us, n, c);
-      if constexpr (STAGE2) u0own = load_cons(U0, n, c);
+      if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
 #endif
       Cons<T> fxlo, fxhi, fylo, fyhi;
       fxlo = lds_getf(ff, L_::FS, L_::xs(ty, tx));
@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
         o.e = h * u0own.e + h * (uown.e - invV * acc.e);
         cfl.add(cons2prim<T, FLOOR64>(o, g, G.gamma), g);
       }
-      store_cons(Uout, n, c, o);
+      store_cons_stream(Uout, n, c, o);
     }
 
     if constexpr (RANK == 3) {
