@@ -36,6 +36,66 @@ namespace vh {
 
 enum { NB_SAME = 0, NB_COARSER = 1, NB_FINER = 2, NB_BOUNDARY = 3 };
 
+// Map from non-negative 64-bit keys (cell indices, cell·6 + slot) to
+// non-negative ints, open addressing with linear probing: the table build does
+// a few lookups per cell on every linked block face, where node-based maps
+// spend most of the build's time.
+class FlatMap {
+ public:
+  void reserve(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n) cap <<= 1;
+    keys_.assign(cap, -1);
+    vals_.assign(cap, 0);
+    mask_ = cap - 1;
+    n_ = 0;
+  }
+  int find(long long k) const {
+    for (size_t i = slot(k);; i = (i + 1) & mask_) {
+      if (keys_[i] == k) return vals_[i];
+      if (keys_[i] < 0) return -1;
+    }
+  }
+  // Sets key k to v; returns false (and leaves the value) when k is present and keep is set.
+  bool put(long long k, int v, bool keep = false) {
+    if (2 * (n_ + 1) > keys_.size()) grow();
+    for (size_t i = slot(k);; i = (i + 1) & mask_) {
+      if (keys_[i] == k) {
+        if (keep) return false;
+        vals_[i] = v;
+        return true;
+      }
+      if (keys_[i] < 0) {
+        keys_[i] = k;
+        vals_[i] = v;
+        ++n_;
+        return true;
+      }
+    }
+  }
+  template <class F>
+  void for_each(F f) const {
+    for (size_t i = 0; i < keys_.size(); ++i)
+      if (keys_[i] >= 0) f(keys_[i], vals_[i]);
+  }
+  size_t size() const { return n_; }
+
+ private:
+  size_t slot(long long k) const { return (size_t)(((unsigned long long)k * 0x9E3779B97F4A7C15ULL) >> 20) & mask_; }
+  void grow() {
+    std::vector<long long> k0;
+    std::vector<int> v0;
+    k0.swap(keys_);
+    v0.swap(vals_);
+    reserve(k0.size());
+    for (size_t i = 0; i < k0.size(); ++i)
+      if (k0[i] >= 0) put(k0[i], v0[i]);
+  }
+  std::vector<long long> keys_;
+  std::vector<int> vals_;
+  size_t mask_ = 0, n_ = 0;
+};
+
 // Tile limits of block_stage_kernel (BNT lanes, LDS sized for these bounds).
 static const int kLanes = 256, kMaxW = 432, kMaxFS = 592, kMaxE = 32;
 
@@ -222,15 +282,13 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
 
   // ---- blocks, representative maps, link faces ----
   H.blocks.assign(nb, BlockDev{});
-  std::unordered_map<long long, int> facemap;   // emitting cell * 6 + slot -> face
-  {
-    long long cells_on_linked_planes = 0;
-    for (int b = 0; b < nb; ++b)
-      for (int d = 0; d < rank; ++d)
-        for (int side = 0; side < 2; ++side)
-          if (!T.cand[6 * b + 2 * d + side].empty()) cells_on_linked_planes += plane_size(&T.dims[3 * b], d);
-    facemap.reserve((size_t)cells_on_linked_planes + 16);
-  }
+  FlatMap facemap;   // emitting cell * 6 + slot -> face
+  long long cells_on_linked_planes = 0;
+  for (int b = 0; b < nb; ++b)
+    for (int d = 0; d < rank; ++d)
+      for (int side = 0; side < 2; ++side)
+        if (!T.cand[6 * b + 2 * d + side].empty()) cells_on_linked_planes += plane_size(&T.dims[3 * b], d);
+  facemap.reserve((size_t)cells_on_linked_planes + 16);
   long long fine[4];
   int nfine = 0;
   for (int b = 0; b < nb; ++b) {
@@ -276,7 +334,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             f.iR = (int)(side ? rep : g);
             f.axis = d;
             f.area = T.area(b, d);
-            facemap[g * 6 + slot] = (int)H.faces.size();
+            facemap.put(g * 6 + slot, (int)H.faces.size());
             H.faces.push_back(f);
           }
       }
@@ -301,17 +359,28 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   // ---- link-cell records ----
   // A slot lists at most 2^(rank-1) <= 4 faces: [0] count, [1..4] faces.
   using SlotList = std::array<int, 5>;
-  std::unordered_map<long long, int> recof;   // cell -> record
+  FlatMap recof;   // cell -> record
   recof.reserve(2 * H.faces.size() + 16);
   std::vector<int> recblock;
+  std::vector<long long> reccell;             // record -> cell
   std::vector<int> recgrpblock;               // record*6 + slot -> emitting block of a type-2 slot
   std::vector<LinkCellDev> recs;
   std::vector<SlotList> slotlists;            // record*6 + slot
+  {
+    // at most two records per link face, and one per cell on a linked face plane
+    const size_t nrec = std::min((size_t)cells_on_linked_planes, 2 * H.faces.size());
+    recs.reserve(nrec);
+    recblock.reserve(nrec);
+    reccell.reserve(nrec);
+    recgrpblock.reserve(6 * nrec);
+    slotlists.reserve(6 * nrec);
+  }
   auto record_for = [&](long long g, int b) -> int {
-    auto it = recof.find(g);
-    if (it != recof.end()) return it->second;
+    const int found = recof.find(g);
+    if (found >= 0) return found;
     const int r = (int)recs.size();
-    recof.emplace(g, r);
+    recof.put(g, r);
+    reccell.push_back(g);
     LinkCellDev R{};
     recs.push_back(R);
     recblock.push_back(b);
@@ -342,8 +411,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             int type = 0, emitter_block = -1;
             SlotList fl{0, -1, -1, -1, -1};
             auto face_of = [&](long long cell, int sl) -> int {
-              auto it = facemap.find(cell * 6 + sl);
-              return it == facemap.end() ? -1 : it->second;
+              return facemap.find(cell * 6 + sl);
             };
             if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
               type = 1;
@@ -373,21 +441,23 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
       }
   }
   H.linkcells.resize(recs.size());
+  H.slot_faces.reserve(2 * H.faces.size());   // each link face is listed by both of its cells
   for (size_t r = 0; r < recs.size(); ++r) {
     LinkCellDev R = recs[r];
-    std::vector<std::pair<int, int>> grp;   // (emitting block, slot)
+    std::pair<int, int> grp[6];   // (emitting block, slot), at most one per slot
+    int ng = 0;
     for (int s = 0; s < 6; ++s) {
       const SlotList& sl = slotlists[6 * r + s];
       R.start[s] = (int)H.slot_faces.size();
       R.count[s] = sl[0];
       for (int i = 1; i <= sl[0]; ++i) H.slot_faces.push_back(sl[i]);
-      if (R.type[s] == 2) grp.push_back({recgrpblock[6 * r + s], s});
+      if (R.type[s] == 2) grp[ng++] = {recgrpblock[6 * r + s], s};
     }
-    std::sort(grp.begin(), grp.end());
-    R.ngrp = (int)grp.size();
+    std::sort(grp, grp + ng);
+    R.ngrp = ng;
     R.npre = 0;
     for (int q = 0; q < 6; ++q) R.grp[q] = -1;
-    for (size_t q = 0; q < grp.size(); ++q) {
+    for (int q = 0; q < ng; ++q) {
       if (q > 0 && grp[q].first == grp[q - 1].first)
         return fail(-1, "a cell touches block %s%lld through two faces", "", grp[q].first);
       R.grp[q] = grp[q].second;
@@ -423,9 +493,9 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
             ijk[d] = side ? bk.n[d] - 1 : 0;
             ijk[e1] = u;
             ijk[e2] = v;
-            auto it = recof.find(T.index(b, ijk));
-            if (it == recof.end()) continue;
-            pm[plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] = it->second;
+            const int r = recof.find(T.index(b, ijk));
+            if (r < 0) continue;
+            pm[plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] = r;
             any = true;
           }
         if (!any) continue;
@@ -437,16 +507,16 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   }
   // a record may belong to a cell that is on a linked face only through
   // another slot (edge/corner cells): make sure every record is reachable
-  for (auto& kv : recof) {
+  for (int rec = 0; rec < (int)reccell.size(); ++rec) {
     long long ijk[3];
-    const int b = T.locate(kv.first, ijk);
+    const int b = T.locate(reccell[rec], ijk);
     const BlockDev& bk = H.blocks[b];
     bool found = false;
     for (int d = 0; d < rank && !found; ++d)
       for (int side = 0; side < 2 && !found; ++side) {
         const int slot = 2 * d + side;
         if (ijk[d] != (side ? bk.n[d] - 1 : 0) || bk.lcoff[slot] < 0) continue;
-        found = H.lcmap[bk.lcoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] == kv.second;
+        found = H.lcmap[bk.lcoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] == rec;
       }
     if (!found) return fail(-1, "unreachable link-cell record in block %s%lld", "", b);
   }
@@ -454,7 +524,8 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   // ---- flux-register lists (coarse cells with finer neighbours) ----
   H.regs.assign(T.maxlevel + 1, RegList{});
   {
-    std::vector<std::pair<long long, int>> order(recof.begin(), recof.end());
+    std::vector<std::pair<long long, int>> order(reccell.size());
+    for (size_t r = 0; r < reccell.size(); ++r) order[r] = {reccell[r], (int)r};
     std::sort(order.begin(), order.end());
     for (auto& kv : order) {
       const LinkCellDev& R = H.linkcells[kv.second];
