@@ -201,7 +201,7 @@ static void ad_face_value_vjp(const vo_grid* m, const double* W, int64_t c, int 
     int64_t n = vo_ncell(m);
     double Wc[5], Wl[5], Wr[5];
     for (int f = 0; f < 5; ++f) Wc[f] = W[f * n + c];
-    int64_t jl, jr;
+    int64_t jl = -1, jr = -1;
     int hl = vo_neighbor(m, c, axis, 0, &jl), hr = vo_neighbor(m, c, axis, 1, &jr);
     if (hl) for (int f = 0; f < 5; ++f) Wl[f] = W[f * n + jl];
     else ghost_f64(Wc, m->bc[2 * axis], axis, Wl);
