@@ -51,6 +51,7 @@ class FlatMap {
     n_ = 0;
   }
   int find(long long k) const {
+    if (keys_.empty()) return -1;
     for (size_t i = slot(k);; i = (i + 1) & mask_) {
       if (keys_[i] == k) return vals_[i];
       if (keys_[i] < 0) return -1;
