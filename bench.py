@@ -80,9 +80,16 @@ def cpu_baseline(U, n, steps=2):
         dt = O.compute_dt_omp(g, Uc, 0.3, threads=threads)
         O.step_omp(g, Uc, dt, threads=threads, work=work)
     el = time.perf_counter() - t0
+    # the same oracle on one thread (RefMesh's serial loops), one step
+    t0 = time.perf_counter()
+    dt = O.compute_dt_omp(g, Uc, 0.3, threads=1)
+    O.step_omp(g, Uc, dt, threads=1, work=work)
+    el1 = time.perf_counter() - t0
     return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} SSP-RK2 steps (CFL dt each) of the same {n}^3 FP64 workload, oracle in "
-                      f"gather form with OpenMP on {threads} host threads, {el:.2f} s wall"}
+                      f"gather form with OpenMP on {threads} host threads, {el:.2f} s wall",
+            "serial": {"value": n ** 3 / el1, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+                       "sample": f"1 further SSP-RK2 step of the same workload on one thread, {el1:.2f} s wall"}}
 
 
 def blocks_leg(V, args):
