@@ -11,7 +11,11 @@
 #     exact (bit-identical) or fast arithmetic profile;
 #   * `HIPBlocks`: block-structured AMR meshes (hanging-node faces, flux
 #     registers, subcycling) advanced by one persistent launch per RK stage;
-#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls);
+#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls),
+#     and `HIPRollout` (`hip_forward!` / `hip_backward`): a checkpointed rollout;
+#   * `hip_step_bflux!`: step! with the boundary-flux register (src/reflux.jl);
+#   * `HIPMuscl`: PPMKernels' unsplit RK2 MUSCL step with flux recording
+#     (`hip_fluxrec`, `hip_flux_register`);
 #   * `hip_enable_gravity!` / `hip_solve_poisson!`: self-gravity of a HIPGrid
 #     (enable_gravity! / solve_poisson!, src/gravity.jl:63-80, 248-322): the
 #     device CG Poisson solve, the gravity source in every RK stage and the
@@ -299,6 +303,63 @@ function hip_max_wavespeed(m::HIPMuscl)
     return s[]
 end
 
+# ------------------------------------------------------------------ boundary-flux register
+
+"""
+    hip_step_bflux!(g::HIPGrid, dt, reg::Vector) -> reg
+
+step!(sim, dt; bflux) (src/driver.jl:333-359 with src/reflux.jl:105-132): one step that
+also adds ½dt·F·area of every domain-boundary face, per RK stage, to `reg` (5 values of
+the state type per slot, `hip_bflux_count(g)` slots; planes axis 1 lo, axis 1 hi, axis 2
+lo, …, each column-major over the other axes). The register accumulates, as the
+reference's does; zero it to start a new interval.
+"""
+function hip_step_bflux!(g::HIPGrid, dt::Real, reg::Vector)
+    nb = sizeof(eltype(reg)) * length(reg)
+    length(reg) == 5 * hip_bflux_count(g) || error("register must hold 5 × hip_bflux_count(g) values")
+    d = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:vh_buffer_alloc, libvespahip), Cint, (Ref{Ptr{Cvoid}}, Int64), d, nb))
+    try
+        GC.@preserve reg begin
+            _check(ccall((:vh_buffer_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), d[], reg, nb))
+            _check(ccall((:vh_grid_step_bflux, libvespahip), Cint, (Ptr{Cvoid}, Float64, Ptr{Cvoid}), g.ptr,
+                         Float64(dt), d[]))
+            _check(ccall((:vh_buffer_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), reg, d[], nb))
+        end
+    finally
+        ccall((:vh_buffer_free, libvespahip), Cint, (Ptr{Cvoid},), d[])
+    end
+    return reg
+end
+
+hip_bflux_count(g::HIPGrid) = Int(ccall((:vh_grid_bflux_count, libvespahip), Int64, (Ptr{Cvoid},), g.ptr))
+
+# ------------------------------------------------------------------ MUSCL flux recording
+
+"The fluxes recorded by the last `hip_muscl_step!(…; record = true)`: 3 × nfields × ncell, [axis][field][cell]."
+function hip_fluxrec(m::HIPMuscl)
+    out = Vector{m.eltype}(undef, 3 * m.nfields * m.ncell)
+    GC.@preserve out _check(ccall((:vh_muscl_fluxrec_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, out))
+    return out
+end
+
+"""
+    hip_flux_register(m::HIPMuscl, dt, act) -> (planes, interior)
+
+boundary_flux_register (lib/PPMKernels/src/reflux_capture.jl:86-131) of the last recording
+step as Float64 ∫F·area·dt arrays, axis by axis (layout in include/vespahip.h); act = the
+active cells per axis (dims .- 2ng).
+"""
+function hip_flux_register(m::HIPMuscl, dt::Real, act::NTuple{3,Int})
+    np = sum(10 * prod(act[[b for b in 1:3 if b != a]]) for a in 1:3)
+    ni = sum(5 * (act[a] - 1) * prod(act[[b for b in 1:3 if b != a]]) for a in 1:3)
+    planes = Vector{Float64}(undef, np)
+    interior = Vector{Float64}(undef, max(ni, 1))
+    GC.@preserve planes interior _check(ccall((:vh_muscl_flux_register, libvespahip), Cint,
+        (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}), m.ptr, Float64(dt), planes, interior))
+    return planes, interior
+end
+
 # ------------------------------------------------------------------ reverse mode
 
 """
@@ -366,6 +427,58 @@ function hip_phi(g::HIPGrid)
     return out
 end
 
+"""
+    HIPRollout(g::HIPGrid, nsteps; every = 20)
+
+Checkpointed rollout of `nsteps` steps with the device CFL dt: `hip_forward!` keeps the
+state every `every` steps and each step's dt, `hip_backward` recomputes each segment and
+applies the step VJPs in reverse (the pullback of the whole rollout).
+"""
+mutable struct HIPRollout
+    ptr::Ptr{Cvoid}
+    grid::HIPGrid
+    nsteps::Int
+end
+
+function HIPRollout(g::HIPGrid, nsteps::Integer; every::Integer = 20)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    _check(ccall((:vh_adjoint_create, libvespahip), Cint, (Ref{Ptr{Cvoid}}, Ptr{Cvoid}, Int64, Int32),
+                 h, g.ptr, Int64(nsteps), Int32(every)))
+    r = HIPRollout(h[], g, Int(nsteps))
+    finalizer(r) do x
+        x.ptr == C_NULL || ccall((:vh_adjoint_destroy, libvespahip), Cint, (Ptr{Cvoid},), x.ptr)
+        x.ptr = C_NULL
+    end
+    return r
+end
+
+"Run the rollout forward from the grid's current state (left at the final state)."
+hip_forward!(r::HIPRollout, cfl::Real) =
+    (_check(ccall((:vh_adjoint_forward, libvespahip), Cint, (Ptr{Cvoid}, Float64), r.ptr, Float64(cfl))); r)
+
+"∂L/∂U₀ for a loss whose gradient with respect to the final state is W̄ (host vectors, 5 × ncell Float64)."
+function hip_backward(r::HIPRollout, W̄::Vector{Float64})
+    nb = 8 * length(W̄)
+    dW = Ref{Ptr{Cvoid}}(C_NULL); dU = Ref{Ptr{Cvoid}}(C_NULL)
+    for d in (dW, dU)
+        _check(ccall((:vh_buffer_alloc, libvespahip), Cint, (Ref{Ptr{Cvoid}}, Int64), d, nb))
+    end
+    out = similar(W̄)
+    try
+        GC.@preserve W̄ out begin
+            _check(ccall((:vh_buffer_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), dW[], W̄, nb))
+            _check(ccall((:vh_adjoint_backward, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                         r.ptr, dW[], dU[]))
+            _check(ccall((:vh_buffer_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Int64), out, dU[], nb))
+        end
+    finally
+        for d in (dW, dU)
+            ccall((:vh_buffer_free, libvespahip), Cint, (Ptr{Cvoid},), d[])
+        end
+    end
+    return out
+end
+
 # The rrule a maintainer registers with ChainRulesCore (not loaded here):
 #   function ChainRulesCore.rrule(::typeof(hip_step_state), g::HIPGrid, U, dt)
 #       Unew = hip_step_state(g, U, dt)
@@ -374,6 +487,7 @@ end
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
        HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
-       hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_max_wavespeed
+       hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_max_wavespeed, hip_step_bflux!, hip_bflux_count,
+       hip_fluxrec, hip_flux_register, HIPRollout, hip_forward!, hip_backward
 
 end # module
