@@ -97,7 +97,9 @@ def blocks_leg(V, args):
     {8, 16, 24, 32}, glued in random conforming and 2:1 pairs, smooth seeded
     state, single-rate SSP-RK2 with the device CFL dt. One persistent launch per
     RK stage replayed from a HIP graph (headline of the leg), the same launches
-    enqueued eagerly, and one launch per block per stage (the comparison loop)."""
+    enqueued eagerly, and one launch per block per stage (the comparison loop).
+    Tile width, z planes per tile and the persistent workgroup count are picked
+    by the autotuner first (timed graph replays of each candidate)."""
     import numpy as np
     from vespa_jl_amd import blockmesh as BM
     m = BM.random_block_pairs(args.blocks, seed=2024, rank=3, edges=(8, 16, 24, 32))
@@ -111,10 +113,19 @@ def blocks_leg(V, args):
         s = BM.BlockSimulation(m, dtype, cfl=0.3)
         s.set_mode(args.mode)
         s.upload(U.astype(dtype))
+        tuned = None
+        if not args.no_autotune:
+            from vespa_jl_amd import autotune
+            t0 = time.perf_counter()
+            best = autotune.tune_blocks(s, use_cache=False)
+            tuned = {"max_tile_x": best["max_tile_x"], "zchunk": best["zchunk"],
+                     "max_workgroups": best["max_workgroups"], "candidates": len(best["trials"]),
+                     "seconds": time.perf_counter() - t0}
+            log(f"[bench] blocks {key} autotune: {tuned}")
         info = s.info()
         esz = np.dtype(dtype).itemsize
         r = {"tables": {k: info[k] for k in ("tiles", "link_faces", "register_cells")},
-             "zchunk": info["zchunk"], "workgroups": info["workgroups"]}
+             "zchunk": info["zchunk"], "workgroups": info["workgroups"], "autotune": tuned}
         methods = args.block_methods.split(",")
         for method, nst in (("graph", args.block_steps), ("persistent", args.block_steps),
                             ("per_block", max(2, args.block_steps // 4))):
@@ -478,6 +489,7 @@ def main():
     ap.add_argument("--fd-directions", type=int, default=16)
     ap.add_argument("--blocks", type=int, default=3000, help="blocks of the batched-blocks leg")
     ap.add_argument("--block-steps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true", help="batched-blocks leg: keep the default launch shape")
     ap.add_argument("--block-methods", default="graph,persistent,per_block",
                     help="launch methods of the blocks leg, in order (Float32 runs the first only)")
     ap.add_argument("--calib-copy", action="store_true",
