@@ -117,3 +117,32 @@ def test_describe_rejects_bad_meshes(bad, match):
                          [[1] * 6] * 2)
     with pytest.raises(VespaHIPError, match=match):
         BM.describe(m)
+
+
+def _ring_quadtree(rounds):
+    """A quadtree of 6x6 root blocks (8x8 cells, 3 levels) refined around growing rings:
+    deterministic meshes of 48 .. ~1200 blocks with every kind of coarse/fine face."""
+    from vespa_jl_amd import amr
+    qb = amr.QuadBlocks(6, 8, 3)
+    for r in range(rounds):
+        x, y = qb.cell_centers()
+        rr = np.sqrt(x * x + y * y)
+        g = np.zeros(len(qb.blocks))
+        for q in range(len(qb.blocks)):
+            s = rr[q * 64:(q + 1) * 64]
+            g[q] = 1.0 if ((s > 0.05 + 0.03 * r) & (s < 0.15 + 0.03 * r)).any() else 0.0
+        qb.refine(qb.tags_from_indicator(g, 0.5, symmetric=True))
+    return qb.mesh()
+
+
+@pytest.mark.parametrize("rounds,counts", [
+    (3, {"cells": 21504, "tiles": 336, "link_faces": 5632, "link_cells": 9260, "register_cells": 692,
+         "slot_faces": 11264}),
+    (8, {"cells": 74496, "tiles": 1164, "link_faces": 19008, "link_cells": 32348, "register_cells": 1052,
+         "slot_faces": 38016}),
+])
+def test_describe_counts_of_refined_quadtrees(rounds, counts):
+    """Table sizes of the host builder on refined quadtrees, as first recorded with
+    node-based maps (the builder's lookups changed, its tables must not)."""
+    m = _ring_quadtree(rounds)
+    assert BM.describe(m) == counts
