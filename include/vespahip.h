@@ -51,6 +51,10 @@ int vh_abi_version(void);
 /* Number of visible HIP devices (0 when none; never an error). */
 int vh_device_count(void);
 
+/* Compute units of the current device (no counterpart in the reference: the
+ * launch autotuner sizes its workgroup candidates and keys its cache by it). */
+int vh_device_cu_count(int32_t* cus);
+
 /* ---------------------------------------------------------------------------
  * Tier 1 — exact drop-in for compute_face_fluxes!(::KAFlux, model, F, WL, WR, axisv)
  * (ext/VespaKAFluxExt.jl:37-63; contract src/ka_flux.jl:40-53): HLLC flux
@@ -59,7 +63,9 @@ int vh_device_count(void);
  *
  * Host-pointer form: copies in, runs on the device, copies back (the reference
  * kernel's own host-to-device crossing, ext/VespaKAFluxExt.jl:51-61), reusing
- * device staging buffers across calls.
+ * device staging buffers across calls. An axis value outside 1..3 is rejected
+ * (-1) before anything is copied, as the reference's tuple index raises
+ * (src/riemann.jl:36-37).
  */
 int vh_face_flux_f64(const double* WL, const double* WR, const int32_t* axis, int64_t nf,
                      double gamma, double* F);
@@ -67,7 +73,9 @@ int vh_face_flux_f32(const float* WL, const float* WR, const int32_t* axis, int6
                      double gamma, float* F);
 
 /* Device-pointer form of the same contract, enqueued on `stream` (a
- * hipStream_t, NULL = the default stream). No synchronisation. */
+ * hipStream_t, NULL = the default stream). No synchronisation. The device
+ * axis array is not validated (that would need a host copy): a value <= 1
+ * computes an x face and a value >= 3 a z face. */
 int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf,
                          double gamma, double* dF, void* stream);
 int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf,
@@ -354,7 +362,10 @@ int64_t vh_muscl_ncell(const vh_muscl* m);
  * energy ρe as a sixth field (initialised to 0; upload the six fields after this call),
  * eint is selected by _dual_eint, ρe is advected with the HLL flux ρe·u and synced with
  * τ at the end of each step (lib/PPMKernels/src/muscl_grid.jl:34-115); recorded
- * fluxes then hold 6 fields per axis. Idempotent; eta1 may be changed later. */
+ * fluxes then hold 6 fields per axis. Idempotent; eta1 may be changed later. The
+ * first call moves the state to a new device allocation, so it is rejected (-1,
+ * nothing changed) once vh_muscl_state_device_ptr has handed out the address:
+ * enable dual energy first, then take the pointer. */
 int vh_muscl_enable_dual_energy(vh_muscl* m, double eta1);
 /* 5 (or 6) fields × ncell values, field-major. */
 int vh_muscl_upload(vh_muscl* m, const void* U);

@@ -55,6 +55,15 @@ def test_argument_validation_without_device(V):
     assert b"gamma" in L.vh_last_error()
     assert L.vh_face_flux_f64(None, None, None, -1, 1.4, None) != 0
     assert L.vh_face_flux_f64(None, None, None, 0, 1.4, None) == 0      # nf == 0 no-op
+    # axis outside 1..3 is rejected before any copy (the reference's tuple index raises)
+    W = np.ones((3, 5))
+    F = np.zeros((3, 5))
+    dp = C.POINTER(C.c_double)
+    for bad in (0, 4, -1):
+        ax = np.array([1, bad, 2], dtype=np.int32)
+        rc = L.vh_face_flux_f64(W.ctypes.data_as(dp), W.ctypes.data_as(dp),
+                                ax.ctypes.data_as(C.POINTER(C.c_int32)), 3, 1.4, F.ctypes.data_as(dp))
+        assert rc == -1 and b"axis" in L.vh_last_error()
 
 
 def test_host_mirror_validation(V):

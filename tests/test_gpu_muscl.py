@@ -225,6 +225,36 @@ def test_dual_energy_array_form(O):
     np.testing.assert_array_equal(fr, fo)
 
 
+def test_dual_energy_refused_after_device_pointer_is_shared():
+    """Enabling dual energy reallocates the state; once the device address has
+    been handed out the call is refused and the context stays usable."""
+    import ctypes as C
+    dims, ng = (10, 9, 8), 2
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10)
+    try:
+        L = V.load_library()
+        p = C.c_void_p()
+        assert L.vh_muscl_state_device_ptr(g._h, C.byref(p)) == 0 and p.value
+        assert L.vh_muscl_enable_dual_energy(g._h, 1e-3) == -1
+        assert b"state_device_ptr" in L.vh_last_error()
+        p2 = C.c_void_p()
+        assert L.vh_muscl_state_device_ptr(g._h, C.byref(p2)) == 0 and p2.value == p.value
+        U = smooth_state(dims, seed=2, dtype=np.float64)
+        g.upload(U)
+        g.step(0.002)
+        assert np.isfinite(g.state()).all()
+    finally:
+        g.close()
+    # enabling first, then taking the pointer, is the supported order
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10, dual=True, eta1=1e-3)
+    try:
+        p = C.c_void_p()
+        assert V.load_library().vh_muscl_state_device_ptr(g._h, C.byref(p)) == 0
+        assert V.load_library().vh_muscl_enable_dual_energy(g._h, 2e-3) == 0     # already on: eta1 only
+    finally:
+        g.close()
+
+
 # Fast profile: per field, max |fast − oracle| / max |oracle| after three steps.
 FAST_TOL = {np.float64: 1e-12, np.float32: 1e-5}
 

@@ -7,7 +7,7 @@ load it with ``importlib`` under the module name ``vespa_jl_amd`` (as the tests,
 ``bench.py`` and ``__graft_entry__.py`` do).
 """
 from ._lib import (DeviceBuffer, BC_OUTFLOW, BC_PERIODIC, BC_REFLECTING, EXPORTED, LIB_PATH, VH_EXACT, VH_F32, VH_F64, VH_FAST,
-                   VespaHIPError, check, device_count, load_library)
+                   VespaHIPError, check, device_count, device_cu_count, load_library)
 from .hydro import (BoundaryFluxRegister, HIPFlux, IdealHydro, Outflow, Periodic, Problem, Reflecting, Simulation,
                     bflux_count, bflux_plane,
                     UniformMesh, compute_dt, compute_face_fluxes, cons2prim_array, conserved_totals,
@@ -18,7 +18,7 @@ from .muscl import MusclGrid, cons_to_prim, muscl_step_3d, prim_to_cons, total_f
 __all__ = [
     "DeviceBuffer",
     "BC_OUTFLOW", "BC_PERIODIC", "BC_REFLECTING", "EXPORTED", "LIB_PATH", "VH_EXACT", "VH_F32", "VH_F64", "VH_FAST",
-    "VespaHIPError", "check", "device_count", "load_library",
+    "VespaHIPError", "check", "device_count", "device_cu_count", "load_library",
     "BoundaryFluxRegister", "bflux_count", "bflux_plane",
     "HIPFlux", "IdealHydro", "Outflow", "Periodic", "Problem", "Reflecting", "Simulation",
     "UniformMesh", "compute_dt", "compute_face_fluxes", "cons2prim_array", "conserved_totals",

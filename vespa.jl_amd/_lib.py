@@ -18,7 +18,7 @@ BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
 
 # Every symbol include/vespahip.h declares (checked by the CPU test suite).
 EXPORTED = (
-    "vh_last_error", "vh_abi_version", "vh_device_count",
+    "vh_last_error", "vh_abi_version", "vh_device_count", "vh_device_cu_count",
     "vh_face_flux_f64", "vh_face_flux_f32", "vh_face_flux_dev_f64", "vh_face_flux_dev_f32",
     "vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download",
     "vh_buffer_copy",
@@ -66,6 +66,8 @@ def load_library(path: str | None = None):
     L.vh_last_error.restype = C.c_char_p
     L.vh_abi_version.restype = C.c_int
     L.vh_device_count.restype = C.c_int
+    L.vh_device_cu_count.argtypes = [C.POINTER(C.c_int32)]
+    L.vh_device_cu_count.restype = C.c_int
     for name, tp in (("f64", dp), ("f32", fp)):
         f = getattr(L, f"vh_face_flux_{name}")
         f.argtypes = [tp, tp, i32p, C.c_int64, C.c_double, tp]
@@ -234,3 +236,10 @@ class DeviceBuffer:
 
 def device_count() -> int:
     return int(load_library().vh_device_count())
+
+
+def device_cu_count() -> int:
+    """Compute units of the current HIP device."""
+    n = C.c_int32()
+    check(load_library().vh_device_cu_count(C.byref(n)))
+    return int(n.value)

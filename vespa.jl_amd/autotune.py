@@ -50,8 +50,14 @@ def _store(d: dict) -> None:
         pass
 
 
+def _device_sig() -> str:
+    from ._lib import device_cu_count
+    return f"cu{device_cu_count()}"
+
+
 def _grid_key(sim) -> str:
-    return f"grid:{sim.mesh.rank}:{'x'.join(map(str, sim.mesh.dims))}:{sim.eltype.name}:{sim.mode}"
+    return (f"grid:{sim.mesh.rank}:{'x'.join(map(str, sim.mesh.dims))}:{sim.eltype.name}:{sim.mode}:"
+            f"{_device_sig()}")
 
 
 def tune_grid(sim, steps: int = 5, zchunks=(8, 16, 32), use_cache: bool = True) -> dict:
@@ -85,7 +91,7 @@ def tune_grid(sim, steps: int = 5, zchunks=(8, 16, 32), use_cache: bool = True) 
 def _blocks_key(bsim) -> str:
     m = bsim.mesh
     sig = (int(m.nblocks), int(m.ncell), int(m.rank), int(np.bincount(m.dims[:, 0]).argmax()))
-    return f"blocks:{':'.join(map(str, sig))}:{bsim.eltype.name}"
+    return f"blocks:{':'.join(map(str, sig))}:{bsim.eltype.name}:{_device_sig()}"
 
 
 def tune_blocks(bsim, steps: int = 3, max_tile_x=(16, 32), zchunks=(0, 8, 16), wgs_per_cu=(0, 4),
@@ -100,7 +106,8 @@ def tune_blocks(bsim, steps: int = 3, max_tile_x=(16, 32), zchunks=(0, 8, 16), w
         bsim.set_launch(zchunk=c["zchunk"], max_workgroups=c["max_workgroups"])
         return dict(c, cached=True)
     U0 = bsim.state()
-    ncu = 256
+    from ._lib import device_cu_count
+    ncu = device_cu_count()
     trials = []
     for mt in max_tile_x:
         bsim.set_tiling(mt)

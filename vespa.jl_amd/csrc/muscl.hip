@@ -132,6 +132,7 @@ struct vh_muscl {
   void* frec = nullptr;
   double* partials = nullptr;
   hipStream_t stream = nullptr;
+  bool state_ptr_shared = false;   // vh_muscl_state_device_ptr has handed out U
 };
 
 namespace {
@@ -244,6 +245,7 @@ int vh_muscl_download(vh_muscl* m, void* U) {
 int vh_muscl_state_device_ptr(vh_muscl* m, void** ptr) {
   if (!m || !ptr) return fail(-1, "null argument%s", "");
   *ptr = m->U;
+  m->state_ptr_shared = true;
   return 0;
 }
 
@@ -251,8 +253,16 @@ int vh_muscl_enable_dual_energy(vh_muscl* m, double eta1) {
   if (!m) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
   VH_HIP(hipStreamSynchronize(m->stream));
+  if (m->G.nf == 6) {
+    m->G.eta1 = eta1;
+    return 0;
+  }
+  // Adding the sixth field moves the state to a new allocation: refuse once a
+  // caller holds the old address (it would point at freed memory).
+  if (m->state_ptr_shared)
+    return fail(-1, "vh_muscl_enable_dual_energy after vh_muscl_state_device_ptr: the state would move%s; "
+                "enable dual energy before taking the device pointer", "");
   m->G.eta1 = eta1;
-  if (m->G.nf == 6) return 0;
   const size_t b5 = (size_t)5 * m->G.ncell * m->esize, b6 = (size_t)6 * m->G.ncell * m->esize;
   void *U = nullptr, *U1 = nullptr;
   if (hipMalloc(&U, b6) != hipSuccess || hipMalloc(&U1, b6) != hipSuccess) {

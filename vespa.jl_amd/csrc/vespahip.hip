@@ -234,6 +234,15 @@ int vh_device_count(void) {
   return n;
 }
 
+int vh_device_cu_count(int32_t* cus) {
+  if (!cus) return fail(-1, "null argument%s", "");
+  int dev = 0, n = 0;
+  VH_HIP(hipGetDevice(&dev));
+  VH_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+  *cus = n;
+  return 0;
+}
+
 // ---- tier 1 ----
 
 int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* daxis, int64_t nf, double gamma,
@@ -274,6 +283,8 @@ static int face_flux_host(const T* WL, const T* WR, const int32_t* ax, int64_t n
   if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
   if (nf == 0) return 0;
   if (!WL || !WR || !ax || !F) return fail(-1, "null pointer%s", "");
+  for (int64_t i = 0; i < nf; ++i)
+    if (ax[i] < 1 || ax[i] > 3) return fail(-1, "axis must be 1, 2 or 3%s (face %lld)", "", i);
   int dev = 0;
   VH_HIP(hipGetDevice(&dev));
   const size_t nv = (size_t)nf * 5 * sizeof(T);
