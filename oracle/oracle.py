@@ -35,7 +35,8 @@ class Grid(C.Structure):
 
 
 def build() -> None:
-    """Compile liboracle.so from oracle/vespa_oracle.c (gcc, -ffp-contract=off)."""
+    """Compile liboracle.so from oracle/vespa_oracle.c and oracle/jvp_oracle.cc
+    (gcc/g++, -ffp-contract=off)."""
     subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
 
 
@@ -99,6 +100,11 @@ def lib():
         L.vo_c2p_vjp_f64.argtypes = [dp, C.c_double, dp, dp]
         L.vo_hllc_vjp_f64.argtypes = [dp, dp, C.c_double, C.c_int, dp, dp, dp]
         L.vo_exact_riemann_sample.argtypes = [dp, dp, C.c_double, C.c_double, dp]
+        L.vo_jvp_max_directions.restype = C.c_int
+        L.vo_rollout_jvp_f64.argtypes = [gp, dp, C.c_int64, dp, C.c_int, dp, C.c_int, dp, dp]
+        L.vo_rollout_jvp_f64.restype = C.c_int
+        L.vo_c2p_jvp_f64.argtypes = [dp, C.c_double, dp, dp, dp]
+        L.vo_hllc_jvp_f64.argtypes = [dp, dp, C.c_double, C.c_int, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -233,6 +239,43 @@ def step_vjp(g: Grid, U: np.ndarray, dt: float, wbar: np.ndarray) -> np.ndarray:
     if lib().vo_step_vjp_f64(C.byref(g), _ptr(U), float(dt), _ptr(wbar), _ptr(out)) != 0:
         raise ValueError("invalid grid")
     return out
+
+
+def rollout_jvp(g: Grid, U: np.ndarray, dts, V: np.ndarray, threads: int = 0):
+    """Forward mode of len(dts) steps with the given dts held fixed: returns
+    (final state, J·V) for tangent directions V shaped (ndir, *U.shape), ndir <=
+    jvp_max_directions(). The final state equals the oracle's steps bit for bit;
+    non-smooth points take the reverse mode's branches (jvp_oracle.cc)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    if V.ndim != U.ndim + 1 or V.shape[1:] != U.shape:
+        raise ValueError("V must be (ndir, *U.shape)")
+    dts = np.ascontiguousarray(np.atleast_1d(dts), dtype=np.float64)
+    ndir = V.shape[0]
+    out = np.empty_like(U)
+    JV = np.empty_like(V)
+    if lib().vo_rollout_jvp_f64(C.byref(g), _ptr(U), len(dts), _ptr(dts), ndir, _ptr(V), int(threads),
+                                _ptr(out), _ptr(JV)) != 0:
+        raise ValueError("invalid grid or direction count")
+    return out, JV
+
+
+def jvp_max_directions() -> int:
+    return int(lib().vo_jvp_max_directions())
+
+
+def cons2prim_jvp(U, gamma, dU):
+    U, dU = (np.ascontiguousarray(a, dtype=np.float64) for a in (U, dU))
+    W, dW = np.zeros(5), np.zeros(5)
+    lib().vo_c2p_jvp_f64(_ptr(U), float(gamma), _ptr(dU), _ptr(W), _ptr(dW))
+    return W, dW
+
+
+def hllc_jvp(WL, WR, gamma, axis, dWL, dWR):
+    WL, WR, dWL, dWR = (np.ascontiguousarray(a, dtype=np.float64) for a in (WL, WR, dWL, dWR))
+    F, dF = np.zeros(5), np.zeros(5)
+    lib().vo_hllc_jvp_f64(_ptr(WL), _ptr(WR), float(gamma), int(axis), _ptr(dWL), _ptr(dWR), _ptr(F), _ptr(dF))
+    return F, dF
 
 
 def cons2prim_vjp(U, gamma, Wbar):

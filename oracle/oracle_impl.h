@@ -16,7 +16,11 @@
 #define OCAT(a, b) OCAT2(a, b)
 #define ON(name) OCAT(name, OSUF)
 
-/* Julia min/max on IEEE floats: NaN-propagating, -0.0 < +0.0. */
+/* Julia min/max on IEEE floats: NaN-propagating, -0.0 < +0.0. An includer
+ * that defines OIMPL_OWN_MINMAX supplies ON(jmin)/ON(jmax) itself (the
+ * dual-number instantiation, which also fixes which argument's tangent a tie
+ * takes). */
+#ifndef OIMPL_OWN_MINMAX
 static inline OT ON(jmin)(OT x, OT y) {
     if (x != x) return x;
     if (y != y) return y;
@@ -31,6 +35,7 @@ static inline OT ON(jmax)(OT x, OT y) {
     if (x > y) return x;
     return signbit(y) ? x : y;
 }
+#endif
 
 /* src/eos.jl:8-25 cons2prim. floor64 != 0 selects the FP32 form's deviation:
  * the pressure and its positivity floor are evaluated in double from the Tf
