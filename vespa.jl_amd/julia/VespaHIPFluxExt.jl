@@ -11,8 +11,10 @@
 #     exact (bit-identical) or fast arithmetic profile;
 #   * `HIPBlocks`: block-structured AMR meshes (hanging-node faces, flux
 #     registers, subcycling) advanced by one persistent launch per RK stage;
-#   * `hip_step_vjp`: the pullback of one step (the reverse mode an rrule calls),
-#     and `HIPRollout` (`hip_forward!` / `hip_backward`): a checkpointed rollout;
+#   * `hip_step_vjp`: the pullback of one step, and `HIPRollout` (`hip_forward!` /
+#     `hip_backward`): a checkpointed rollout; `hip_step_state` / `hip_rollout_state`
+#     are the state-to-state functions whose ChainRulesCore `rrule`s live in
+#     VespaHIPChainRulesExt.jl (a weak-dependency extension);
 #   * `hip_step_bflux!`: step! with the boundary-flux register (src/reflux.jl);
 #   * `HIPMuscl`: PPMKernels' unsplit RK2 MUSCL step with flux recording
 #     (`hip_fluxrec`, `hip_flux_register`);
@@ -130,7 +132,14 @@ end
 hip_set_mode!(g::HIPGrid, fast::Bool) =
     (_check(ccall((:vh_grid_set_mode, libvespahip), Cint, (Ptr{Cvoid}, Int32), g.ptr, fast ? VH_FAST : VH_EXACT)); g)
 
-"step!(sim, dt) on the device (src/driver.jl:333-359)."
+"""
+    hip_step!(sim, g::HIPGrid, dt)
+
+step!(sim, dt) on the device (src/driver.jl:333-359). Advances `sim.t` and
+`sim.step` only: the new state stays in device memory, so `sim`'s field arrays
+are stale until `hip_download!(sim, g)` (keeping the state resident is what
+removes the per-step PCIe round trip). `hip_evolve!` downloads at its end.
+"""
 function hip_step!(sim, g::HIPGrid, dt::Real)
     _check(ccall((:vh_grid_step, libvespahip), Cint, (Ptr{Cvoid}, Float64), g.ptr, Float64(dt)))
     sim.t += dt
@@ -479,15 +488,48 @@ function hip_backward(r::HIPRollout, W̄::Vector{Float64})
     return out
 end
 
-# The rrule a maintainer registers with ChainRulesCore (not loaded here):
-#   function ChainRulesCore.rrule(::typeof(hip_step_state), g::HIPGrid, U, dt)
-#       Unew = hip_step_state(g, U, dt)
-#       return Unew, Ū -> (NoTangent(), NoTangent(), hip_step_vjp(g, U, dt, Ū), NoTangent())
-#   end
+"""
+    hip_step_state(g::HIPGrid, U, dt) -> U′
+
+The step as a function of the state (the form `rrule` differentiates): uploads U
+(5 × ncell Float64, field-major), runs one step!(·, dt) on the device and returns
+the new state as a fresh host vector. The grid's device state is overwritten.
+"""
+function hip_step_state(g::HIPGrid, U::Vector{Float64}, dt::Real)
+    g.eltype === Float64 || error("hip_step_state: the reverse mode is Float64 only")
+    length(U) == 5 * g.ncell || error("hip_step_state: U must hold 5 × ncell values")
+    GC.@preserve U _check(ccall((:vh_grid_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, U))
+    _check(ccall((:vh_grid_step, libvespahip), Cint, (Ptr{Cvoid}, Float64), g.ptr, Float64(dt)))
+    out = similar(U)
+    GC.@preserve out _check(ccall((:vh_grid_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, out))
+    return out
+end
+
+"""
+    hip_rollout_state(r::HIPRollout, U₀, cfl) -> U_N
+
+The whole checkpointed rollout as a function of its initial state: uploads U₀,
+runs `r.nsteps` steps with the device CFL dt (checkpoints kept on the device) and
+returns the final state. Its pullback is `hip_backward(r, Ū)` on the dt schedule
+of this forward pass.
+"""
+function hip_rollout_state(r::HIPRollout, U₀::Vector{Float64}, cfl::Real)
+    g = r.grid
+    length(U₀) == 5 * g.ncell || error("hip_rollout_state: U₀ must hold 5 × ncell values")
+    GC.@preserve U₀ _check(ccall((:vh_grid_upload, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, U₀))
+    hip_forward!(r, cfl)
+    out = similar(U₀)
+    GC.@preserve out _check(ccall((:vh_grid_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), g.ptr, out))
+    return out
+end
+
+# The ChainRulesCore rules for hip_step_state and hip_rollout_state live in
+# VespaHIPChainRulesExt.jl (a weak-dependency extension: loaded only together
+# with ChainRulesCore).
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
        HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
        hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_max_wavespeed, hip_step_bflux!, hip_bflux_count,
-       hip_fluxrec, hip_flux_register, HIPRollout, hip_forward!, hip_backward
+       hip_fluxrec, hip_flux_register, HIPRollout, hip_forward!, hip_backward, hip_step_state, hip_rollout_state
 
 end # module
