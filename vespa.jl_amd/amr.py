@@ -202,29 +202,29 @@ def sedov_quadblocks_state(qb: QuadBlocks, E0: float = 1.0, rho0: float = 1.0, p
     return U.astype(dtype)
 
 
-def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.05, cfl: float = 0.4,
-              refine_above: float = 0.1, every: int = 8, dtype=np.float64, mode: str = "exact",
-              snapshots=(), max_steps: int = 100000, symmetric: bool = True):
-    """Sedov blast on [-0.5, 0.5]² with base (nroot·B)² cells and up to maxlevel
-    levels of refinement, reflecting walls. Regrid before the first step and
-    every `every` steps (evolve! with a RefinementPolicy, src/driver.jl:483-508);
-    the state stays on the device from the first upload to the final download.
-    Returns a dict with the final blocks/state, step count, leaf counts, the
-    raster density at the requested snapshot times and the wall time spent in
-    the regrid phases (ms): indicator kernel + per-block maxima to the host,
-    host tagging and refinement, the new mesh's context (tables), and the remap."""
-    qb = QuadBlocks(nroot, B, maxlevel)
-    U0 = sedov_quadblocks_state(qb, dtype=np.float64).astype(dtype)
-    totals0 = (U0.astype(np.float64) * qb.volumes()).sum(axis=1)
+def evolve_amr(qb: QuadBlocks, U0: np.ndarray, tfinal: float, cfl: float = 0.4, refine_above: float = 0.1,
+               every: int = 8, dtype=np.float64, mode: str = "exact", snapshots=(), max_steps: int = 100000,
+               symmetric: bool = False):
+    """evolve! with a RefinementPolicy (src/driver.jl:483-508) on the quadtree
+    `qb` from the conserved state U0 (5, ncell) in its leaf order: regrid before
+    the first step and every `every` steps (refine-only), leaves advanced by the
+    batched kernel, the state on the device from the first upload to the final
+    download. Returns a dict with the final blocks/state, step count, leaf
+    counts, the raster density at the requested snapshot times and the wall
+    time spent in the regrid phases (ms): indicator kernel + per-block maxima to
+    the host, host tagging and refinement, the new mesh's context (tables), and
+    the remap."""
+    totals0 = (np.asarray(U0, dtype=np.float64) * qb.volumes()).sum(axis=1)
     sim = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
     sim.set_mode(mode)
-    sim.upload(U0)
+    sim.upload(np.ascontiguousarray(U0, dtype=dtype))
     t, steps = 0.0, 0
     snaps = {}
     pending = sorted(snapshots)
     leaves_hist = []
     phase = {"indicator": 0.0, "refine": 0.0, "tables": 0.0, "remap": 0.0}
     nregrid = ncheck = 0
+    B = qb.B
     try:
         while t < tfinal * (1 - 1e-12) and steps < max_steps:
             t0 = time.perf_counter()
@@ -273,6 +273,32 @@ def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.0
             "max_leaves": max(leaves_hist) if leaves_hist else 0, "totals0": totals0, "totals": totals,
             "snapshots": snaps, "indicator_calls": ncheck, "regrids": nregrid,
             "regrid_ms": {k: 1e3 * v for k, v in phase.items()}}
+
+
+def run_sedov(nroot: int = 6, B: int = 8, maxlevel: int = 3, tfinal: float = 0.05, cfl: float = 0.4,
+              refine_above: float = 0.1, every: int = 8, dtype=np.float64, mode: str = "exact",
+              snapshots=(), max_steps: int = 100000, symmetric: bool = True):
+    """Sedov blast on [-0.5, 0.5]² with base (nroot·B)² cells and up to maxlevel
+    levels of refinement, reflecting walls, through evolve_amr."""
+    qb = QuadBlocks(nroot, B, maxlevel)
+    U0 = sedov_quadblocks_state(qb, dtype=np.float64).astype(dtype)
+    return evolve_amr(qb, U0, tfinal, cfl=cfl, refine_above=refine_above, every=every, dtype=dtype, mode=mode,
+                      snapshots=snapshots, max_steps=max_steps, symmetric=symmetric)
+
+
+def oblique_sod_state(qb: QuadBlocks, angle: float = np.pi / 4, gamma: float = 1.4, dtype=np.float64) -> np.ndarray:
+    """Sod's Riemann problem with its diaphragm through the centre of the square
+    along the normal n = (cos angle, sin angle): left state (ρ 1, p 1) where
+    n·x < 0, right state (ρ 0.125, p 0.1), at rest (the planar problem of
+    src/problem.jl:61-71 turned by `angle`). Its exact solution is the 1D one in
+    ξ = n·x until boundary effects arrive."""
+    x, y = qb.cell_centers()
+    xi = np.cos(angle) * x + np.sin(angle) * y
+    left = xi < 0
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    z = np.zeros_like(rho)
+    return np.stack([rho, z, z, z, p / (gamma - 1.0)]).astype(dtype)
 
 
 def shock_radius(rho_raster: np.ndarray, x0: float = -0.5, L: float = 1.0) -> float:
