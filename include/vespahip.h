@@ -81,6 +81,20 @@ int vh_face_flux_dev_f64(const double* dWL, const double* dWR, const int32_t* da
 int vh_face_flux_dev_f32(const float* dWL, const float* dWR, const int32_t* daxis, int64_t nf,
                          double gamma, float* dF, void* stream);
 
+/* No counterpart in the reference. Allocation-free form of vh_face_flux_f64/_f32
+ * for callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc): the same exact HLLC fluxes, with WL, WR, F and axis
+ * staged in a fixed device array that the library holds on each GPU it runs on
+ * until the process ends (VH_NOALLOC_MAX_FACES faces: 31 MiB). dtype VH_F64 or
+ * VH_F32 selects the element type of WL, WR and F (host memory, [nf][5]).
+ * nf > VH_NOALLOC_MAX_FACES is rejected (-1) before anything is copied, as is an
+ * axis outside 1..3. Enqueued on `stream` (a hipStream_t, NULL = the default
+ * stream); returns after F holds the fluxes. Calls no hipMalloc/hipFree. Calls
+ * that overlap on one GPU share the staging array and race. */
+#define VH_NOALLOC_MAX_FACES 262144
+int vh_face_flux_noalloc(const void* WL, const void* WR, const int32_t* axis, int64_t nf,
+                         double gamma, int32_t dtype, void* F, void* stream);
+
 /* Device buffers for callers without a GPU array package of their own (the
  * reference's KA.allocate / copyto!, ext/VespaKAFluxExt.jl:51-54,59). */
 int vh_buffer_alloc(void** dptr, int64_t bytes);

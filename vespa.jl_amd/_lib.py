@@ -20,6 +20,7 @@ BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
 EXPORTED = (
     "vh_last_error", "vh_abi_version", "vh_device_count", "vh_device_cu_count",
     "vh_face_flux_f64", "vh_face_flux_f32", "vh_face_flux_dev_f64", "vh_face_flux_dev_f32",
+    "vh_face_flux_noalloc",
     "vh_buffer_alloc", "vh_buffer_free", "vh_buffer_upload", "vh_buffer_download",
     "vh_buffer_copy",
     "vh_grid_create", "vh_grid_destroy", "vh_grid_ncell", "vh_grid_set_mode", "vh_grid_get_mode", "vh_grid_upload", "vh_grid_download",
@@ -75,6 +76,8 @@ def load_library(path: str | None = None):
         f = getattr(L, f"vh_face_flux_dev_{name}")
         f.argtypes = [vp, vp, vp, C.c_int64, C.c_double, vp, vp]
         f.restype = C.c_int
+    L.vh_face_flux_noalloc.argtypes = [vp, vp, i32p, C.c_int64, C.c_double, C.c_int32, vp, vp]
+    L.vh_face_flux_noalloc.restype = C.c_int
     L.vh_buffer_alloc.argtypes = [C.POINTER(vp), C.c_int64]
     L.vh_buffer_free.argtypes = [vp]
     L.vh_buffer_upload.argtypes = [vp, vp, C.c_int64]

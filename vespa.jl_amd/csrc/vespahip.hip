@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/vespahip.h"
@@ -324,6 +325,73 @@ int vh_face_flux_f64(const double* WL, const double* WR, const int32_t* axis, in
 int vh_face_flux_f32(const float* WL, const float* WR, const int32_t* axis, int64_t nf, double gamma,
                      float* F) {
   return face_flux_host<float>(WL, WR, axis, nf, gamma, F);
+}
+
+}  // extern "C"
+
+// ---- allocation-free host-pointer form ----
+// vh_face_flux_noalloc stages WL, WR, F and the axes in `scratch`, a fixed array
+// in device memory that every GPU running the library holds until the process
+// ends. In host code `scratch` names a host-side stand-in, never the device
+// memory: its device address is fetched once per device with
+// hipGetSymbolAddress (with that device current) and only that address is
+// passed to copies and kernels. Calls that overlap on one GPU share the array
+// and race; each call leaves its staging behind (nothing needs to start zero).
+static constexpr int64_t kNoAllocMaxFaces = VH_NOALLOC_MAX_FACES;
+static constexpr size_t kNoAllocValBytes = (size_t)kNoAllocMaxFaces * 5 * sizeof(double);  // one [nf][5] array
+static constexpr size_t kNoAllocBytes = 3 * kNoAllocValBytes + (size_t)kNoAllocMaxFaces * sizeof(int32_t);
+static_assert(kNoAllocValBytes % 256 == 0, "staging arrays stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices];   // device address of `scratch` per device
+
+static int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(-1, "device index%s %lld is beyond the supported range", "", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(-2, "hipGetSymbolAddress(scratch) failed: %s (code %lld)", hipGetErrorString(e), (long long)e);
+    g_scratch_dev[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" {
+
+int vh_face_flux_noalloc(const void* WL, const void* WR, const int32_t* axis, int64_t nf, double gamma,
+                         int32_t dtype, void* F, void* stream) {
+  if (nf < 0) return fail(-1, "nf must be >= 0%s (got %lld)", "", nf);
+  if (dtype != VH_F64 && dtype != VH_F32) return fail(-1, "dtype must be VH_F64 or VH_F32%s (got %lld)", "", dtype);
+  if (nf > kNoAllocMaxFaces)
+    return fail(-1, "vh_face_flux_noalloc: nf = %s%lld exceeds the limit of VH_NOALLOC_MAX_FACES (262144) faces "
+                "per call; split the batch or use vh_face_flux_f64/_f32", "", nf);
+  if (nf == 0) return 0;
+  if (!WL || !WR || !axis || !F) return fail(-1, "null pointer%s", "");
+  for (int64_t i = 0; i < nf; ++i)
+    if (axis[i] < 1 || axis[i] > 3) return fail(-1, "axis must be 1, 2 or 3%s (face %lld)", "", i);
+  int dev = 0;
+  VH_HIP(hipGetDevice(&dev));
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(dev, &base)) return rc;
+  const size_t nv = (size_t)nf * 5 * (dtype == VH_F64 ? sizeof(double) : sizeof(float));
+  unsigned char* dWL = base;
+  unsigned char* dWR = base + kNoAllocValBytes;
+  unsigned char* dF = base + 2 * kNoAllocValBytes;
+  int32_t* dax = (int32_t*)(base + 3 * kNoAllocValBytes);
+  hipStream_t s = (hipStream_t)stream;
+  VH_HIP(hipMemcpyAsync(dWL, WL, nv, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dWR, WR, nv, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dax, axis, (size_t)nf * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  exact::launch_face_flux(dtype == VH_F64 ? 0 : 1, nf, s, dWL, dWR, dax, gamma, dF);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(F, dF, nv, hipMemcpyDeviceToHost, s));
+  VH_HIP(hipStreamSynchronize(s));
+  return 0;
 }
 
 int vh_buffer_alloc(void** dptr, int64_t bytes) {
