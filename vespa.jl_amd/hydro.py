@@ -244,7 +244,12 @@ class Simulation:
         self.eltype = np.dtype(eltype)
         if self.eltype not in (np.float64, np.float32):
             raise ValueError(f"eltype must be float64 or float32, got {self.eltype}")
-        self.flux = flux if flux is not None else HIPFlux()
+        # sim.flux (src/driver.jl:44): None runs the fused device stage kernels; a
+        # flux backend (HIPFlux()) runs accumulate_flux! in the reference's batched
+        # form through it (src/ka_flux.jl:128-162; batched.step_batched).
+        if flux is not None and not isinstance(flux, HIPFlux):
+            raise TypeError(f"no compute_face_fluxes method for {type(flux).__name__}")
+        self.flux = flux
         self.t = 0.0
         self.step_count = 0
         self.grav = None   # GravityField when self-gravity is on (gravity.enable_gravity)
@@ -254,6 +259,7 @@ class Simulation:
         lo = (C.c_double * 3)(*[mesh.domain[d][0] if d < rank else 0.0 for d in range(3)])
         hi = (C.c_double * 3)(*[mesh.domain[d][1] if d < rank else 1.0 for d in range(3)])
         bc = _bcs_array(prob.bcs, rank)
+        self._bc_pairs = [(int(bc[2 * d]), int(bc[2 * d + 1])) for d in range(rank)]
         h = C.c_void_p()
         check(self._lib.vh_grid_create(C.byref(h), rank, dims, lo, hi, bc, float(prob.gamma),
                                        _lib.VH_F64 if self.eltype == np.float64 else _lib.VH_F32))
@@ -351,7 +357,14 @@ def step(sim: Simulation, dt: float, bflux: "BoundaryFluxRegister | None" = None
     """One SSP-RK2 step (src/driver.jl:333-359); advances sim.t and the step
     count. With a BoundaryFluxRegister, ½dt·F·area of every domain-boundary face
     is added to it per RK stage (src/reflux.jl:105-132)."""
-    if bflux is None:
+    if sim.flux is not None:
+        if bflux is not None:
+            raise ValueError("the boundary-flux register is recorded by the fused device path (sim.flux = None)")
+        if sim.grav is not None:
+            raise ValueError("self-gravity runs in the fused device path (sim.flux = None)")
+        from .batched import step_batched
+        step_batched(sim, dt)
+    elif bflux is None:
         check(sim._lib.vh_grid_step(sim._h, float(dt)))
     else:
         if bflux.flux.dtype != sim.eltype or bflux.flux.shape != (bflux_count(sim), 5):
@@ -424,6 +437,10 @@ def bflux_plane(rank: int, dim: int, start, stop, g0, flux_off: int, comp: int, 
 def run_steps(sim: Simulation, nsteps: int) -> Simulation:
     """nsteps steps with the CFL dt reduced on the device before each one (no
     clip, no host synchronisation inside the loop)."""
+    if sim.flux is not None:
+        for _ in range(int(nsteps)):
+            step(sim, compute_dt(sim))
+        return sim
     t = C.c_double(sim.t)
     k = C.c_int64()
     check(sim._lib.vh_grid_run(sim._h, float(sim.problem.cfl), 0.0, int(nsteps), C.byref(t), C.byref(k)))
@@ -437,7 +454,7 @@ def evolve(sim: Simulation, max_steps: int = 0, callback=None, callback_every: i
     tfinal − t), final step clipped to land on tfinal."""
     if callback is not None:
         callback(sim, "init")
-    if callback is None:
+    if callback is None and sim.flux is None:
         t = C.c_double(sim.t)
         k = C.c_int64()
         check(sim._lib.vh_grid_run(sim._h, float(sim.problem.cfl), float(sim.problem.tfinal),
@@ -453,7 +470,7 @@ def evolve(sim: Simulation, max_steps: int = 0, callback=None, callback_every: i
             dt = min(compute_dt(sim), tf - sim.t)
             step(sim, dt)
             n += 1
-            if callback_every > 0 and sim.step_count % callback_every == 0:
+            if callback is not None and callback_every > 0 and sim.step_count % callback_every == 0:
                 callback(sim, "step")
         sim.sync()
     if callback is not None:
