@@ -27,6 +27,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
 
+STAGE_SOURCES = ("common.h", "hydro.h", "stencil.h", "profiles.h", "stages.hip")
+
+
+def stage_sources_hash() -> str:
+    """sha256 over the sources the uniform stage kernels are compiled from."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in STAGE_SOURCES:
+        with open(os.path.join(ROOT, "vespa.jl_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -67,11 +80,38 @@ def timed_run(V, sim, steps, warmup):
     return wall, ev
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and those this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "available_to_process": avail}
+
+
+def baseline_threads():
+    """Every host thread this process may use, capped by OMP_NUM_THREADS when the
+    environment sets it (the GPU host allots 16 threads per GPU and exports
+    OMP_NUM_THREADS=16)."""
+    avail = host_cpu()["available_to_process"]
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
+
+
 def cpu_baseline(U, n, steps=2):
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = baseline_threads()
     g = O.make_grid((n, n, n), [(0.0, 1.0)] * 3, "periodic", 5.0 / 3.0)
     Uc = np.ascontiguousarray(U.copy())
     work = np.empty(10 * n ** 3, dtype=Uc.dtype)
@@ -86,6 +126,7 @@ def cpu_baseline(U, n, steps=2):
     O.step_omp(g, Uc, dt, threads=1, work=work)
     el1 = time.perf_counter() - t0
     return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "cpu": host_cpu(),
             "sample": f"{steps} SSP-RK2 steps (CFL dt each) of the same {n}^3 FP64 workload, oracle in "
                       f"gather form with OpenMP on {threads} host threads, {el:.2f} s wall",
             "serial": {"value": n ** 3 / el1, "unit": "zone-updates/s", "cores": 1, "kind": "port",
@@ -109,6 +150,7 @@ def blocks_leg(V, args):
                        f"level-1 blocks in 2:1 pairs, conforming pairs, smooth state rho,p in [0.5,2], "
                        f"|v|<=0.3, gamma=5/3, CFL 0.3 device dt, fast profile",
            "cells": m.ncell}
+    U_end, dts = {}, []
     for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
         s = BM.BlockSimulation(m, dtype, cfl=0.3)
         s.set_mode(args.mode)
@@ -141,8 +183,25 @@ def blocks_leg(V, args):
                 f"{m.ncell / (t['step_ms'] * 1e-3) / 1e9:.2f} G zone-updates/s")
         if key == "f64" and "graph" in r and "per_block" in r:
             r["speedup_graph_vs_per_block"] = r["per_block"]["ms_per_step"] / r["graph"]["ms_per_step"]
+        # precision check: the same steps (FP64's CFL dt schedule) from the same state
+        s.upload(U.astype(dtype))
+        if key == "f64":
+            dts = []
+            for _ in range(10):
+                dts.append(s.compute_dt())
+                s.step(dts[-1])
+        else:
+            for dt in dts:
+                s.step(dt)
+        U_end[key] = s.state().astype(np.float64)
         s.close()
         res[key] = r
+    den = np.abs(U_end["f64"]).mean(axis=1)
+    res["f32"]["rel_L1_vs_fp64_per_field"] = dict(zip(
+        ["density", "momentum_x", "momentum_y", "momentum_z", "energy"],
+        (np.abs(U_end["f32"] - U_end["f64"]).mean(axis=1) / den).tolist()))
+    res["f32"]["l1_after_steps"] = len(dts)
+    log(f"[bench] blocks f32 rel L1 vs f64 after {len(dts)} steps: {res['f32']['rel_L1_vs_fp64_per_field']}")
     if not args.no_cpu_baseline:
         if ROOT not in sys.path:
             sys.path.insert(0, ROOT)
@@ -160,6 +219,48 @@ def blocks_leg(V, args):
                                          f"({len(sel)} blocks, {sub.ncell} cells) of the same block table, "
                                          f"serial block oracle, {el:.2f} s"}
         log(f"[bench] blocks CPU {sub.ncell / el / 1e6:.2f} M zone-updates/s")
+    return res
+
+
+def sod_leg(V, args):
+    """Sod shock tube, 400 cells on [0,1], γ = 1.4, outflow, CFL 0.4, to t = 0.2
+    (src/problem.jl:61-71): FP64 (exact profile) and the FP32 form on the GPU,
+    L1(ρ) of each against the exact Riemann solution, the FP32 form's relative
+    L1 per field against FP64, and the FP64 run against the oracle's."""
+    import numpy as np
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    prob = V.sod_problem(400)
+    x = (np.arange(400) + 0.5) / 400
+    exact = np.array([O.exact_riemann_sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4, (xi - 0.5) / 0.2)[0]
+                      for xi in x])
+    res = {"workload": "Sod 400 cells, gamma 1.4, outflow, CFL 0.4, t = 0.2"}
+    ends = {}
+    for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
+        sim = V.Simulation(V.UniformMesh(prob.dims, prob.domain), prob, eltype=dtype)
+        sim.set_mode("exact")
+        U0 = sim.state().copy()
+        t0 = time.perf_counter()
+        V.evolve(sim)
+        el = time.perf_counter() - t0
+        U = sim.state().astype(np.float64)
+        ends[key] = U
+        res[key] = {"steps": sim.step_count, "t": sim.t, "wall_s": el,
+                    "l1_rho_vs_exact": float(np.abs(U[0] - exact).mean())}
+        if key == "f64":
+            g = O.make_grid((400,), ((0.0, 1.0),), "outflow", 1.4)
+            Uo = np.ascontiguousarray(U0.copy())
+            to, ko = O.evolve(g, Uo, 0.2, 0.4)
+            res[key]["oracle_bitwise"] = bool(np.array_equal(Uo, sim.state()) and ko == sim.step_count and
+                                              to == sim.t)
+        sim.close()
+    den = np.abs(ends["f64"]).mean(axis=1)
+    res["f32"]["rel_L1_vs_fp64_per_field"] = dict(zip(
+        ["density", "momentum_x", "momentum_y", "momentum_z", "energy"],
+        (np.abs(ends["f32"] - ends["f64"]).mean(axis=1) / np.maximum(den, 1e-300)).tolist()))
+    log(f"[bench] Sod-400: L1 f64 {res['f64']['l1_rho_vs_exact']:.4e}, f32 {res['f32']['l1_rho_vs_exact']:.4e}, "
+        f"oracle bitwise {res['f64']['oracle_bitwise']}")
     return res
 
 
@@ -216,13 +317,20 @@ def adjoint_leg(V, args):
         return float(np.sum((sim.state()[0] - target) ** 2))
 
     rng = np.random.default_rng(2024)
-    fd = [{"adjoint": 0.0, "fd": 0.0, "rel": 0.0}] if args.fd_directions <= 0 else []
-    for _ in range(args.fd_directions):
-        v = rng.normal(size=U0.shape)
-        v /= np.linalg.norm(v)
-        ad = float(np.sum(G * v))
-        f = (loss_of(U0 + 1e-6 * v) - loss_of(U0 - 1e-6 * v)) / 2e-6
-        fd.append({"adjoint": ad, "fd": f, "rel": abs(ad - f) / max(abs(f), 1e-300)})
+    dirs = [rng.normal(size=U0.shape) for _ in range(max(args.fd_directions, 0))]
+    dirs = [v / np.linalg.norm(v) for v in dirs]
+    fd = [{"adjoint": 0.0, "fd": 0.0, "rel": 0.0}] if not dirs else []
+    sweep = {}
+    for eps in (1e-4, 1e-5, 1e-6, 1e-7):
+        rels = []
+        for v in dirs:
+            ad = float(np.sum(G * v))
+            f = (loss_of(U0 + eps * v) - loss_of(U0 - eps * v)) / (2 * eps)
+            rels.append(abs(ad - f) / max(abs(f), 1e-300))
+            if eps == 1e-6:
+                fd.append({"adjoint": ad, "fd": f, "rel": rels[-1]})
+        if rels:
+            sweep[f"{eps:g}"] = {"max_rel": max(rels), "median_rel": float(np.median(rels))}
     ro.close()
     sim.close()
     res = {"workload": f"KH {n}^2 periodic, gamma=1.4, {nst} steps CFL 0.4, checkpoints every 20, FP64",
@@ -230,7 +338,7 @@ def adjoint_leg(V, args):
            "ratio_backward_over_forward": info["backward_ms"] / info["forward_ms"],
            "device_bytes_rollout": info["device_bytes"], "loss": loss0,
            "fd_eps": 1e-6, "fd_max_rel": max(d["rel"] for d in fd), "fd_median_rel": float(np.median([d["rel"] for d in fd])),
-           "fd": fd}
+           "fd": fd, "fd_eps_sweep": sweep}
     log(f"[bench] adjoint: fwd {info['forward_ms']:.2f} ms, bwd {info['backward_ms']:.2f} ms, "
         f"ratio {res['ratio_backward_over_forward']:.2f}, FD max rel {res['fd_max_rel']:.2e}")
     # GPU gradient vs the oracle's chained VJPs (same schedule) at 128^2
@@ -266,6 +374,20 @@ def adjoint_leg(V, args):
                                  "max_rel_diff": float(np.abs(G2 - wr).max() / np.abs(wr).max()),
                                  "oracle_seconds": el}
     log(f"[bench] adjoint vs oracle at {m}^2: {res['vs_oracle_gradient']['max_rel_diff']:.2e}")
+    # the transpose identity <G, v> = <w, J v> with J v from the oracle's forward
+    # mode (dual numbers, the reverse mode's branch rules) on the same schedule
+    nd = O.jvp_max_directions()
+    Vd = np.random.default_rng(11).normal(size=(nd,) + Us.shape)
+    t0 = time.perf_counter()
+    Un, JV = O.rollout_jvp(g, Us, d2, Vd, threads=baseline_threads())
+    el = time.perf_counter() - t0
+    lhs = Vd.reshape(nd, -1) @ G2.ravel()
+    rhs = JV.reshape(nd, -1) @ w2.ravel()
+    res["transpose_identity"] = {"grid": [m, m], "steps": nst, "directions": nd,
+                                 "max_rel_gap": float(np.abs(lhs - rhs).max() / np.abs(rhs).max()),
+                                 "forward_mode_states_bitwise": bool(np.array_equal(Un, UF2)),
+                                 "oracle_seconds": el}
+    log(f"[bench] adjoint transpose identity at {m}^2: {res['transpose_identity']['max_rel_gap']:.2e}")
     return res
 
 
@@ -473,6 +595,7 @@ def main():
     ap.add_argument("--no-other-mode", action="store_true",
                     help="skip the FP64 run in the other arithmetic profile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sod", action="store_true", help="skip the Sod-400 leg")
     ap.add_argument("--no-blocks", action="store_true", help="skip the batched-blocks leg")
     ap.add_argument("--skip-uniform", action="store_true",
                     help="skip the 256^3 legs (profiling runs of the other legs only; no headline value)")
@@ -543,11 +666,16 @@ def main():
         U64_end = sim.state() if not (args.no_fp32 and args.no_other_mode) else None
         sim.close()
 
-        traffic = None
+        traffic, traffic_src = None, None
         tfile = os.path.join(ROOT, "profiles", "traffic_rk_stage2_256cubed_f64.json")
-        if os.path.exists(tfile) and n == 256:
+        if os.path.exists(tfile) and n == 256 and args.mode == "fast" and args.tile < 0 and args.zchunk == 0:
             try:
-                traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+                rec = json.load(open(tfile))
+                # counter bytes of a rocprofv3 --pmc pass over this kernel; reported only
+                # while the kernel sources still hash to what was profiled
+                if rec.get("stage_sources_sha256") == stage_sources_hash():
+                    traffic = rec.get("hbm_bytes_per_launch")
+                    traffic_src = "profiles/traffic_rk_stage2_256cubed_f64.json"
             except (ValueError, OSError):
                 traffic = None
 
@@ -572,6 +700,7 @@ def main():
                        "zchunk": launch["zchunk"], "workgroups_per_stage": launch["workgroups"]},
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                         "traffic_source": traffic_src,
                          "kernel": "rk_stage_kernel<double,3,...,STAGE2=true>",
                          "bytes_per_launch": stage2_bytes, "mean_launch_ms": ev["stage2_ms"]},
             "step_detail": {"stage1_ms": ev["stage1_ms"], "stage2_ms": ev["stage2_ms"],
@@ -626,6 +755,8 @@ def main():
         else:
             out["cpu_baseline"] = None
 
+    if not args.no_sod:
+        out["sod"] = sod_leg(V, args)
     if not args.no_blocks:
         out["batched_blocks"] = blocks_leg(V, args)
     if not args.no_adjoint:
