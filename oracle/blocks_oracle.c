@@ -275,3 +275,6 @@ int bo_validate(const bo_mesh* m) {
 #undef BSUF
 #undef BSQRT
 #undef BFABS
+
+/* Self-gravity on block meshes (composite Poisson operator, CG, g, source). */
+#include "blocks_gravity.c"

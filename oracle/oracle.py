@@ -405,6 +405,23 @@ class BlockOracle:
                                                            C.c_int, C.c_int, C.POINTER(C.c_double)]
                 getattr(L, f"bo_evolve_{suf}").restype = C.c_int64
                 getattr(L, f"bo_indicator_{suf}").argtypes = [vp, vp, vp, vp]
+            L.bo_poisson_create.restype = vp
+            L.bo_poisson_create.argtypes = [vp]
+            L.bo_poisson_destroy.argtypes = [vp]
+            L.bo_poisson_nnz.argtypes = [vp]
+            L.bo_poisson_nnz.restype = C.c_int64
+            L.bo_poisson_export.argtypes = [vp] * 7
+            for suf in ("f64", "f32"):
+                getattr(L, f"bo_poisson_apply_{suf}").argtypes = [vp, vp, vp]
+                getattr(L, f"bo_poisson_rhs_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, vp]
+                getattr(L, f"bo_poisson_solve_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, C.c_int64, C.c_double,
+                                                                  vp, vp, C.POINTER(C.c_double)]
+                getattr(L, f"bo_poisson_solve_{suf}").restype = C.c_int64
+                getattr(L, f"bo_grav_accel_{suf}").argtypes = [vp, vp, vp, vp]
+                getattr(L, f"bo_step_grav_{suf}").argtypes = [vp, vp, vp, C.c_double, vp, C.c_int, vp]
+                getattr(L, f"bo_step_grav_{suf}").restype = C.c_int
+                getattr(L, f"bo_compute_dt_grav_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, C.c_double]
+                getattr(L, f"bo_compute_dt_grav_{suf}").restype = C.c_double
             L._bo_typed = True
         self.L = L
         self.mesh = mesh
@@ -420,9 +437,72 @@ class BlockOracle:
         self.ncell = int(L.bo_ncell(self.h))
 
     def __del__(self):
+        if getattr(self, "_poisson", None):
+            self.L.bo_poisson_destroy(self._poisson)
+            self._poisson = None
         if getattr(self, "h", None):
             self.L.bo_destroy(self.h)
             self.h = None
+
+    # ---- self-gravity on the block mesh (blocks_gravity.c) ----
+    def _P(self):
+        if not getattr(self, "_poisson", None):
+            self._poisson = self.L.bo_poisson_create(self.h)
+        return self._poisson
+
+    def poisson_csr(self):
+        """(rowptr, colidx, coef, glo, ghi, gsum) of the composite operator and
+        the gravity stencil ([cell][axis] for the last three)."""
+        P = self._P()
+        n, nnz = self.ncell, int(self.L.bo_poisson_nnz(P))
+        rp, ci = np.empty(n + 1, np.int64), np.empty(nnz, np.int64)
+        cf = np.empty(nnz)
+        glo, ghi = np.empty((n, 3), np.int64), np.empty((n, 3), np.int64)
+        gs = np.empty((n, 3))
+        self.L.bo_poisson_export(P, *[a.ctypes.data for a in (rp, ci, cf, glo, ghi, gs)])
+        return rp, ci, cf, glo, ghi, gs
+
+    def poisson_apply(self, x):
+        x = np.ascontiguousarray(x)
+        out = np.empty_like(x)
+        getattr(self.L, f"bo_poisson_apply_{_suf(x.dtype)}")(self._P(), x.ctypes.data, out.ctypes.data)
+        return out
+
+    def poisson_rhs(self, U, G, project_mean):
+        U = np.ascontiguousarray(U)
+        b = np.empty(self.ncell, dtype=U.dtype)
+        getattr(self.L, f"bo_poisson_rhs_{_suf(U.dtype)}")(self._P(), U.ctypes.data, float(G), int(project_mean),
+                                                           b.ctypes.data)
+        return b
+
+    def poisson_solve(self, U, G, project_mean, maxiter, tol, phi):
+        """CG from the warm start phi (updated in place); returns (iters, relres)."""
+        U = np.ascontiguousarray(U)
+        assert phi.flags.c_contiguous and phi.dtype == U.dtype
+        work = np.empty(4 * self.ncell, dtype=U.dtype)
+        rr = C.c_double()
+        it = getattr(self.L, f"bo_poisson_solve_{_suf(U.dtype)}")(self._P(), U.ctypes.data, float(G),
+                                                                  int(project_mean), int(maxiter), float(tol),
+                                                                  phi.ctypes.data, work.ctypes.data, C.byref(rr))
+        return int(it), rr.value
+
+    def grav_accel(self, phi):
+        phi = np.ascontiguousarray(phi)
+        g = np.empty(3 * self.ncell, dtype=phi.dtype)
+        getattr(self.L, f"bo_grav_accel_{_suf(phi.dtype)}")(self.h, self._P(), phi.ctypes.data, g.ctypes.data)
+        return g.reshape(3, self.ncell)
+
+    def step_grav(self, U, dt, phi, floor64=False):
+        assert U.flags.c_contiguous
+        phi = np.ascontiguousarray(phi, dtype=U.dtype)
+        work = np.empty(10 * self.ncell, dtype=U.dtype)
+        getattr(self.L, f"bo_step_grav_{_suf(U.dtype)}")(self.h, self._P(), U.ctypes.data, float(dt), work.ctypes.data,
+                                                         int(floor64), phi.ctypes.data)
+
+    def compute_dt_grav(self, U, cfl, G, floor64=False):
+        U = np.ascontiguousarray(U)
+        return getattr(self.L, f"bo_compute_dt_grav_{_suf(U.dtype)}")(self.h, U.ctypes.data, float(cfl),
+                                                                     int(floor64), float(G))
 
     def validate(self) -> int:
         return int(self.L.bo_validate(self.h))
