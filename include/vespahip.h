@@ -32,7 +32,10 @@ extern "C" {
 #endif
 
 enum { VH_F64 = 0, VH_F32 = 1 };
-enum { VH_BC_PERIODIC = 0, VH_BC_OUTFLOW = 1, VH_BC_REFLECTING = 2 };
+/* Boundary codes. VH_BC_PARENT is ParentGhost (lib/MeshInterface/src/MeshInterface.jl:86-99,
+ * src/driver.jl:127-134), uniform grids only: the side's boundary ghost comes from the
+ * caller (vh_grid_set_parent_ghost). */
+enum { VH_BC_PERIODIC = 0, VH_BC_OUTFLOW = 1, VH_BC_REFLECTING = 2, VH_BC_PARENT = 3 };
 /* Arithmetic profile of a grid context's stage kernels:
  *   VH_EXACT  the reference's evaluation order, no FMA contraction, correctly
  *             rounded division and square root: bit-identical to the reference
@@ -113,9 +116,9 @@ int vh_buffer_copy(void* ddst, const void* dsrc, int64_t bytes);
 typedef struct vh_grid vh_grid;
 
 /* Create a context over a UniformMesh(dims, domain) (lib/RefMesh/src/RefMesh.jl:95-100)
- * of rank 1..3. dims[d], lo[d], hi[d] for d < rank; bc[2*d + side] (side 0 lo, 1 hi).
- * Periodic must be set on both sides of an axis, and a periodic axis needs >= 2
- * cells. Allocates the state and the stage buffer on the current device. */
+ * of rank 1..3. dims[d], lo[d], hi[d] for d < rank; bc[2*d + side] (side 0 lo, 1 hi,
+ * VH_BC_* codes). Periodic must be set on both sides of an axis, and a periodic axis
+ * needs >= 2 cells. Allocates the state and the stage buffer on the current device. */
 int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const double* lo,
                    const double* hi, const int32_t* bc, double gamma, int32_t dtype);
 int vh_grid_destroy(vh_grid* g);
@@ -152,6 +155,20 @@ int vh_grid_step(vh_grid* g, double dt);
  * register accumulates; the caller zeroes it. Returns after the step has
  * completed, so the register can be read at once. */
 int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux);
+
+/* ParentGhost states (the closure ghost(axis, side, cell) of a ParentGhost side,
+ * src/driver.jl:127-134): the primitive state (ρ, vx, vy, vz, p) at the state's
+ * precision of every boundary face, 5 values per face in the boundary-flux
+ * register's slot order (axis 0 lo plane, axis 0 hi, axis 1 lo, ...; each plane
+ * column-major over the other axes; vh_grid_bflux_count faces in all, entries of
+ * non-ParentGhost planes are not read). A ParentGhost side takes that state as
+ * the outer neighbour of its boundary cells' slopes and as the outer state of its
+ * boundary faces, in both RK stages of every step until the next call (the
+ * reference's closure is evaluated per use; a caller whose parent data change
+ * between steps uploads them before each step). Steps of a grid with a
+ * ParentGhost side fail until this has been called; the reverse mode refuses
+ * such grids. Synchronises. */
+int vh_grid_set_parent_ghost(vh_grid* g, const void* host);
 int64_t vh_grid_bflux_count(const vh_grid* g);
 
 /* evolve!-style loop with the CFL dt computed on the device each step
@@ -286,6 +303,16 @@ int vh_blocks_describe(int32_t rank, int32_t nblocks, int32_t maxlevel, const do
                        const int32_t* level, const int32_t* dims, const int64_t* origin,
                        const int32_t* bc, int32_t zchunk, int64_t* counts6);
 
+/* A 64-bit digest (FNV-1a over the bytes) of every table the builder derives
+ * for a mesh (block records, representative and link-cell maps, link faces and
+ * records, register and tile lists), built with nthreads host threads (0: the
+ * builder's default, 1: serial). For checking that builds agree: the tables do
+ * not depend on the thread count. No device needed. (No counterpart in the
+ * reference: a check of this library's parallel builder.) */
+int vh_blocks_table_digest(int32_t rank, int32_t nblocks, int32_t maxlevel, const double* h0, const int32_t* level,
+                           const int32_t* dims, const int64_t* origin, const int32_t* bc, int32_t zchunk,
+                           int32_t nthreads, uint64_t* digest);
+
 /* Create a block-mesh context (the HGBackend Simulation, src/driver.jl:27-79
  * with lib/HGBackend/src/HGBackend.jl:300-340) on the current device. */
 int vh_blocks_create(vh_blocks** out, int32_t rank, int32_t nblocks, int32_t maxlevel,
@@ -346,6 +373,54 @@ int vh_blocks_indicator(vh_blocks* m, double* block_max, double* cell_g);
  * is replaced; returns after the copy has completed. */
 int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_block);
 
+/* vh_blocks_regrid: regrid! in place (src/driver.jl:440-462 with the remap of
+ * refine!, lib/RefMesh/src/RefMesh.jl:284-302): the context takes the new block
+ * table (nblocks, level, dims, origin, bc as for vh_blocks_create; same rank,
+ * maxlevel, h0 and gamma) and its state is carried over by the injection of
+ * vh_blocks_remap (src_block[b] = the block of the current mesh containing new
+ * block b). The stream, dtype, arithmetic profile, launch shape and clock stay;
+ * the tables are built by the host builder's threads and uploaded in one copy
+ * into a second device arena, and the state buffers are reused when large enough
+ * (grown by a quarter over the need otherwise), so a regrid allocates only when
+ * the mesh outgrows every earlier one. With self-gravity on, the operator is
+ * rebuilt for the new mesh and φ is carried over by the same injection (the next
+ * solve's warm start). Invalid tables leave the context unchanged; a HIP error
+ * after that leaves it fit only for vh_blocks_destroy. Returns after the new
+ * state is in place. */
+int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const int32_t* dims,
+                     const int64_t* origin, const int32_t* bc, const int32_t* src_block);
+
+/* Self-gravity on a block mesh (src/gravity.jl; the composite operator of
+ * _build_poisson_ka_struct, src/gravity.jl:455-600, and its device CG,
+ * ext/VespaKAGravityExt.jl:21-115). The operator A = −∇²·V is built on the host
+ * from the block face walk as CSR rows (one edge per face, coef = area/d with
+ * d = ½(w_left + w_right) along the axis; coarse↔fine faces once from the fine
+ * side with the fine area). φ's boundaries are the mesh's own: periodic block
+ * sides wrap, other uncovered sides are homogeneous Neumann. With gravity on,
+ * every RK stage adds the source (0, −ρg·V, −(ρv·g)·V) of g = −∇φ (held over
+ * the step) to the flux sum, the CFL dt folds in the free-fall limiter
+ * √(4πGρ)/½ (over the leaves of the level asked for), and vh_blocks_run solves
+ * the Poisson equation before each root step, g held over every substep when
+ * subcycling (src/driver.jl:498,589-592); vh_blocks_step uses the current φ.
+ *
+ * enable_gravity!(sim; G, maxiter, tol, project_mean) (src/gravity.jl:63-80):
+ * builds the operator, allocates φ (zero) and the CG vectors; rebuilds on a
+ * second call. Synchronises. */
+int vh_blocks_enable_gravity(vh_blocks* m, double G, int64_t maxiter, double tol, int32_t project_mean);
+int vh_blocks_disable_gravity(vh_blocks* m);
+/* solve_poisson! (src/gravity.jl:248-322): CG warm-started from φ, Float64
+ * dots, mean projection when enabled. Synchronises. */
+int vh_blocks_solve_poisson(vh_blocks* m, int64_t* iters, double* relres);
+/* φ (ncell values of the state dtype, leaf order) to / from host memory. */
+int vh_blocks_phi_upload(vh_blocks* m, const void* host);
+int vh_blocks_phi_download(vh_blocks* m, void* host);
+/* out = A·x on device arrays of ncell values (Float64 row sums in CSR order,
+ * _lap_csr_cpu!, src/gravity.jl:326-336). Synchronises. */
+int vh_blocks_apply_laplacian(vh_blocks* m, const void* dx, void* dout);
+/* g = −∇φ of the current φ for every leaf (_gax, src/gravity.jl:403-410): 3·ncell
+ * values (g_x, g_y, g_z fields) into device memory. Synchronises. */
+int vh_blocks_gravity_accel(vh_blocks* m, void* dg);
+
 /* Time nsteps single-rate steps (device dt) with HIP events after two untimed
  * steps. method 0: one persistent launch per stage, eager; 1: one launch per
  * block per stage (the comparison loop); 2: persistent launches replayed from a
@@ -360,8 +435,8 @@ int vh_blocks_time_steps(vh_blocks* m, double cfl, int64_t nsteps, int32_t metho
  * ghosted 3D grid: dims = cells per axis including ng ghost zones on each side (the
  * active region is ng .. dims-ng-1 per axis), column-major (x fastest), state SoA
  * U[f*ncell + cell] with f = D, S1, S2, S3, τ (τ = ρ·etot). Ghost zones are not
- * refilled by a step (the reference's bc! = nothing); vh_muscl_fill_periodic is
- * fill_periodic!. Arithmetic at the state's precision, bit-identical to the reference's
+ * refilled by a step unless a bc! is set (vh_muscl_set_bc); vh_muscl_fill_periodic
+ * is fill_periodic!. Arithmetic at the state's precision, bit-identical to the reference's
  * operation order. The dual-energy field is optional (vh_muscl_enable_dual_energy).
  */
 typedef struct vh_muscl vh_muscl;
@@ -406,8 +481,25 @@ int vh_muscl_fluxrec_download(vh_muscl* m, void* frec);
  * values per axis); interior (may be NULL) the hi faces of active cells 1 .. act_a−1,
  * [field][u2][u1][c] (5·(act_a−1)·act_b1·act_b2 values per axis). */
 int vh_muscl_flux_register(vh_muscl* m, double dt, double* planes, double* interior);
-/* fill_periodic! (lib/PPMKernels/src/muscl_grid.jl:644-670) on the five fields. */
+/* fill_periodic! (lib/PPMKernels/src/muscl_grid.jl:644-670) on the state's fields. */
 int vh_muscl_fill_periodic(vh_muscl* m);
+
+/* bc! of muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:316-322, 329, 339, 350):
+ * refill the ghost zones before each RK stage. kind VH_MUSCL_BC_NONE is the
+ * reference's bc! = nothing (the default); VH_MUSCL_BC_PERIODIC runs
+ * fill_periodic! on the device before each stage, with no host round trip;
+ * VH_MUSCL_BC_CALLBACK calls fn(user, state, stream, stage) on the host before
+ * stage 0 and stage 1. `state` is device memory holding the stage's input
+ * (nfields·ncell values, the layout of the state); fn fills its ghost zones in
+ * order on `stream` (a hipStream_t: enqueue a kernel there, or synchronise it
+ * and copy) and returns 0; any other value abandons the step with an error and
+ * leaves the state as it was. As in the reference, the step's combination uses
+ * the input as it was before the first refill (D0 is copied before bcfill!): a
+ * step with bc! copies the input once to a second buffer, refills it there and
+ * refills the stage-1 state in place. */
+typedef int (*vh_muscl_bc_fn)(void* user, void* state, void* stream, int32_t stage);
+enum { VH_MUSCL_BC_NONE = 0, VH_MUSCL_BC_PERIODIC = 1, VH_MUSCL_BC_CALLBACK = 2 };
+int vh_muscl_set_bc(vh_muscl* m, int32_t kind, vh_muscl_bc_fn fn, void* user);
 /* max_wavespeed (lib/PPMKernels/src/muscl_grid.jl:672-695) over every cell. */
 int vh_muscl_max_wavespeed(vh_muscl* m, double* out);
 int vh_muscl_sync(vh_muscl* m);

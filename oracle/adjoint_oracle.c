@@ -226,7 +226,7 @@ static void ad_face_value_vjp(const vo_grid* m, const double* W, int64_t c, int 
 
 /* out = A'(U)ᵀ lam (A = accumulate_flux, src/driver.jl:209-281). */
 int vo_accumulate_vjp_f64(const vo_grid* m, const double* U, const double* lam, double* out) {
-    if (vo_check(m)) return -1;
+    if (vo_check(m) || vo_has_parent(m)) return -1;
     int64_t n = vo_ncell(m);
     double g = m->gamma;
     double* W = (double*)malloc(sizeof(double) * 5 * n);
@@ -284,7 +284,7 @@ int vo_accumulate_vjp_f64(const vo_grid* m, const double* U, const double* lam, 
 
 /* ∂L/∂U for L depending on U2 = step(U, dt) through w = ∂L/∂U2 (dt fixed). */
 int vo_step_vjp_f64(const vo_grid* m, const double* U, double dt, const double* wbar, double* Ubar) {
-    if (vo_check(m)) return -1;
+    if (vo_check(m) || vo_has_parent(m)) return -1;
     int64_t n = vo_ncell(m);
     double invV = dt / (1.0 * vo_cell_volume(m));
     double* U1 = (double*)malloc(sizeof(double) * 5 * n);

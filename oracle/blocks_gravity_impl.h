@@ -98,10 +98,12 @@ void QN(bo_grav_accel)(const bo_mesh* m, const bo_poisson* P, const BT* phi, BT*
     }
 }
 
-/* apply_gravity_source! (src/gravity.jl:412-447) on every leaf. */
-static void QN(bo_grav_source)(const bo_mesh* m, const bo_poisson* P, const BT* phi, const BT* U, BT* acc) {
+/* apply_gravity_source! (src/gravity.jl:412-447) on the leaves of a level
+ * (level < 0: every leaf). */
+static void QN(bo_grav_source)(const bo_mesh* m, const bo_poisson* P, const BT* phi, const BT* U, BT* acc, int level) {
     int64_t n = P->n;
     for (int b = 0; b < m->nblocks; ++b) {
+        if (level >= 0 && m->level[b] != level) continue;
         BT V = (BT)bo_volume(m, b);
         for (int64_t c = m->offset[b]; c < m->offset[b + 1]; ++c) {
             BT g[3];
@@ -117,50 +119,91 @@ static void QN(bo_grav_source)(const bo_mesh* m, const bo_poisson* P, const BT* 
     }
 }
 
-/* step! with gravity on a block mesh, every leaf (src/driver.jl:333-359): the
- * source after each flux sum, φ held over the step. work holds 10·ncell values. */
-int QN(bo_step_grav)(const bo_mesh* m, const bo_poisson* P, BT* U, double dt, BT* work, int floor64, const BT* phi) {
-    int64_t n = m->ncell;
-    BT* u0 = work;
-    BT* acc = work + 5 * n;
-    memcpy(u0, U, sizeof(BT) * 5 * n);
-    BT half = (BT)0.5;
-    for (int stage = 0; stage < 2; ++stage) {
-        QCAT(bo_accumulate, BSUF)(m, U, acc, floor64, NULL, 0);
-        QN(bo_grav_source)(m, P, phi, U, acc);
-        for (int b = 0; b < m->nblocks; ++b) {
-            BT invV = (BT)dt / ((BT)1.0 * (BT)bo_volume(m, b));
-            for (int64_t c = m->offset[b]; c < m->offset[b + 1]; ++c)
-                for (int f = 0; f < 5; ++f) {
-                    int64_t i = f * n + c;
-                    if (stage == 0) U[i] = u0[i] - invV * acc[i];
-                    else U[i] = half * u0[i] + half * (U[i] - invV * acc[i]);
-                }
-        }
-    }
-    return 0;
-}
-
-/* compute_dt over every leaf with the free-fall limiter √(4πGρ)/0.5. */
-double QN(bo_compute_dt_grav)(const bo_mesh* m, const BT* U, double cfl, int floor64, double G) {
+/* The free-fall limiter √(4πGρ)/0.5 over the leaves of a level (_gravity_invdt,
+ * src/gravity.jl:452; compute_dt folds it in, src/driver.jl:186-188). */
+static double QN(bo_grav_invdt)(const bo_mesh* m, const BT* U, int level, int floor64, double G) {
     int64_t n = m->ncell;
     BT gT = (BT)m->gamma;
-    double invdt = 0.0, ginv = 0.0;
+    double ginv = 0.0;
     const double fourpi = 4.0 * 3.141592653589793;
-    for (int b = 0; b < m->nblocks; ++b)
+    for (int b = 0; b < m->nblocks; ++b) {
+        if (level >= 0 && m->level[b] != level) continue;
         for (int64_t c = m->offset[b]; c < m->offset[b + 1]; ++c) {
             BT Uc[5], W[5];
             for (int f = 0; f < 5; ++f) Uc[f] = U[f * n + c];
             QCAT(c2p, BSUF)(Uc, gT, m->gamma, floor64, W);
-            BT cs = BSQRT(gT * W[4] / W[0]);
-            for (int d = 0; d < m->rank; ++d)
-                invdt = vo_jmax_f64_(invdt, (double)(BFABS(W[d + 1]) + cs) / bo_width(m, b, d));
             double rho = (double)W[0];
             ginv = vo_jmax_f64_(ginv, rho > 0.0 ? sqrt(fourpi * G * rho) / 0.5 : 0.0);
         }
-    invdt = vo_jmax_f64_(invdt, ginv);
-    if (invdt == 0.0) return INFINITY;
-    return cfl / invdt;
+    }
+    return ginv;
+}
+
+/* The gravity hook of the block stepping: φ held over a (root) step; root()
+ * re-solves the Poisson equation warm-started from φ. */
+typedef struct {
+    const bo_mesh* m;
+    const bo_poisson* P;
+    BT* phi;
+    double G, tol;
+    int64_t maxiter;
+    int project_mean;
+    BT* work;          /* 4n: the CG vectors */
+    int64_t solves, iters;
+    double worst_relres;
+} QN(bo_gctx);
+
+static void QN(bo_ghook_add)(const void* ctx, const BT* U, BT* acc, int level) {
+    const QN(bo_gctx)* g = (const QN(bo_gctx)*)ctx;
+    QN(bo_grav_source)(g->m, g->P, g->phi, U, acc, level);
+}
+
+static double QN(bo_ghook_invdt)(const void* ctx, const BT* U, int level, int fl64) {
+    const QN(bo_gctx)* g = (const QN(bo_gctx)*)ctx;
+    return QN(bo_grav_invdt)(g->m, U, level, fl64, g->G);
+}
+
+static void QN(bo_ghook_root)(const void* ctx, const BT* U) {
+    QN(bo_gctx)* g = (QN(bo_gctx)*)ctx;
+    double rr = 0.0;
+    g->iters += QN(bo_poisson_solve)(g->P, U, g->G, g->project_mean, g->maxiter, g->tol, g->phi, g->work, &rr);
+    g->solves += 1;
+    if (rr > g->worst_relres) g->worst_relres = rr;
+}
+
+/* step! with gravity on a block mesh, every leaf (src/driver.jl:333-359): the
+ * source after each flux sum, φ held over the step. work holds 10·ncell values. */
+int QN(bo_step_grav)(const bo_mesh* m, const bo_poisson* P, BT* U, double dt, BT* work, int floor64, const BT* phi) {
+    QN(bo_gctx) g = {m, P, (BT*)phi, 0.0, 0.0, 0, 0, NULL, 0, 0, 0.0};
+    QCAT(bo_hook, BSUF) hk = {QN(bo_ghook_add), NULL, NULL, &g};
+    QCAT(bo_step_regs, BSUF)(m, U, dt, -1, work, floor64, NULL, NULL, 0, &hk);
+    return 0;
+}
+
+/* compute_dt over every leaf with the free-fall limiter. */
+double QN(bo_compute_dt_grav)(const bo_mesh* m, const BT* U, double cfl, int floor64, double G) {
+    QN(bo_gctx) g = {m, NULL, NULL, G, 0.0, 0, 0, NULL, 0, 0, 0.0};
+    QCAT(bo_hook, BSUF) hk = {NULL, QN(bo_ghook_invdt), NULL, &g};
+    return QCAT(bo_compute_dt_h, BSUF)(m, U, cfl, -1, floor64, &hk);
+}
+
+/* evolve! with self-gravity (src/driver.jl:483-508, 581-605): a Poisson solve
+ * (warm-started from phi) before each root step, g held over it and, when
+ * subcycling, over every fine substep; the source in each stage of the stepped
+ * level; the free-fall limiter in compute_dt of each level. Returns the root
+ * steps taken; *iters (may be NULL) the CG iterations summed over the solves,
+ * *relres (may be NULL) the largest final relative residual. */
+int64_t QN(bo_evolve_grav)(const bo_mesh* m, const bo_poisson* P, BT* U, double tfinal, double cfl, int64_t max_steps,
+                           int subcycle, int floor64, double G, int project_mean, int64_t maxiter, double tol, BT* phi,
+                           double* t_io, int64_t* iters, double* relres) {
+    BT* work = (BT*)malloc(sizeof(BT) * 4 * P->n);
+    QN(bo_gctx) g = {m, P, phi, G, tol, maxiter, project_mean, work, 0, 0, 0.0};
+    QCAT(bo_hook, BSUF) hk = {QN(bo_ghook_add), QN(bo_ghook_invdt), QN(bo_ghook_root), &g};
+    int64_t k = QCAT(bo_evolve_h, BSUF)(m, U, tfinal, cfl, max_steps, subcycle, floor64, t_io, &hk);
+    free(work);
+    if (iters) *iters = g.iters;
+    if (relres) *relres = g.worst_relres;
+    return k;
 }
 
 #undef QN

@@ -15,6 +15,18 @@ typedef struct {
     BT scale;
 } BN(bo_reg);
 
+/* A source term hooked into the stages (self-gravity, src/driver.jl:347-375):
+ * add() after each flux sum over the leaves of the stepped level (level < 0:
+ * every leaf); invdt() is the limiter folded into compute_dt of that level;
+ * root() runs before each root step of evolve! (the Poisson solve,
+ * src/driver.jl:498,592). Any member may be NULL. */
+typedef struct {
+    void (*add)(const void* ctx, const BT* U, BT* acc, int level);
+    double (*invdt)(const void* ctx, const BT* U, int level, int fl64);
+    void (*root)(const void* ctx, const BT* U);
+    const void* ctx;
+} BN(bo_hook);
+
 static void BN(bo_getU)(const BT* U, int64_t n, int64_t c, BT out[5]) {
     for (int f = 0; f < 5; ++f) out[f] = U[f * n + c];
 }
@@ -125,7 +137,7 @@ static int BN(bo_in_level)(const bo_mesh* m, int b, int level) { return level < 
 /* step_level! / step!(sim, dt; level) (src/driver.jl:333-378): level < 0 updates
  * every leaf. work: 10*ncell values. */
 static void BN(bo_step_regs)(const bo_mesh* m, BT* U, double dt, int level, BT* work, int fl64, BN(bo_reg) * *regs,
-                             const double* signs, int nregs) {
+                             const double* signs, int nregs, const BN(bo_hook) * hk) {
     int64_t n = m->ncell;
     BT* u0 = work;
     BT* acc = work + 5 * n;
@@ -138,6 +150,7 @@ static void BN(bo_step_regs)(const bo_mesh* m, BT* U, double dt, int level, BT* 
     for (int stage = 0; stage < 2; ++stage) {
         for (int r = 0; r < nregs; ++r) regs[r]->scale = (BT)(signs[r] * (0.5 * dt));
         BN(bo_accumulate)(m, U, acc, fl64, regs, nregs);
+        if (hk && hk->add) hk->add(hk->ctx, U, acc, level);
         for (int b = 0; b < m->nblocks; ++b) {
             if (!BN(bo_in_level)(m, b, level)) continue;
             BT invV = (BT)dt / ((BT)1.0 * (BT)bo_volume(m, b));
@@ -152,12 +165,14 @@ static void BN(bo_step_regs)(const bo_mesh* m, BT* U, double dt, int level, BT* 
 }
 
 int BN(bo_step)(const bo_mesh* m, BT* U, double dt, int level, BT* work, int floor64) {
-    BN(bo_step_regs)(m, U, dt, level, work, floor64, NULL, NULL, 0);
+    BN(bo_step_regs)(m, U, dt, level, work, floor64, NULL, NULL, 0, NULL);
     return 0;
 }
 
-/* compute_dt(sim; level) (src/driver.jl:169-199). */
-double BN(bo_compute_dt)(const bo_mesh* m, const BT* U, double cfl, int level, int floor64) {
+/* compute_dt(sim; level) (src/driver.jl:169-199), the hook's limiter folded in
+ * after the wave-speed maximum. */
+static double BN(bo_compute_dt_h)(const bo_mesh* m, const BT* U, double cfl, int level, int floor64,
+                                  const BN(bo_hook) * hk) {
     int64_t n = m->ncell;
     BT gT = (BT)m->gamma;
     double invdt = 0.0;
@@ -172,8 +187,13 @@ double BN(bo_compute_dt)(const bo_mesh* m, const BT* U, double cfl, int level, i
                 invdt = vo_jmax_f64_(invdt, (double)(BFABS(W[d + 1]) + cs) / bo_width(m, b, d));
         }
     }
+    if (hk && hk->invdt) invdt = vo_jmax_f64_(invdt, hk->invdt(hk->ctx, U, level, floor64));
     if (invdt == 0.0) return INFINITY;
     return cfl / invdt;
+}
+
+double BN(bo_compute_dt)(const bo_mesh* m, const BT* U, double cfl, int level, int floor64) {
+    return BN(bo_compute_dt_h)(m, U, cfl, level, floor64, NULL);
 }
 
 void BN(bo_totals)(const bo_mesh* m, const BT* U, double out[5]) {
@@ -237,8 +257,8 @@ static void BN(bo_reflux_apply)(const bo_mesh* m, BT* U, BN(bo_reg) * R) {
 
 /* evolve_level! (src/driver.jl:553-578). */
 static void BN(bo_evolve_level)(const bo_mesh* m, BT* U, BT* work, int level, double dt_target, double cfl, int fl64,
-                                BN(bo_reg) * parent) {
-    double dt_stable = BN(bo_compute_dt)(m, U, cfl, level, fl64);
+                                BN(bo_reg) * parent, const BN(bo_hook) * hk) {
+    double dt_stable = BN(bo_compute_dt_h)(m, U, cfl, level, fl64, hk);
     if (!isfinite(dt_stable)) return;
     double q = dt_target / dt_stable * (1 - 1e-12);
     int64_t ns = (int64_t)ceil(q);
@@ -261,9 +281,9 @@ static void BN(bo_evolve_level)(const bo_mesh* m, BT* U, BT* work, int level, do
             memset(own.delta, 0, sizeof(BT) * 5 * m->ncell);
             memset(own.has, 0, m->ncell);
         }
-        BN(bo_step_regs)(m, U, dt_sub, level, work, fl64, regs, signs, nregs);
+        BN(bo_step_regs)(m, U, dt_sub, level, work, fl64, regs, signs, nregs, hk);
         if (has_finer) {
-            BN(bo_evolve_level)(m, U, work, level + 1, dt_sub, cfl, fl64, &own);
+            BN(bo_evolve_level)(m, U, work, level + 1, dt_sub, cfl, fl64, &own, hk);
             BN(bo_reflux_apply)(m, U, &own);
         }
     }
@@ -273,19 +293,20 @@ static void BN(bo_evolve_level)(const bo_mesh* m, BT* U, BT* work, int level, do
 
 /* evolve! (src/driver.jl:483-508, 581-605): single rate (global dt over every
  * leaf) or subcycled. Returns the number of root steps taken. */
-int64_t BN(bo_evolve)(const bo_mesh* m, BT* U, double tfinal, double cfl, int64_t max_steps, int subcycle, int fl64,
-                      double* t_io) {
+static int64_t BN(bo_evolve_h)(const bo_mesh* m, BT* U, double tfinal, double cfl, int64_t max_steps, int subcycle,
+                               int fl64, double* t_io, const BN(bo_hook) * hk) {
     BT* work = (BT*)malloc(sizeof(BT) * 10 * m->ncell);
     double t = *t_io;
     int64_t k = 0;
     while (t < tfinal * (1 - 1e-12) && (max_steps <= 0 || k < max_steps)) {
+        if (hk && hk->root) hk->root(hk->ctx, U);
         if (subcycle) {
-            double dt = vo_jmin_f64_(BN(bo_compute_dt)(m, U, cfl, 0, fl64), tfinal - t);
-            BN(bo_evolve_level)(m, U, work, 0, dt, cfl, fl64, NULL);
+            double dt = vo_jmin_f64_(BN(bo_compute_dt_h)(m, U, cfl, 0, fl64, hk), tfinal - t);
+            BN(bo_evolve_level)(m, U, work, 0, dt, cfl, fl64, NULL, hk);
             t += dt;
         } else {
-            double dt = vo_jmin_f64_(BN(bo_compute_dt)(m, U, cfl, -1, fl64), tfinal - t);
-            BN(bo_step_regs)(m, U, dt, -1, work, fl64, NULL, NULL, 0);
+            double dt = vo_jmin_f64_(BN(bo_compute_dt_h)(m, U, cfl, -1, fl64, hk), tfinal - t);
+            BN(bo_step_regs)(m, U, dt, -1, work, fl64, NULL, NULL, 0, hk);
             t += dt;
         }
         ++k;
@@ -293,6 +314,11 @@ int64_t BN(bo_evolve)(const bo_mesh* m, BT* U, double tfinal, double cfl, int64_
     free(work);
     *t_io = t;
     return k;
+}
+
+int64_t BN(bo_evolve)(const bo_mesh* m, BT* U, double tfinal, double cfl, int64_t max_steps, int subcycle, int fl64,
+                      double* t_io) {
+    return BN(bo_evolve_h)(m, U, tfinal, cfl, max_steps, subcycle, fl64, t_io, NULL);
 }
 
 #undef BN

@@ -9,10 +9,14 @@
 #include <math.h>
 #include <stdint.h>
 
-enum { VO_BC_PERIODIC = 0, VO_BC_OUTFLOW = 1, VO_BC_REFLECTING = 2 };
+enum { VO_BC_PERIODIC = 0, VO_BC_OUTFLOW = 1, VO_BC_REFLECTING = 2, VO_BC_PARENT = 3 };
 
 /* Uniform mesh description (lib/RefMesh/src/RefMesh.jl:89-100). Axes beyond
- * rank have dims = 1 and take no faces. bc[2*axis + side], side 0 = lo, 1 = hi. */
+ * rank have dims = 1 and take no faces. bc[2*axis + side], side 0 = lo, 1 = hi.
+ * VO_BC_PARENT is ParentGhost (lib/MeshInterface/src/MeshInterface.jl:86-99,
+ * src/driver.jl:127-134): the boundary ghost of a face is the primitive state
+ * the closure returns for (axis, side, cell); pg holds those states at the
+ * state's precision, 5 per boundary face in the slots of vo_bface_slot. */
 typedef struct {
     int32_t rank;
     int32_t bc[6];
@@ -20,6 +24,7 @@ typedef struct {
     double lo[3];
     double hi[3];
     double gamma;
+    const void* pg;
 } vo_grid;
 
 static int vo_check(const vo_grid* m) {
@@ -30,9 +35,17 @@ static int vo_check(const vo_grid* m) {
     }
     for (int d = 0; d < m->rank; ++d) {
         int lo = m->bc[2 * d], hi = m->bc[2 * d + 1];
-        if (lo < 0 || lo > 2 || hi < 0 || hi > 2) return 1;
+        if (lo < 0 || lo > 3 || hi < 0 || hi > 3) return 1;
         if ((lo == VO_BC_PERIODIC) != (hi == VO_BC_PERIODIC)) return 1;
+        if ((lo == VO_BC_PARENT || hi == VO_BC_PARENT) && !m->pg) return 1;
     }
+    return 0;
+}
+
+/* The adjoint restatement covers the synthesizing boundary rules only. */
+static int vo_has_parent(const vo_grid* m) {
+    for (int s = 0; s < 2 * m->rank; ++s)
+        if (m->bc[s] == VO_BC_PARENT) return 1;
     return 0;
 }
 

@@ -110,6 +110,7 @@ inline Dual jmax_dual(const Dual& x, const Dual& y) {
 
 #define OIMPL_OWN_MINMAX
 #define OT Dual
+#define OPG double
 #define OSUF dual
 #define OSQRT dsqrt
 #define OFABS dfabs

@@ -167,8 +167,14 @@ static void MN(mu_L)(const MT* U, int64_t N, const int64_t dims[3], int64_t ng, 
  * _dual_eint, Ge advected with the HLL flux ρe·u, and the end-of-step sync
  * (dual_energy_sync!, lib/PPMKernels/src/muscl_grid.jl:97-115). frec (3·nf·N:
  * [axis][field][cell], or NULL) is zeroed and receives ½F per stage. */
-void MN(vo_muscl_step_3d)(MT* U, const int64_t dims[3], int64_t ng, double dt, double gamma, double theta,
-                          double dx, double small_rho, int nf, double eta1, MT* frec) {
+/* muscl_step_3d! with a bc! hook: bc(user, U, stage) refills the ghost zones of
+ * the nf fields before each stage (stage 0: U holds the step's input; stage 1:
+ * the stage-1 state), after the copy U0 of the unfilled input is taken
+ * (lib/PPMKernels/src/muscl_grid.jl:316-322, 329, 339, 350). bc == NULL is
+ * bc! = nothing. */
+void MN(vo_muscl_step_3d_bc)(MT* U, const int64_t dims[3], int64_t ng, double dt, double gamma, double theta,
+                             double dx, double small_rho, int nf, double eta1, MT* frec,
+                             void (*bc)(void* user, MT* U, int stage), void* user) {
     const int64_t N = dims[0] * dims[1] * dims[2];
     MT* U0 = (MT*)malloc(sizeof(MT) * nf * (size_t)N);
     MT* dU = (MT*)malloc(sizeof(MT) * nf * (size_t)N);
@@ -178,8 +184,10 @@ void MN(vo_muscl_step_3d)(MT* U, const int64_t dims[3], int64_t ng, double dt, d
     const MT g = (MT)gamma, e1 = (MT)eta1, sr = (MT)small_rho;
     if (frec)
         for (int64_t i = 0; i < 3 * nf * N; ++i) frec[i] = (MT)0;
+    if (bc) bc(user, U, 0);
     MN(mu_L)(U, N, dims, ng, dtdx, g, (MT)theta, sr, nf, e1, prim, dU, frec, half);
     for (int64_t i = 0; i < nf * N; ++i) U[i] = U0[i] + dU[i];
+    if (bc) bc(user, U, 1);
     MN(mu_L)(U, N, dims, ng, dtdx, g, (MT)theta, sr, nf, e1, prim, dU, frec, half);
     for (int64_t i = 0; i < nf * N; ++i) U[i] = half * (U0[i] + U[i] + dU[i]);
     if (nf == 6)
@@ -193,6 +201,11 @@ void MN(vo_muscl_step_3d)(MT* U, const int64_t dims[3], int64_t ng, double dt, d
     free(U0);
     free(dU);
     free(prim);
+}
+
+void MN(vo_muscl_step_3d)(MT* U, const int64_t dims[3], int64_t ng, double dt, double gamma, double theta,
+                          double dx, double small_rho, int nf, double eta1, MT* frec) {
+    MN(vo_muscl_step_3d_bc)(U, dims, ng, dt, gamma, theta, dx, small_rho, nf, eta1, frec, NULL, NULL);
 }
 
 /* fill_periodic! for one field (lib/PPMKernels/src/muscl_grid.jl:644-670). */

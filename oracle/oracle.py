@@ -19,8 +19,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
-_BC_NAMES = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW, "reflecting": BC_REFLECTING}
+BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING, BC_PARENT = 0, 1, 2, 3
+_BC_NAMES = {"periodic": BC_PERIODIC, "outflow": BC_OUTFLOW, "reflecting": BC_REFLECTING, "parent": BC_PARENT}
 
 
 class Grid(C.Structure):
@@ -31,6 +31,7 @@ class Grid(C.Structure):
         ("lo", C.c_double * 3),
         ("hi", C.c_double * 3),
         ("gamma", C.c_double),
+        ("pg", C.c_void_p),
     ]
 
 
@@ -141,7 +142,33 @@ def make_grid(dims, domain, bcs, gamma) -> Grid:
         g.bc[2 * d] = _BC_NAMES[lo] if isinstance(lo, str) else int(lo)
         g.bc[2 * d + 1] = _BC_NAMES[hi] if isinstance(hi, str) else int(hi)
     g.gamma = float(gamma)
+    g.pg = None
     return g
+
+
+def set_parent_ghost(g: Grid, pg: np.ndarray) -> np.ndarray:
+    """Attach ParentGhost states to g: pg is (nbface, 5) primitive states at
+    the state's precision in the boundary-face slots (axis 0 lo, axis 0 hi,
+    axis 1 lo, ...; each plane column-major over the other axes). Returns the
+    array kept alive for the grid's use."""
+    pg = np.ascontiguousarray(pg)
+    g.pg = pg.ctypes.data
+    g._pg_keep = pg
+    return pg
+
+
+def bface_planes(g: Grid):
+    """(axis, side, first slot, plane shape (other axes, fastest first)) of every
+    boundary plane in slot order."""
+    out, base = [], 0
+    dims = [int(g.dims[d]) for d in range(3)]
+    for d in range(int(g.rank)):
+        other = [dims[e] for e in range(3) if e != d]
+        size = other[0] * other[1]
+        for side in range(2):
+            out.append((d, side, base, tuple(other)))
+            base += size
+    return out
 
 
 def ncell(g: Grid) -> int:
@@ -422,6 +449,11 @@ class BlockOracle:
                 getattr(L, f"bo_step_grav_{suf}").restype = C.c_int
                 getattr(L, f"bo_compute_dt_grav_{suf}").argtypes = [vp, vp, C.c_double, C.c_int, C.c_double]
                 getattr(L, f"bo_compute_dt_grav_{suf}").restype = C.c_double
+                getattr(L, f"bo_evolve_grav_{suf}").argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int64,
+                                                                C.c_int, C.c_int, C.c_double, C.c_int, C.c_int64,
+                                                                C.c_double, vp, C.POINTER(C.c_double),
+                                                                C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+                getattr(L, f"bo_evolve_grav_{suf}").restype = C.c_int64
             L._bo_typed = True
         self.L = L
         self.mesh = mesh
@@ -503,6 +535,21 @@ class BlockOracle:
         U = np.ascontiguousarray(U)
         return getattr(self.L, f"bo_compute_dt_grav_{_suf(U.dtype)}")(self.h, U.ctypes.data, float(cfl),
                                                                      int(floor64), float(G))
+
+    def evolve_grav(self, U, tfinal, cfl, G, phi, max_steps=0, subcycle=False, floor64=False, t0=0.0,
+                    project_mean=True, maxiter=2000, tol=1e-10):
+        """evolve! with self-gravity (blocks_gravity_impl.h bo_evolve_grav): a
+        Poisson solve before each root step, single rate or subcycled. U and
+        phi are updated in place → (t, root steps, CG iterations, worst relres)."""
+        assert U.flags.c_contiguous and phi.flags.c_contiguous and phi.dtype == U.dtype
+        t = C.c_double(t0)
+        it = C.c_int64()
+        rr = C.c_double()
+        k = getattr(self.L, f"bo_evolve_grav_{_suf(U.dtype)}")(
+            self.h, self._P(), U.ctypes.data, float(tfinal), float(cfl), int(max_steps), int(subcycle), int(floor64),
+            float(G), int(project_mean), int(maxiter), float(tol), phi.ctypes.data, C.byref(t), C.byref(it),
+            C.byref(rr))
+        return t.value, int(k), int(it.value), rr.value
 
     def validate(self) -> int:
         return int(self.L.bo_validate(self.h))
@@ -656,6 +703,8 @@ def _muscl_typed():
             getattr(L, f"vo_muscl_flux_line_{suf}").argtypes = [vp] * 5 + [C.c_int64] * 3 + [C.c_double] * 3 + [vp] * 6
             getattr(L, f"vo_muscl_step_3d_{suf}").argtypes = [vp, i64p, C.c_int64] + [C.c_double] * 5 + \
                 [C.c_int, C.c_double, vp]
+            getattr(L, f"vo_muscl_step_3d_bc_{suf}").argtypes = [vp, i64p, C.c_int64] + [C.c_double] * 5 + \
+                [C.c_int, C.c_double, vp, vp, vp]
             getattr(L, f"vo_muscl_fill_periodic_{suf}").argtypes = [vp, i64p, C.c_int64]
             getattr(L, f"vo_muscl_max_wavespeed_{suf}").argtypes = [vp, C.c_int64, C.c_double]
             getattr(L, f"vo_muscl_max_wavespeed_{suf}").restype = C.c_double
@@ -690,6 +739,31 @@ def muscl_step_3d(U, dims, ng, dt, gamma=1.4, theta=1.5, dx=1.0, small_rho=1e-10
     getattr(L, f"vo_muscl_step_3d_{_suf(U.dtype)}")(U.ctypes.data, d, int(ng), float(dt), float(gamma), float(theta),
                                                      float(dx), float(small_rho), nf, float(eta1),
                                                      frec.ctypes.data if record else None)
+    return frec
+
+
+_MU_BC = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_int)
+
+
+def muscl_step_3d_bc(U, dims, ng, dt, bc, gamma=1.4, theta=1.5, dx=1.0, small_rho=1e-10, record=False, eta1=1e-3):
+    """muscl_step_3d! with bc!: bc(V, stage) refills the ghost zones of V, a
+    (nf, N) view of the state, before each stage (muscl_impl.h vo_muscl_step_3d_bc)."""
+    L = _muscl_typed()
+    assert U.flags.c_contiguous and U.shape[0] in (5, 6)
+    nf, N = U.shape
+    d = (C.c_int64 * 3)(*dims)
+    frec = np.empty((3, nf, N), dtype=U.dtype) if record else None
+    ct = C.c_double if U.dtype == np.float64 else C.c_float
+
+    def hook(_user, ptr, stage):
+        V = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(nf, N))
+        bc(V, int(stage))
+
+    cb = _MU_BC(hook)
+    getattr(L, f"vo_muscl_step_3d_bc_{_suf(U.dtype)}")(U.ctypes.data, d, int(ng), float(dt), float(gamma),
+                                                        float(theta), float(dx), float(small_rho), nf, float(eta1),
+                                                        frec.ctypes.data if record else None,
+                                                        C.cast(cb, C.c_void_p), None)
     return frec
 
 

@@ -157,6 +157,21 @@ static inline void ON(ghost)(const OT W[5], int bc, int axis, OT out[5]) {
     if (bc == VO_BC_REFLECTING) out[axis + 1] = -W[axis + 1];
 }
 
+/* _boundary_ghost (src/driver.jl:127-134) at the face on `side` of `axis` of
+ * boundary cell c, given the state W the synthesizing rules start from (the
+ * cell's primitive for a slope, its face value for a face): ParentGhost ignores
+ * W and returns the closure's state for (axis, side, c) (OPG: its element
+ * type); the other rules are ghost_state. */
+static inline void ON(bghost)(const vo_grid* m, const OT W[5], int axis, int side, int64_t c, OT out[5]) {
+    const int bc = m->bc[2 * axis + side];
+    if (bc == VO_BC_PARENT) {
+        const OPG* p = (const OPG*)m->pg + 5 * vo_bface_slot(m, c, axis, side);
+        for (int f = 0; f < 5; ++f) out[f] = (OT)p[f];
+        return;
+    }
+    ON(ghost)(W, bc, axis, out);
+}
+
 /* src/driver.jl:145-159 _plm_slope + src/driver.jl:227-233 _face_value.
  * side: 0 = :lo (h = -½), 1 = :hi (h = +½). */
 static void ON(face_value)(const vo_grid* m, const OT* U, int64_t c, int axis, int side, int fl64,
@@ -171,13 +186,13 @@ static void ON(face_value)(const vo_grid* m, const OT* U, int64_t c, int axis, i
         ON(getU)(U, n, nb, Un);
         ON(c2p)(Un, g, m->gamma, fl64, Wl);
     } else {
-        ON(ghost)(Wc, m->bc[2 * axis], axis, Wl);
+        ON(bghost)(m, Wc, axis, 0, c, Wl);
     }
     if (vo_neighbor(m, c, axis, 1, &nb)) {
         ON(getU)(U, n, nb, Un);
         ON(c2p)(Un, g, m->gamma, fl64, Wr);
     } else {
-        ON(ghost)(Wc, m->bc[2 * axis + 1], axis, Wr);
+        ON(bghost)(m, Wc, axis, 1, c, Wr);
     }
     OT h = side ? (OT)0.5 : (OT)-0.5;
     for (int f = 0; f < 5; ++f) {
@@ -211,7 +226,7 @@ static void ON(accumulate)(const vo_grid* m, const OT* U, OT* acc, int fl64, OT*
                 ON(acc_add)(acc, n, c, F, aT);
                 ON(acc_add)(acc, n, j, F, -aT);
             } else {
-                ON(ghost)(WL, m->bc[2 * d + 1], d, Wg);
+                ON(bghost)(m, WL, d, 1, c, Wg);
                 ON(hllc)(WL, Wg, g, d + 1, F);
                 ON(acc_add)(acc, n, c, F, aT);
                 if (bflux) {
@@ -221,7 +236,7 @@ static void ON(accumulate)(const vo_grid* m, const OT* U, OT* acc, int fl64, OT*
             }
             if (!vo_neighbor(m, c, d, 0, &j)) {
                 ON(face_value)(m, U, c, d, 0, fl64, WR);
-                ON(ghost)(WR, m->bc[2 * d], d, Wg);
+                ON(bghost)(m, WR, d, 0, c, Wg);
                 ON(hllc)(Wg, WR, g, d + 1, F);
                 if (bflux) {
                     OT* b = bflux + 5 * vo_bface_slot(m, c, d, 0);
@@ -350,7 +365,7 @@ static void ON(face_flux_of)(const vo_grid* m, const OT* U, int64_t c, int d, in
             ON(face_value)(m, U, j, d, 0, fl64, WR);
             ON(hllc)(WL, WR, g, d + 1, F);
         } else {
-            ON(ghost)(WL, m->bc[2 * d + 1], d, Wg);
+            ON(bghost)(m, WL, d, 1, c, Wg);
             ON(hllc)(WL, Wg, g, d + 1, F);
         }
     } else {
@@ -361,7 +376,7 @@ static void ON(face_flux_of)(const vo_grid* m, const OT* U, int64_t c, int d, in
             ON(hllc)(WL, WR, g, d + 1, F);
         } else {
             *kind = 1;
-            ON(ghost)(WR, m->bc[2 * d], d, Wg);
+            ON(bghost)(m, WR, d, 0, c, Wg);
             ON(hllc)(Wg, WR, g, d + 1, F);
         }
     }

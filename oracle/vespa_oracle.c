@@ -36,23 +36,27 @@ int64_t vo_bface_count(const vo_grid* m) {
 }
 
 #define OT double
+#define OPG double
 #define OSUF f64
 #define OSQRT sqrt
 #define OFABS fabs
 #include "oracle_impl.h"
 #include "gravity_impl.h"
 #undef OT
+#undef OPG
 #undef OSUF
 #undef OSQRT
 #undef OFABS
 
 #define OT float
+#define OPG float
 #define OSUF f32
 #define OSQRT sqrtf
 #define OFABS fabsf
 #include "oracle_impl.h"
 #include "gravity_impl.h"
 #undef OT
+#undef OPG
 #undef OSUF
 #undef OSQRT
 #undef OFABS

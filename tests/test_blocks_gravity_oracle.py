@@ -196,3 +196,73 @@ def test_gravity_pulls_toward_an_overdensity_on_two_levels(O):
     assert mom_r[(r2 > 0.01) & (r2 < 0.09)].mean() < 0           # infall
     mass = lambda u: float((u[0] * qb.volumes()).sum())
     assert abs(mass(U1) - mass(U)) <= 1e-13 * mass(U)
+
+
+def _blob_state(qb, amp=2.0):
+    x, y = qb.cell_centers()
+    U = np.zeros((5, qb.mesh().ncell))
+    U[0] = 1 + amp * np.exp(-(x * x + y * y) / 0.02)
+    U[4] = 1.0 / (G14 - 1)
+    return U
+
+
+def test_evolve_with_gravity_single_rate_and_subcycled(O):
+    """evolve! with self-gravity (src/driver.jl:483-508, 581-605): a Poisson
+    solve before each root step, the source in the stages of the stepped level,
+    the free-fall limiter in each level's dt.
+
+    * single rate: the loop solve → compute_dt → sourced step, bit for bit;
+    * G = 0: the plain evolve!, single rate and subcycled, bit for bit (the hook
+      adds exact zeros);
+    * one level: subcycling takes one substep of the root dt, so it is the
+      single-rate run bit for bit (the hook at level ≥ 0 is the hook at −1);
+    * two levels in a closed box: mass conserved to round-off (the reflux
+      keeps the coarse↔fine faces conservative), infall toward the blob, and
+      within a few per cent of the single-rate run (g is held over the fine
+      substeps: first order in the coupling)."""
+    qb = two_level_quadtree(tags=((0, 1, 1), (0, 2, 1), (0, 1, 2), (0, 2, 2)), bc=BM.REF)
+    m = qb.mesh()
+    bo = O.BlockOracle(m)
+    U0 = _blob_state(qb)
+    # single rate against the explicit loop
+    Ue, phie = np.ascontiguousarray(U0.copy()), np.zeros(m.ncell)
+    t, k, it, rr = bo.evolve_grav(Ue, 1.0, 0.4, 1.0, phie, max_steps=3)
+    assert k == 3 and it > 0 and rr <= 1e-10
+    Ul, phil, tl = np.ascontiguousarray(U0.copy()), np.zeros(m.ncell), 0.0
+    for _ in range(3):
+        bo.poisson_solve(Ul, 1.0, True, 2000, 1e-10, phil)
+        dt = bo.compute_dt_grav(Ul, 0.4, 1.0)
+        bo.step_grav(Ul, dt, phil)
+        tl += dt
+    assert t == tl and np.array_equal(Ue, Ul) and np.array_equal(phie, phil)
+    # G = 0 is the plain evolve!
+    for sub in (False, True):
+        Ug, phig = np.ascontiguousarray(U0.copy()), np.zeros(m.ncell)
+        Up = np.ascontiguousarray(U0.copy())
+        tg, kg, _, _ = bo.evolve_grav(Ug, 0.02, 0.4, 0.0, phig, subcycle=sub)
+        tp, kp = bo.evolve(Up, 0.02, 0.4, subcycle=sub)
+        assert (tg, kg) == (tp, kp) and np.array_equal(Ug, Up) and not phig.any()
+    # one level: subcycled == single rate
+    q1 = amr.QuadBlocks(4, 8, 1, bc=BM.REF)
+    b1 = O.BlockOracle(q1.mesh())
+    runs = []
+    for sub in (False, True):
+        U, phi = np.ascontiguousarray(_blob_state(q1)), np.zeros(q1.mesh().ncell)
+        runs.append((b1.evolve_grav(U, 0.02, 0.4, 1.0, phi, subcycle=sub)[:2], U))
+    assert runs[0][0] == runs[1][0] and np.array_equal(runs[0][1], runs[1][1])
+    # two levels, subcycled
+    tf = 0.05
+    Us, phis = np.ascontiguousarray(U0.copy()), np.zeros(m.ncell)
+    ts, ks, _, _ = bo.evolve_grav(Us, tf, 0.4, 1.0, phis, subcycle=True)
+    Ur, phir = np.ascontiguousarray(U0.copy()), np.zeros(m.ncell)
+    tr, kr, _, _ = bo.evolve_grav(Ur, tf, 0.4, 1.0, phir)
+    assert abs(ts - tf) < 1e-12 and abs(tr - tf) < 1e-12 and ks < kr    # the root steps take the coarse dt
+    vol = qb.volumes()
+    mass = lambda u: float((u[0] * vol).sum())
+    assert abs(mass(Us) - mass(U0)) <= 1e-13 * mass(U0)
+    x, y = qb.cell_centers()
+    r2 = x * x + y * y
+    mom_r = Us[1] * x + Us[2] * y
+    assert mom_r[(r2 > 0.01) & (r2 < 0.09)].mean() < 0
+    d = float((np.abs(Us[0] - Ur[0]) * vol).sum() / (np.abs(Ur[0] - U0[0]) * vol).sum())
+    assert d < 0.05, d

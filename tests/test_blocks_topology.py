@@ -146,3 +146,32 @@ def test_describe_counts_of_refined_quadtrees(rounds, counts):
     node-based maps (the builder's lookups changed, its tables must not)."""
     m = _ring_quadtree(rounds)
     assert BM.describe(m) == counts
+
+
+@pytest.mark.parametrize("name,zchunk,digest", [
+    ("ring3", 0, 0xf9056b64bf2defc7),
+    ("ring8", 0, 0xff84b41d15663f51),
+    ("pairs300_3d", 0, 0xd9a2bf2a2e96a910),
+    ("pairs300_3d", 8, 0xbe8548bde1bc6ebc),
+    ("pairs1000_2d", 0, 0xf95618b0ccd893a3),
+])
+def test_threaded_builder_tables_equal_serial_ones(name, zchunk, digest):
+    """The builder splits its work over host threads (contiguous block chunks,
+    joined in order). Every derived table must be the serial walk's: the digests
+    were recorded with the earlier single-threaded builder (hash-map lookups),
+    and every thread count reproduces them."""
+    m = {"ring3": lambda: _ring_quadtree(3), "ring8": lambda: _ring_quadtree(8),
+         "pairs300_3d": lambda: BM.random_block_pairs(300, seed=1),
+         "pairs1000_2d": lambda: BM.random_block_pairs(1000, seed=3, rank=2)}[name]()
+    for nt in (1, 2, 3, 8, 0):
+        assert BM.table_digest(m, zchunk, nt) == digest, nt
+
+
+def test_table_digest_sees_a_changed_table():
+    """The digest covers the tile lists (zchunk) and the face walk (one block moved)."""
+    m = BM.random_block_pairs(40, seed=2)
+    assert BM.table_digest(m, 0) != BM.table_digest(m, 8)
+    q = _ring_quadtree(1)
+    q2 = BM.BlockMesh(q.rank, q.maxlevel, q.h0[:q.rank], q.gamma, q.level[::-1].copy(), q.dims[::-1].copy(),
+                      q.origin[::-1].copy(), q.bc[::-1].copy())
+    assert BM.describe(q2) == BM.describe(q) and BM.table_digest(q2) != BM.table_digest(q)

@@ -321,3 +321,107 @@ def test_fast_profile_benchmark_workload(dtype):
             g.close()
     assert np.isfinite(res["fast"]).all()
     assert _rel_err(res["fast"], res["exact"]) <= FAST_TOL[dtype]
+
+
+# ---------------- bc! (ghost refill before each RK stage) ----------------
+
+def _outflow_fill(V, dims, ng):
+    """Zero-gradient ghost zones: every ghost cell copies the nearest active
+    cell (a host bc!; V is (nf, N) or (nf, nz, ny, nx))."""
+    nx, ny, nz = dims
+    A = V.reshape((V.shape[0], nz, ny, nx))
+    ix = np.clip(np.arange(nx), ng, nx - ng - 1)
+    iy = np.clip(np.arange(ny), ng, ny - ng - 1)
+    iz = np.clip(np.arange(nz), ng, nz - ng - 1)
+    A[...] = A[:, iz][:, :, iy][:, :, :, ix].copy()
+
+
+def _periodic_fill(O, V, dims, ng):
+    for q in range(V.shape[0]):
+        O.muscl_fill_periodic(V[q].reshape(-1), dims, ng)
+
+
+@pytest.mark.parametrize("dual", [False, True])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bc_periodic_on_device_bitwise(O, dtype, dual):
+    """bc! = fill_periodic! before each stage, filled by the library on the device,
+    against the oracle's step with the same refill hook: every cell bit for bit
+    over two steps (ghost cells combine the unfilled input, as the reference)."""
+    dims, ng = (22, 19, 13), 3
+    print(f"muscl bc periodic {np.dtype(dtype).name} dual={dual}", flush=True)
+    U = smooth_state(dims, seed=7, dtype=dtype)
+    if dual:
+        U = with_ge(U)
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10, eltype=dtype, dual=dual)
+    try:
+        g.set_bc("periodic")
+        g.upload(U)
+        Uo = np.ascontiguousarray(U.copy())
+        for _ in range(2):
+            g.step(0.02)
+            O.muscl_step_3d_bc(Uo, dims, ng, 0.02, lambda V, s: _periodic_fill(O, V, dims, ng), G14, 1.5, 0.05)
+        assert np.array_equal(g.state(), Uo)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_bc_host_callback_outflow_bitwise(O, dtype):
+    """A host bc! (zero-gradient ghosts) through the C ABI's callback: the library
+    hands the stage's device state to the caller before each stage; against the
+    oracle's step with the same hook, every cell bit for bit. A no-op callback is
+    bc! = nothing; a failing one is an error and leaves the state as it was."""
+    dims, ng = (21, 21, 21), 4
+    print(f"muscl bc host outflow {np.dtype(dtype).name}", flush=True)
+    U = smooth_state(dims, seed=9, dtype=dtype)
+    g = M.MusclGrid(dims, ng, G14, 1.5, 0.05, 1e-10, eltype=dtype)
+    stages = []
+    try:
+        g.set_bc(lambda V, s: (stages.append(s), _outflow_fill(V, dims, ng)))
+        g.upload(U)
+        g.step(0.02, fluxrec=True)
+        Uo = np.ascontiguousarray(U.copy())
+        fo = O.muscl_step_3d_bc(Uo, dims, ng, 0.02, lambda V, s: _outflow_fill(V, dims, ng), G14, 1.5, 0.05,
+                                record=True)
+        assert stages == [0, 1]
+        assert np.array_equal(g.state(), Uo)
+        assert np.array_equal(g.fluxrec(), fo)
+        # no-op callback == no bc!
+        g.upload(U)
+        g.set_bc(lambda V, s: None)
+        g.step(0.02)
+        a = g.state()
+        g.set_bc(None)
+        g.upload(U)
+        g.step(0.02)
+        assert np.array_equal(a, g.state())
+        # a failing callback
+        g.upload(U)
+
+        def bad(V, s):
+            if s == 1:
+                raise RuntimeError("no ghost data")
+        g.set_bc(bad)
+        with pytest.raises(VespaHIPError, match="bc! callback"):
+            g.step(0.02)
+        assert np.array_equal(g.state(), U)
+    finally:
+        g.close()
+
+
+def test_bc_array_form(O):
+    """muscl_step_3d(...; bc=...) on host arrays: bc!(D, S1, S2, S3, Tau)."""
+    dims, ng = (12, 11, 10), 3
+    U = smooth_state(dims, seed=3, dtype=np.float64)
+    f = [u.copy() for u in U]
+    calls = []
+
+    def bc(*fs):
+        calls.append(len(fs))
+        for q in fs:
+            O.muscl_fill_periodic(q, dims, ng)
+    M.muscl_step_3d(*f, dims, ng, dt=0.02, gamma=G14, dx=0.05, bc=bc)
+    Uo = np.ascontiguousarray(U.copy())
+    O.muscl_step_3d_bc(Uo, dims, ng, 0.02, lambda V, s: _periodic_fill(O, V, dims, ng), G14, 1.5, 0.05)
+    assert calls == [5, 5]
+    assert np.array_equal(np.stack(f), Uo)

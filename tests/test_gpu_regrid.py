@@ -193,3 +193,101 @@ def test_sedov_run_with_device_regrid_bitwise(O, dtype):
     assert r["blocks"] == qb.blocks
     assert r["t"] == t and r["steps"] == steps
     assert np.array_equal(r["U"], U)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_regrid_in_place_equals_new_context(O, dtype):
+    """vh_blocks_regrid (regrid! in place) against a fresh context of the new
+    mesh filled by vh_blocks_remap and against the oracle's remap: the same
+    state bit for bit after each of several rounds (the first outgrows the
+    buffers, later small rounds reuse them), and the same steps afterwards, in
+    the profile and launch shape the context had before."""
+    print(f"regrid in place {np.dtype(dtype).name}", flush=True)
+    qb = amr.QuadBlocks(6, 8, 3)
+    U = amr.sedov_quadblocks_state(qb).astype(dtype)
+    x, y = qb.cell_centers()
+    U[0] += (0.5 * np.exp(-(x * x + y * y) / 0.01)).astype(dtype)
+    s = BM.BlockSimulation(qb.mesh(), dtype)
+    s.set_mode("exact")
+    s.set_launch(zchunk=0, max_workgroups=0, use_graph=True)
+    s.upload(U)
+    try:
+        s.evolve(max_steps=2)
+        for r in range(4):
+            Ub, old, old_mesh = s.state(), list(qb.blocks), qb.mesh()
+            tags = qb.tags_from_indicator(s.indicator(), 0.02 if r == 0 else 0.3, symmetric=True)
+            if not tags:
+                tags = {next(b for b in qb.blocks if b[0] < qb.maxlevel)}
+            qb.refine(tags)
+            src = qb.source_blocks(old)
+            ref = BM.BlockSimulation(qb.mesh(), dtype)
+            ref.set_mode("exact")
+            try:
+                ref.remap_from(s, src)
+                s.regrid(qb.mesh(), src)
+                assert s.ncell == ref.ncell == qb.mesh().ncell
+                assert np.array_equal(s.state(), O.blocks_remap(qb.mesh(), old_mesh, src, Ub)), f"round {r}"
+                assert np.array_equal(s.state(), ref.state()), f"round {r}"
+                s.evolve(max_steps=2)
+                ref.evolve(max_steps=2)
+                assert s.t == ref.t
+                assert np.array_equal(s.state(), ref.state()), f"round {r} steps"
+            finally:
+                ref.close()
+    finally:
+        s.close()
+
+
+def test_regrid_in_place_with_gravity_rebuilds_the_operator(O):
+    """With self-gravity on, the regrid rebuilds the composite operator for the
+    new mesh (A·x equals a fresh context's and the oracle's) and carries φ over
+    by the same injection as the state."""
+    print("regrid in place with gravity", flush=True)
+    qb = amr.QuadBlocks(4, 8, 2, bc=BM.REF)
+    x, y = qb.cell_centers()
+    U = np.zeros((5, len(x)))
+    U[0] = 1 + 2.0 * np.exp(-(x * x + y * y) / 0.02)
+    U[4] = 1.0 / 0.4
+    s = BM.BlockSimulation(qb.mesh(), np.float64)
+    try:
+        s.upload(U)
+        s.enable_gravity(G=1.0, maxiter=2000, tol=1e-10)
+        s.solve_poisson()
+        phi0 = s.phi
+        old, old_mesh = list(qb.blocks), qb.mesh()
+        qb.refine({(0, 1, 1), (0, 2, 1), (0, 1, 2), (0, 2, 2)})
+        src = qb.source_blocks(old)
+        s.regrid(qb.mesh(), src)
+        m = qb.mesh()
+        phi_expect = O.blocks_remap(m, old_mesh, src, phi0[None, :])[0]
+        assert np.array_equal(s.phi, phi_expect)
+        xr = np.random.default_rng(3).normal(size=m.ncell)
+        assert np.array_equal(s.apply_laplacian(xr), O.BlockOracle(m).poisson_apply(xr))
+        it, rr = s.solve_poisson()
+        assert rr <= 1e-10
+        s.evolve(max_steps=2)
+    finally:
+        s.close()
+
+
+def test_regrid_rejects_bad_tables_and_keeps_the_context():
+    qb = amr.QuadBlocks(2, 8, 1)
+    s = BM.BlockSimulation(qb.mesh(), np.float64)
+    try:
+        x, y = qb.cell_centers()
+        U0 = np.stack([1 + 0.1 * x, 0.1 * y, 0 * x, 0 * x, 2 + 0 * x])
+        s.upload(U0)
+        old = list(qb.blocks)
+        mesh0 = qb.mesh()
+        qb.refine({qb.blocks[0]})
+        bad = qb.source_blocks(old).copy()
+        bad[0] = 99
+        with pytest.raises(VespaHIPError, match="out of range"):
+            s.regrid(qb.mesh(), bad)
+        wrong = BM.BlockMesh(2, 2, qb.mesh().h0[:2], qb.mesh().gamma, [0], [[8, 8, 1]], [[0, 0, 0]], [[1] * 6])
+        with pytest.raises(ValueError, match="maxlevel"):
+            s.regrid(wrong, [0])
+        assert s.ncell == mesh0.ncell and np.array_equal(s.state(), U0)   # untouched
+        s.evolve(max_steps=1)
+    finally:
+        s.close()

@@ -421,3 +421,96 @@ def test_dual_energy_gas_energy_is_advected_conservatively(O):
         div = sum((frec[a, f, idx + ea[a]] - frec[a, f, idx]) * (dt / dx) for a in range(3))
         assert np.abs((A[f, idx] - A0[f, idx]) + div).max() < 1e-13 * np.abs(A0[f]).max()
     assert A[5].min() > 0
+
+
+# ---------------- bc! (ghost refill before each RK stage) ----------------
+
+def _periodic_field(n, seed=5, nf=5):
+    """A smooth periodic active state of n³ cells (5 or 6 fields)."""
+    c = 2 * np.pi * (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    rho = 1.0 + 0.3 * np.sin(X + 0.2) * np.cos(Y) + 0.1 * np.sin(2 * Z + 0.5)
+    vx, vy, vz = 0.3 * np.cos(Y + Z), 0.2 * np.sin(X - Z), 0.25 * np.cos(X + 0.3)
+    pr = 0.7 + 0.2 * np.cos(X + Y + Z)
+    eint = pr / ((G14 - 1) * rho)
+    etot = eint + 0.5 * (vx ** 2 + vy ** 2 + vz ** 2)
+    f = [rho, rho * vx, rho * vy, rho * vz, rho * etot] + ([rho * eint] if nf == 6 else [])
+    return np.stack(f)
+
+
+def _ghosted(act, ng):
+    """(nf, n, n, n) active fields → (nf, N) with periodic ghost zones."""
+    return np.stack([np.pad(a, ng, mode="wrap") for a in act]).reshape(act.shape[0], -1).copy()
+
+
+def _periodic_bc(O, dims, ng):
+    def bc(V, stage):
+        for q in range(V.shape[0]):
+            O.muscl_fill_periodic(V[q], dims, ng)
+    return bc
+
+
+@pytest.mark.parametrize("nf", [5, 6])
+def test_bc_refills_before_each_stage_as_a_larger_periodic_domain(O, nf):
+    """muscl_step_3d!(...; bc! = fill_periodic!) on an n³ periodic box equals, on
+    the active cells, the step without bc! of a 3×3×3 tiling of the box with
+    its own periodic ghosts (centre tile): a refill before each stage reproduces
+    the periodic neighbourhood exactly; one before the first stage only would
+    leave stale stage-1 ghosts, which the tiling does not have."""
+    n, ng, dt, dx = 8, 3, 0.05, 0.5
+    act = _periodic_field(n, nf=nf)
+    U = _ghosted(act, ng)
+    dims = (n + 2 * ng,) * 3
+    O.muscl_step_3d_bc(U, dims, ng, dt, _periodic_bc(O, dims, ng), G14, 1.5, dx)
+    big = np.stack([np.tile(a, (3, 3, 3)) for a in act])
+    B = _ghosted(big, ng)
+    bdims = (3 * n + 2 * ng,) * 3
+    O.muscl_step_3d(B, bdims, ng, dt, G14, 1.5, dx)
+    small = U.reshape((nf,) + dims[::-1])[:, ng:ng + n, ng:ng + n, ng:ng + n]
+    centre = B.reshape((nf,) + bdims[::-1])[:, ng + n:ng + 2 * n, ng + n:ng + 2 * n, ng + n:ng + 2 * n]
+    assert np.array_equal(small, centre)
+    # a refill before the first stage only differs (the stage-1 ghosts are stale)
+    U2 = _ghosted(act, ng)
+    first_only = _periodic_bc(O, dims, ng)
+    O.muscl_step_3d_bc(U2, dims, ng, dt, lambda V, s: first_only(V, s) if s == 0 else None, G14, 1.5, dx)
+    assert not np.array_equal(U2.reshape((nf,) + dims[::-1])[:, ng:ng + n, ng:ng + n, ng:ng + n], centre)
+
+
+def test_bc_noop_is_bc_nothing_and_ghosts_combine_the_unfilled_input(O):
+    """A bc! that writes nothing reproduces bc! = nothing on every cell. With
+    fill_periodic! as bc!, two inputs that differ only in their ghost zones give
+    the same active cells (both are refilled before each stage) but different
+    ghost cells: the step's combination uses the copy of the input taken before
+    the first refill (muscl_grid.jl copies D0 before bcfill!)."""
+    n, ng, dt, dx = 8, 3, 0.05, 0.5
+    dims = (n + 2 * ng,) * 3
+    U = _ghosted(_periodic_field(n), ng)
+    ref = U.copy()
+    O.muscl_step_3d(ref, dims, ng, dt, G14, 1.5, dx)
+    a = U.copy()
+    O.muscl_step_3d_bc(a, dims, ng, dt, lambda V, s: None, G14, 1.5, dx)
+    assert np.array_equal(a, ref)
+    ghost = np.ones(dims[::-1], dtype=bool)
+    ghost[ng:-ng, ng:-ng, ng:-ng] = False
+    ghost = ghost.reshape(-1)
+    B = U.copy()
+    B[:, ghost] *= np.random.default_rng(2).uniform(0.5, 1.5, size=(5, ghost.sum()))
+    A = U.copy()
+    for V in (A, B):
+        O.muscl_step_3d_bc(V, dims, ng, dt, _periodic_bc(O, dims, ng), G14, 1.5, dx)
+    assert np.array_equal(A[:, ~ghost], B[:, ~ghost])
+    assert (A[:, ghost] != B[:, ghost]).mean() > 0.9
+
+
+def test_bc_periodic_conserves_over_steps(O):
+    n, ng, dt, dx = 12, 3, 0.05, 0.5
+    dims = (n + 2 * ng,) * 3
+    U = _ghosted(_periodic_field(n), ng)
+    t0 = [_total(U[f], dims, ng, dx) for f in range(5)]
+    for _ in range(3):
+        O.muscl_step_3d_bc(U, dims, ng, dt, _periodic_bc(O, dims, ng), G14, 1.5, dx)
+    t1 = [_total(U[f], dims, ng, dx) for f in range(5)]
+    assert abs(t1[0] - t0[0]) <= 1e-13 * abs(t0[0])
+    assert abs(t1[4] - t0[4]) <= 1e-13 * abs(t0[4])
+    for f in (1, 2, 3):
+        assert abs(t1[f] - t0[f]) <= 1e-12

@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("VESPAHIP_LIB") or os.path.join(PKG_DIR, "libvespahip.
 
 VH_F64, VH_F32 = 0, 1
 VH_EXACT, VH_FAST = 0, 1
-BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING = 0, 1, 2
+BC_PERIODIC, BC_OUTFLOW, BC_REFLECTING, BC_PARENT = 0, 1, 2, 3
 
 # Every symbol include/vespahip.h declares (checked by the CPU test suite).
 EXPORTED = (
@@ -31,16 +31,18 @@ EXPORTED = (
     "vh_blocks_set_mode", "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt",
     "vh_blocks_step", "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch",
     "vh_blocks_info", "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling",
-    "vh_blocks_indicator", "vh_blocks_remap",
+    "vh_blocks_indicator", "vh_blocks_remap", "vh_blocks_regrid", "vh_blocks_table_digest",
+    "vh_blocks_enable_gravity", "vh_blocks_disable_gravity", "vh_blocks_solve_poisson", "vh_blocks_phi_upload",
+    "vh_blocks_phi_download", "vh_blocks_apply_laplacian", "vh_blocks_gravity_accel",
     "vh_grid_step_vjp", "vh_adjoint_create", "vh_adjoint_destroy", "vh_adjoint_forward",
     "vh_adjoint_backward", "vh_adjoint_info", "vh_grid_set_adjoint_form",
-    "vh_grid_step_bflux", "vh_grid_bflux_count",
+    "vh_grid_step_bflux", "vh_grid_bflux_count", "vh_grid_set_parent_ghost",
     "vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
     "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel",
     "vh_muscl_create", "vh_muscl_destroy", "vh_muscl_ncell", "vh_muscl_upload", "vh_muscl_download",
     "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
     "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps",
-    "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode",
+    "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode", "vh_muscl_set_bc",
 )
 
 
@@ -133,11 +135,14 @@ def load_library(path: str | None = None):
     L.vh_blocks_time_steps.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, dp, dp]
     L.vh_blocks_indicator.argtypes = [vp, dp, dp]
     L.vh_blocks_remap.argtypes = [vp, vp, i32p]
+    L.vh_blocks_regrid.argtypes = [vp, C.c_int32, i32p, i32p, i64p, i32p, i32p]
+    L.vh_blocks_table_digest.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double), i32p, i32p, i64p,
+                                         i32p, C.c_int32, C.c_int32, C.POINTER(C.c_uint64)]
     for n in ("vh_blocks_describe", "vh_blocks_create", "vh_blocks_destroy", "vh_blocks_set_mode",
               "vh_blocks_upload", "vh_blocks_download", "vh_blocks_compute_dt", "vh_blocks_step",
               "vh_blocks_run", "vh_blocks_totals", "vh_blocks_set_launch", "vh_blocks_info",
               "vh_blocks_sync", "vh_blocks_time_steps", "vh_blocks_set_tiling", "vh_blocks_indicator",
-              "vh_blocks_remap"):
+              "vh_blocks_remap", "vh_blocks_regrid", "vh_blocks_table_digest"):
         getattr(L, n).restype = C.c_int
     L.vh_grid_step_vjp.argtypes = [vp, vp, C.c_double, vp, vp]
     L.vh_adjoint_create.argtypes = [C.POINTER(vp), vp, C.c_int64, C.c_int32]
@@ -148,8 +153,17 @@ def load_library(path: str | None = None):
     L.vh_grid_set_adjoint_form.argtypes = [vp, C.c_int32]
     L.vh_grid_step_bflux.argtypes = [vp, C.c_double, vp]
     L.vh_grid_step_bflux.restype = C.c_int
+    L.vh_grid_set_parent_ghost.argtypes = [vp, vp]
+    L.vh_grid_set_parent_ghost.restype = C.c_int
     L.vh_grid_bflux_count.argtypes = [vp]
     L.vh_grid_bflux_count.restype = C.c_int64
+    L.vh_blocks_enable_gravity.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32]
+    L.vh_blocks_disable_gravity.argtypes = [vp]
+    L.vh_blocks_solve_poisson.argtypes = [vp, i64p, dp]
+    L.vh_blocks_phi_upload.argtypes = [vp, vp]
+    L.vh_blocks_phi_download.argtypes = [vp, vp]
+    L.vh_blocks_apply_laplacian.argtypes = [vp, vp, vp]
+    L.vh_blocks_gravity_accel.argtypes = [vp, vp]
     L.vh_grid_enable_gravity.argtypes = [vp, C.c_double, C.c_int32, C.c_int64, C.c_double, C.c_int32]
     L.vh_grid_disable_gravity.argtypes = [vp]
     L.vh_grid_solve_poisson.argtypes = [vp, i64p, dp]
@@ -175,10 +189,11 @@ def load_library(path: str | None = None):
     L.vh_muscl_flux_register.argtypes = [vp, C.c_double, dp, dp]
     L.vh_muscl_enable_dual_energy.argtypes = [vp, C.c_double]
     L.vh_muscl_set_mode.argtypes = [vp, C.c_int32]
+    L.vh_muscl_set_bc.argtypes = [vp, C.c_int32, vp, vp]
     for n in ("vh_muscl_create", "vh_muscl_destroy", "vh_muscl_upload", "vh_muscl_download",
               "vh_muscl_state_device_ptr", "vh_muscl_set_tiles", "vh_muscl_step", "vh_muscl_fluxrec_download",
               "vh_muscl_fill_periodic", "vh_muscl_max_wavespeed", "vh_muscl_sync", "vh_muscl_time_steps",
-              "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode"):
+              "vh_muscl_flux_register", "vh_muscl_enable_dual_energy", "vh_muscl_set_mode", "vh_muscl_set_bc"):
         getattr(L, n).restype = C.c_int
     for n in ("vh_grid_enable_gravity", "vh_grid_disable_gravity", "vh_grid_solve_poisson", "vh_grid_phi_upload",
               "vh_grid_phi_download", "vh_grid_apply_laplacian", "vh_grid_gravity_accel"):

@@ -12,9 +12,12 @@ a density_gradient_indicator (src/driver.jl:400-414) above `refine_above`, then
 
 The state never leaves the device during a run: the indicator is evaluated by
 a kernel that returns one maximum per block, the host turns those into tags
-and a new block list (the tree is a few thousand entries), the library builds
-the new mesh's tables, and a remap kernel injects each parent cell's conserved
-values into its children device to device, so Σ U·V is unchanged. Between
+and a new block list (the tree is a few thousand entries), and the device
+context regrids in place: the library's threaded builder makes the new mesh's
+tables, uploads them in one copy next to the current ones, and a remap kernel
+injects each parent cell's conserved values into its children device to
+device, so Σ U·V is unchanged; no device memory is allocated unless the mesh
+outgrows every earlier one. Between
 regrids the leaves are advanced by the batched kernel (one persistent launch
 per RK stage).
 """
@@ -212,8 +215,10 @@ def evolve_amr(qb: QuadBlocks, U0: np.ndarray, tfinal: float, cfl: float = 0.4, 
     download. Returns a dict with the final blocks/state, step count, leaf
     counts, the raster density at the requested snapshot times and the wall
     time spent in the regrid phases (ms): indicator kernel + per-block maxima to
-    the host, host tagging and refinement, the new mesh's context (tables), and
-    the remap."""
+    the host, host tagging and refinement (tree edit and the new block table),
+    and the in-place regrid of the device context (tables built by the host
+    builder's threads, uploaded in one copy, and the device remap, all under
+    `tables`)."""
     totals0 = (np.asarray(U0, dtype=np.float64) * qb.volumes()).sum(axis=1)
     sim = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
     sim.set_mode(mode)
@@ -222,7 +227,7 @@ def evolve_amr(qb: QuadBlocks, U0: np.ndarray, tfinal: float, cfl: float = 0.4, 
     snaps = {}
     pending = sorted(snapshots)
     leaves_hist = []
-    phase = {"indicator": 0.0, "refine": 0.0, "tables": 0.0, "remap": 0.0}
+    phase = {"indicator": 0.0, "refine": 0.0, "tables": 0.0}
     nregrid = ncheck = 0
     B = qb.B
     try:
@@ -237,21 +242,12 @@ def evolve_amr(qb: QuadBlocks, U0: np.ndarray, tfinal: float, cfl: float = 0.4, 
                 old = list(qb.blocks)
                 qb.refine(tags)
                 src = qb.source_blocks(old)
+                mesh = qb.mesh()
                 t2 = time.perf_counter()
-                new = BlockSimulation(qb.mesh(), dtype, cfl=cfl)
-                new.set_mode(mode)
+                sim.regrid(mesh, src)   # tables, their upload and the remap, in place
                 t3 = time.perf_counter()
-                try:
-                    new.remap_from(sim, src)
-                except Exception:
-                    new.close()
-                    raise
-                t4 = time.perf_counter()
-                sim.close()
-                sim = new
                 phase["refine"] += t2 - t1
                 phase["tables"] += t3 - t2
-                phase["remap"] += t4 - t3
                 nregrid += 1
             sim.t = t
             stop = tfinal

@@ -282,6 +282,19 @@ def describe(mesh: BlockMesh, zchunk: int = 0) -> dict:
     return dict(zip(COUNT_NAMES, [int(v) for v in out]))
 
 
+def table_digest(mesh: BlockMesh, zchunk: int = 0, nthreads: int = 0) -> int:
+    """Digest of every table the library's builder derives for `mesh`, built
+    with `nthreads` host threads (0: default, 1: serial); no device needed."""
+    import ctypes as C
+    from ._lib import check, load_library
+    L = load_library()
+    h0, lv, dm, og, bc, keep = _mesh_args(mesh)
+    out = C.c_uint64()
+    check(L.vh_blocks_table_digest(mesh.rank, mesh.nblocks, mesh.maxlevel, h0, lv, dm, og, bc, int(zchunk),
+                                   int(nthreads), C.byref(out)))
+    return int(out.value)
+
+
 class BlockSimulation:
     """Device-resident simulation on a block mesh (the HGBackend Simulation,
     src/driver.jl:27-79): one persistent launch per RK stage over every block,
@@ -333,6 +346,65 @@ class BlockSimulation:
     def set_mode(self, mode: str):
         from . import _lib
         self._check(self._lib.vh_blocks_set_mode(self._h, {"exact": _lib.VH_EXACT, "fast": _lib.VH_FAST}[mode]))
+
+    # ---- self-gravity (src/gravity.jl) on the block mesh ----
+    def enable_gravity(self, G: float = 1.0, maxiter: int = 500, tol: float | None = None,
+                       project_mean: bool = True):
+        """enable_gravity!(sim; G, maxiter, tol, project_mean) (src/gravity.jl:63-80):
+        the composite Poisson operator of this mesh, φ = 0. tol defaults to 1e-8
+        (Float64) / 1e-5 (Float32) as the reference's."""
+        if tol is None:
+            tol = 1e-8 if self.eltype == np.float64 else 1e-5
+        self._check(self._lib.vh_blocks_enable_gravity(self._h, float(G), int(maxiter), float(tol),
+                                                       int(bool(project_mean))))
+        self.gravity_G = float(G)
+
+    def disable_gravity(self):
+        self._check(self._lib.vh_blocks_disable_gravity(self._h))
+        self.gravity_G = None
+
+    def solve_poisson(self) -> tuple:
+        """solve_poisson! (src/gravity.jl:248-322) → (iterations, relative residual)."""
+        C = self._C
+        it, rr = C.c_int64(), C.c_double()
+        self._check(self._lib.vh_blocks_solve_poisson(self._h, C.byref(it), C.byref(rr)))
+        return int(it.value), float(rr.value)
+
+    @property
+    def phi(self) -> np.ndarray:
+        out = np.empty(self.ncell, dtype=self.eltype)
+        self._check(self._lib.vh_blocks_phi_download(self._h, out.ctypes.data_as(self._C.c_void_p)))
+        return out
+
+    @phi.setter
+    def phi(self, value):
+        v = np.ascontiguousarray(value, dtype=self.eltype)
+        if v.size != self.ncell:
+            raise ValueError(f"φ must hold {self.ncell} values")
+        self._check(self._lib.vh_blocks_phi_upload(self._h, v.ctypes.data_as(self._C.c_void_p)))
+
+    def apply_laplacian(self, x) -> np.ndarray:
+        """A·x with A = −∇²·V, the composite operator of this mesh."""
+        from ._lib import DeviceBuffer
+        x = np.ascontiguousarray(x, dtype=self.eltype)
+        bx, bo = DeviceBuffer(x.nbytes).upload(x), DeviceBuffer(x.nbytes)
+        try:
+            self._check(self._lib.vh_blocks_apply_laplacian(self._h, bx.ptr, bo.ptr))
+            return bo.download(np.empty_like(x))
+        finally:
+            bx.free()
+            bo.free()
+
+    def gravity_accel(self) -> np.ndarray:
+        """g = −∇φ of the current φ, shape (3, ncell)."""
+        from ._lib import DeviceBuffer
+        out = np.empty((3, self.ncell), dtype=self.eltype)
+        b = DeviceBuffer(out.nbytes)
+        try:
+            self._check(self._lib.vh_blocks_gravity_accel(self._h, b.ptr))
+            return b.download(out)
+        finally:
+            b.free()
 
     def set_launch(self, zchunk: int = 8, max_workgroups: int = 0, use_graph: bool = True):
         self._check(self._lib.vh_blocks_set_launch(self._h, int(zchunk), int(max_workgroups), int(bool(use_graph))))
@@ -402,6 +474,25 @@ class BlockSimulation:
         self._check(self._lib.vh_blocks_remap(self._h, src._h, sb.ctypes.data_as(C.POINTER(C.c_int32))))
         self.t = src.t
         self.step_count = src.step_count
+
+    def regrid(self, mesh: BlockMesh, src_block):
+        """regrid! in place (src/driver.jl:440-462): this simulation moves onto
+        `mesh` (same rank, maxlevel, h0 and gamma), block b taking the cells of
+        current block src_block[b] (the same block or its ancestor) by injection
+        (prolong!, lib/RefMesh/src/RefMesh.jl:284-302). The stream, precision,
+        profile, launch shape and clock stay; device buffers are reused."""
+        C = self._C
+        if (mesh.rank, mesh.maxlevel, tuple(mesh.h0), mesh.gamma) != (self.mesh.rank, self.mesh.maxlevel,
+                                                                     tuple(self.mesh.h0), self.mesh.gamma):
+            raise ValueError("regrid keeps rank, maxlevel, h0 and gamma")
+        sb = np.ascontiguousarray(src_block, dtype=np.int32)
+        if sb.shape != (mesh.nblocks,):
+            raise ValueError(f"src_block must hold one source block per block ({mesh.nblocks})")
+        _, lv, dm, og, bc, keep = _mesh_args(mesh)
+        self._check(self._lib.vh_blocks_regrid(self._h, mesh.nblocks, lv, dm, og, bc,
+                                               sb.ctypes.data_as(C.POINTER(C.c_int32))))
+        self.mesh = mesh
+        self.ncell = int(self._lib.vh_blocks_ncell(self._h))
 
     def time_steps(self, nsteps: int, method: str = "graph", stage2: bool = True) -> dict:
         """Time nsteps single-rate steps: method 'persistent' (eager), 'per_block'

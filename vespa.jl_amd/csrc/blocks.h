@@ -156,7 +156,27 @@ __device__ __forceinline__ void add_link_group(Cons<T>& acc, const BlocksDev& B,
 // One RK stage over a list of tiles (persistent: workgroup w takes tiles w,
 // w + gridDim.x, ...). STAGE2 as in rk_stage_kernel; the CFL partial of the
 // new state covers the workgroup's tiles.
-template <typename T, int RANK, bool STAGE2, bool FLOOR64>
+// apply_gravity_source! (src/gravity.jl:412-447) for cell c of a block with
+// volume V: (0, −ρg_x·V, −ρg_y·V, −ρg_z·V, −(ρv·g)·V) added to the flux sum, g
+// read from the step's field (axes beyond the rank hold 0).
+template <typename T>
+__device__ __forceinline__ void block_grav_add(Cons<T>& acc, const Cons<T>& U, const T* __restrict__ gf, long long n,
+                                               long long c, T V) {
+  const T g0 = gf[c], g1 = gf[n + c], g2 = gf[2 * n + c];
+  const T rho = U.d;
+  T es = T(0);
+  es += U.mx * g0;
+  es += U.my * g1;
+  es += U.mz * g2;
+  const T esV = es * V;
+  acc.d = acc.d + T(0);
+  acc.mx = acc.mx + (-(rho * g0 * V));
+  acc.my = acc.my + (-(rho * g1 * V));
+  acc.mz = acc.mz + (-(rho * g2 * V));
+  acc.e = acc.e + (-esV);
+}
+
+template <typename T, int RANK, bool STAGE2, bool FLOOR64, bool GRAV = false>
 __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 : VH_BLOCK_MIN_WAVES) block_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
                                                                        const double* __restrict__ dtp,
                                                                        double* __restrict__ partials, BlocksDev B) {
@@ -453,6 +473,8 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
           }
         }
 
+        if constexpr (GRAV) block_grav_add<T>(acc, uown, (const T*)B.gravg, n, c, T(bk->V));
+
         Cons<T> o;
         if constexpr (!STAGE2) {
           o.d = uown.d - invV * acc.d;
@@ -484,7 +506,7 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
         }
       }
     }
-    if constexpr (STAGE2) cfl_max = vmax(cfl_max, cfl.value(bk->w));
+    if constexpr (STAGE2) cfl_max = vmax(cfl_max, cfl.value(bk->w, GRAV, B.gravG));
   }
 
   if constexpr (STAGE2) {
@@ -517,6 +539,8 @@ __global__ void __launch_bounds__(256) block_cfl_kernel(const T* __restrict__ U,
     double v = double(vabs(W.vx) + cs) / bk.w[0];
     if (RANK > 1) v = vmax(v, double(vabs(W.vy) + cs) / bk.w[1]);
     if (RANK > 2) v = vmax(v, double(vabs(W.vz) + cs) / bk.w[2]);
+    // free-fall limiter √(4πGρ)/½ with self-gravity (src/gravity.jl:449-452)
+    if (B.gravg) v = vmax(v, double(W.r) > 0.0 ? __builtin_sqrt(4.0 * 3.141592653589793 * B.gravG * double(W.r)) / 0.5 : 0.0);
     m = vmax(m, v);
   }
   m = block_max<256>(m, red);

@@ -22,11 +22,18 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <string>
-#include <unordered_map>
 #include <vector>
+
+#include <unistd.h>
 
 #include "../../include/vespahip.h"
 #include "host.h"
@@ -36,67 +43,6 @@ namespace vh {
 
 enum { NB_SAME = 0, NB_COARSER = 1, NB_FINER = 2, NB_BOUNDARY = 3 };
 
-// Map from non-negative 64-bit keys (cell indices, cell·6 + slot) to
-// non-negative ints, open addressing with linear probing: the table build does
-// a few lookups per cell on every linked block face, where node-based maps
-// spend most of the build's time.
-class FlatMap {
- public:
-  void reserve(size_t n) {
-    size_t cap = 16;
-    while (cap < 2 * n) cap <<= 1;
-    keys_.assign(cap, -1);
-    vals_.assign(cap, 0);
-    mask_ = cap - 1;
-    n_ = 0;
-  }
-  int find(long long k) const {
-    if (keys_.empty()) return -1;
-    for (size_t i = slot(k);; i = (i + 1) & mask_) {
-      if (keys_[i] == k) return vals_[i];
-      if (keys_[i] < 0) return -1;
-    }
-  }
-  // Sets key k to v; returns false (and leaves the value) when k is present and keep is set.
-  bool put(long long k, int v, bool keep = false) {
-    if (2 * (n_ + 1) > keys_.size()) grow();
-    for (size_t i = slot(k);; i = (i + 1) & mask_) {
-      if (keys_[i] == k) {
-        if (keep) return false;
-        vals_[i] = v;
-        return true;
-      }
-      if (keys_[i] < 0) {
-        keys_[i] = k;
-        vals_[i] = v;
-        ++n_;
-        return true;
-      }
-    }
-  }
-  template <class F>
-  void for_each(F f) const {
-    for (size_t i = 0; i < keys_.size(); ++i)
-      if (keys_[i] >= 0) f(keys_[i], vals_[i]);
-  }
-  size_t size() const { return n_; }
-
- private:
-  size_t slot(long long k) const { return (size_t)(((unsigned long long)k * 0x9E3779B97F4A7C15ULL) >> 20) & mask_; }
-  void grow() {
-    std::vector<long long> k0;
-    std::vector<int> v0;
-    k0.swap(keys_);
-    v0.swap(vals_);
-    reserve(k0.size());
-    for (size_t i = 0; i < k0.size(); ++i)
-      if (k0[i] >= 0) put(k0[i], v0[i]);
-  }
-  std::vector<long long> keys_;
-  std::vector<int> vals_;
-  size_t mask_ = 0, n_ = 0;
-};
-
 // Tile limits of block_stage_kernel (BNT lanes, LDS sized for these bounds).
 static const int kLanes = 256, kMaxW = 432, kMaxFS = 592, kMaxE = 32;
 
@@ -105,9 +51,12 @@ struct Topo {
   double h0[3] = {1, 1, 1};
   std::vector<int> level, dims, bc;      // dims: 3 per block, bc: 6 per block
   std::vector<long long> origin, off;    // origin: 3 per block, off: nb + 1
-  std::vector<std::vector<int>> cand;    // 6 per block: blocks touching the face plane
+  // blocks touching face plane k = 6·block + slot: cand_ids[cand_off[k] .. cand_off[k+1])
+  std::vector<int> cand_off, cand_ids;
 
-  long long span(int b) const { return 1LL << (maxlevel - level[b]); }
+  int shift(int b) const { return maxlevel - level[b]; }
+  long long span(int b) const { return 1LL << shift(b); }
+  bool has_cand(int b, int slot) const { return cand_off[6 * b + slot + 1] > cand_off[6 * b + slot]; }
   double width(int b, int d) const { return h0[d] / (double)(1LL << level[b]); }
   double volume(int b) const {
     double v = width(b, 0);
@@ -127,12 +76,15 @@ struct Topo {
       if (off[mid] <= g) lo = mid;
       else hi = mid - 1;
     }
-    const long long l = g - off[lo];
-    const int* n = &dims[3 * lo];
+    cell_ijk(lo, g, ijk);
+    return lo;
+  }
+  void cell_ijk(int b, long long g, long long ijk[3]) const {
+    const int l = (int)(g - off[b]);   // blocks hold fewer than 2^31 cells
+    const int* n = &dims[3 * b];
     ijk[0] = l % n[0];
     ijk[1] = (l / n[0]) % n[1];
-    ijk[2] = l / ((long long)n[0] * n[1]);
-    return lo;
+    ijk[2] = l / (n[0] * n[1]);
   }
   long long index(int b, const long long ijk[3]) const {
     const int* n = &dims[3 * b];
@@ -146,16 +98,12 @@ struct Topo {
     }
     return true;
   }
-  // Neighbour of cell g across face (d, side): kind and representative cell;
-  // for NB_FINER the 2^(rank-1) finer cells on the face in their leaf order.
-  int neighbor(long long g, int d, int side, long long* rep, long long* fine, int* nfine, int* nbblock) const {
-    long long ijk[3];
-    const int b = locate(g, ijk);
-    return neighbor_at(b, ijk, d, side, rep, fine, nfine, nbblock);
-  }
-  // The same for cell ijk of block b (no search for the block).
+  // Neighbour of cell ijk of block b across face (d, side): kind and
+  // representative cell; for NB_FINER the 2^(rank-1) finer cells on the face in
+  // their leaf order. nbblock / nijk (optional): the neighbour's block and the
+  // representative's cell coordinates in it.
   int neighbor_at(int b, const long long ijk[3], int d, int side, long long* rep, long long* fine, int* nfine,
-                  int* nbblock) const {
+                  int* nbblock, long long* nijk = nullptr) const {
     const int* n = &dims[3 * b];
     const long long j = ijk[d] + (side ? 1 : -1);
     if (j >= 0 && j < n[d]) {
@@ -163,6 +111,7 @@ struct Topo {
       t[d] = j;
       *rep = index(b, t);
       if (nbblock) *nbblock = b;
+      if (nijk) for (int e = 0; e < 3; ++e) nijk[e] = t[e];
       return NB_SAME;
     }
     if (bc[6 * b + 2 * d + side] == BC_PERIODIC) {
@@ -170,19 +119,23 @@ struct Topo {
       t[d] = j < 0 ? n[d] - 1 : 0;
       *rep = index(b, t);
       if (nbblock) *nbblock = b;
+      if (nijk) for (int e = 0; e < 3; ++e) nijk[e] = t[e];
       return NB_SAME;
     }
-    const long long s = span(b);
+    const int sb = shift(b);
     long long q[3] = {0, 0, 0};
-    for (int e = 0; e < rank; ++e) q[e] = origin[3 * b + e] + ijk[e] * s;
-    q[d] = side ? q[d] + s : q[d] - 1;
-    for (int c : cand[6 * b + 2 * d + side]) {
+    for (int e = 0; e < rank; ++e) q[e] = origin[3 * b + e] + (ijk[e] << sb);
+    q[d] = side ? q[d] + (1LL << sb) : q[d] - 1;
+    const int k = 6 * b + 2 * d + side;
+    for (int ci = cand_off[k]; ci < cand_off[k + 1]; ++ci) {
+      const int c = cand_ids[ci];
       if (!contains(c, q)) continue;
-      const long long sc = span(c);
+      const int sc = shift(c);
       long long t[3] = {0, 0, 0};
-      for (int e = 0; e < rank; ++e) t[e] = (q[e] - origin[3 * c + e]) / sc;
+      for (int e = 0; e < rank; ++e) t[e] = (q[e] - origin[3 * c + e]) >> sc;   // q lies inside c
       *rep = index(c, t);
       if (nbblock) *nbblock = c;
+      if (nijk) for (int e = 0; e < 3; ++e) nijk[e] = t[e];
       const int dl = level[c] - level[b];
       if (dl == 0) return NB_SAME;
       if (dl == -1) return NB_COARSER;
@@ -246,52 +199,216 @@ static int balanced(int n, int maxT) {
   return (n + nt - 1) / nt;
 }
 
-// Builds every table of a mesh. Returns 0 or a negative code (message set).
-static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
-  const int nb = T.nb, rank = T.rank;
-  // ---- candidate neighbours by face plane ----
-  T.cand.assign(6 * nb, {});
-  for (int d = 0; d < rank; ++d) {
-    std::unordered_map<long long, std::vector<int>> lo_at;
-    for (int c = 0; c < nb; ++c) lo_at[T.origin[3 * c + d]].push_back(c);
-    std::unordered_map<long long, std::vector<int>> hi_at;
-    for (int c = 0; c < nb; ++c) hi_at[T.origin[3 * c + d] + (long long)T.dims[3 * c + d] * T.span(c)].push_back(c);
-    for (int b = 0; b < nb; ++b) {
-      const long long sb = T.span(b);
-      for (int side = 0; side < 2; ++side) {
-        const long long plane = side ? T.origin[3 * b + d] + (long long)T.dims[3 * b + d] * sb : T.origin[3 * b + d];
-        auto& m = side ? lo_at : hi_at;
-        auto it = m.find(plane);
-        if (it == m.end()) continue;
-        for (int c : it->second) {
-          if (c == b) continue;
-          const long long sc = T.span(c);
-          bool overlap = true;
-          for (int e = 0; e < rank; ++e) {
-            if (e == d) continue;
-            const long long alo = T.origin[3 * b + e], ahi = alo + T.dims[3 * b + e] * sb;
-            const long long blo = T.origin[3 * c + e], bhi = blo + T.dims[3 * c + e] * sc;
-            if (bhi <= alo || ahi <= blo) overlap = false;
-          }
-          if (overlap) T.cand[6 * b + 2 * d + side].push_back(c);
-        }
-        if (!T.cand[6 * b + 2 * d + side].empty() && T.bc[6 * b + 2 * d] == BC_PERIODIC)
-          return fail(-1, "block %s%lld touches another block along a periodic axis", "", b);
-      }
+// ----------------------------------------------------------------------------
+// host thread pool for the table builder: run(n, f) calls f(i) for i in [0, n)
+// on the pool's workers and the caller, returning when all are done. Created on
+// first use with min(hardware threads, 16) threads (VESPAHIP_HOST_THREADS
+// overrides, 1 = serial); a child process after fork() builds its own (it
+// inherits no worker threads). The pool is never destroyed: its workers sleep
+// until the process ends.
+class HostPool {
+ public:
+  static int threads() {
+    static const int n = [] {
+      const char* e = getenv("VESPAHIP_HOST_THREADS");
+      int v = e ? atoi(e) : (int)std::thread::hardware_concurrency();
+      return std::max(1, std::min(v, e ? 256 : 16));
+    }();
+    return n;
+  }
+  static void run(int n, const std::function<void(int)>& f) {
+    if (n <= 1 || threads() <= 1) {
+      for (int i = 0; i < n; ++i) f(i);
+      return;
     }
+    static std::mutex create_mu;
+    static HostPool* pool = nullptr;
+    HostPool* p;
+    {
+      std::lock_guard<std::mutex> lk(create_mu);
+      if (!pool || pool->pid_ != getpid()) pool = new HostPool(threads() - 1);   // the old one stays (forked child)
+      p = pool;
+    }
+    p->go(n, f);
   }
 
-  // ---- blocks, representative maps, link faces ----
+ private:
+  explicit HostPool(int workers) : pid_(getpid()) {
+    for (int i = 0; i < workers; ++i) std::thread([this] { loop(); }).detach();
+    nworkers_ = workers;
+  }
+  void loop() {
+    long long seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      int n;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+        f = job_;
+        n = njob_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--active_ == 0) done_.notify_one();
+    }
+  }
+  void go(int n, const std::function<void(int)>& f) {
+    std::lock_guard<std::mutex> run_lk(run_mu_);   // one job at a time
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      job_ = &f;
+      njob_ = n;
+      next_.store(0);
+      active_ = nworkers_;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return active_ == 0; });
+  }
+  pid_t pid_;
+  int nworkers_ = 0;
+  std::mutex run_mu_, mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int njob_ = 0, active_ = 0;
+  long long gen_ = 0;
+  std::atomic<int> next_{0};
+};
+
+// Output of one chunk (a contiguous run of blocks) of the parallel table
+// phases; indices inside are chunk-local until the chunks are joined in order,
+// which reproduces the serial walk's order exactly.
+struct ChunkOut {
+  int b0 = 0, b1 = 0;
+  int rc = 0;
+  std::string err;
+  std::vector<LinkFaceDev> faces;
+  std::vector<LinkCellDev> recs;
+  std::vector<int> slot_faces, lcmap;
+  std::vector<RegList> regs;   // per level
+  int set_error(int code) {
+    if (!rc) {
+      rc = code;
+      err = vh_last_error();
+    }
+    return code;
+  }
+};
+
+// Builds every table of a mesh. Returns 0 or a negative code (message set).
+// nthreads: 0 = the pool's size, 1 = serial; the tables do not depend on it.
+static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int nthreads = 0) {
+  const int nb = T.nb, rank = T.rank;
+  // ---- candidate neighbours by face plane ----
+  // Blocks grouped by the coordinate of their lo (hi) plane along each axis and
+  // sorted inside a group by their lo corner along the first other axis; a
+  // block's side meets the blocks of the opposite group at its own plane whose
+  // extents overlap its own along the other axes. Blocks are disjoint, so the
+  // search along that axis starts one largest extent before the side's lo end.
+  // Blocks are searched in parallel chunks; each side lists its candidates in
+  // ascending block order.
+  const int nthr = nthreads > 0 ? nthreads : HostPool::threads();
+  {
+    struct Key {
+      long long plane, e1lo;
+      int b;
+      bool operator<(const Key& o) const {
+        return plane != o.plane ? plane < o.plane : (e1lo != o.e1lo ? e1lo < o.e1lo : b < o.b);
+      }
+    };
+    std::vector<Key> at[3][2];   // [axis][0: lo planes, 1: hi planes]
+    long long emax[3] = {1, 1, 1};
+    for (int d = 0; d < rank; ++d) {
+      const int e1 = d == 0 ? 1 : 0;
+      at[d][0].resize(nb);
+      at[d][1].resize(nb);
+      for (int c = 0; c < nb; ++c) {
+        const long long o1 = e1 < rank ? T.origin[3 * c + e1] : 0;
+        at[d][0][c] = {T.origin[3 * c + d], o1, c};
+        at[d][1][c] = {T.origin[3 * c + d] + (long long)T.dims[3 * c + d] * T.span(c), o1, c};
+        if (e1 < rank) emax[d] = std::max(emax[d], (long long)T.dims[3 * c + e1] * T.span(c));
+      }
+      std::sort(at[d][0].begin(), at[d][0].end());
+      std::sort(at[d][1].begin(), at[d][1].end());
+    }
+    const int ncc = nb < 256 ? 1 : std::min(4 * nthr, nb);
+    std::vector<std::vector<int>> cids(ncc);          // candidates of the chunk's sides in (block, slot) order
+    std::vector<int> counts(6 * (size_t)nb, 0);
+    std::vector<int> perr(ncc, -1);                    // block with a periodic-axis contact
+    HostPool::run(ncc, [&](int q) {
+      const int b0 = (int)((long long)nb * q / ncc), b1 = (int)((long long)nb * (q + 1) / ncc);
+      int tmp[64];
+      for (int b = b0; b < b1; ++b) {
+        const long long sb = T.span(b);
+        for (int d = 0; d < rank; ++d) {
+          const int e1 = d == 0 ? 1 : 0;
+          const long long a1 = e1 < rank ? T.origin[3 * b + e1] : 0;
+          const long long a1hi = e1 < rank ? a1 + T.dims[3 * b + e1] * sb : 1;
+          for (int side = 0; side < 2; ++side) {
+            const long long plane =
+                side ? T.origin[3 * b + d] + (long long)T.dims[3 * b + d] * sb : T.origin[3 * b + d];
+            const auto& v = at[d][side ? 0 : 1];
+            int k = 0;
+            for (auto it = std::lower_bound(v.begin(), v.end(), Key{plane, a1 - emax[d] + 1, -1});
+                 it != v.end() && it->plane == plane && it->e1lo < a1hi; ++it) {
+              const int c = it->b;
+              if (c == b) continue;
+              const long long sc = T.span(c);
+              bool overlap = true;
+              for (int e = 0; e < rank; ++e) {
+                if (e == d) continue;
+                const long long alo = T.origin[3 * b + e], ahi = alo + T.dims[3 * b + e] * sb;
+                const long long blo = T.origin[3 * c + e], bhi = blo + T.dims[3 * c + e] * sc;
+                if (bhi <= alo || ahi <= blo) overlap = false;
+              }
+              if (!overlap) continue;
+              if (k < 64) tmp[k] = c;
+              ++k;
+            }
+            if (k > 64) {   // more than 64 blocks on one side: collect again without the cap
+              std::vector<int> big;
+              for (auto it = std::lower_bound(v.begin(), v.end(), Key{plane, a1 - emax[d] + 1, -1});
+                   it != v.end() && it->plane == plane && it->e1lo < a1hi; ++it) {
+                const int c = it->b;
+                if (c == b) continue;
+                const long long sc = T.span(c);
+                bool overlap = true;
+                for (int e = 0; e < rank; ++e) {
+                  if (e == d) continue;
+                  const long long alo = T.origin[3 * b + e], ahi = alo + T.dims[3 * b + e] * sb;
+                  const long long blo = T.origin[3 * c + e], bhi = blo + T.dims[3 * c + e] * sc;
+                  if (bhi <= alo || ahi <= blo) overlap = false;
+                }
+                if (overlap) big.push_back(c);
+              }
+              std::sort(big.begin(), big.end());
+              cids[q].insert(cids[q].end(), big.begin(), big.end());
+            } else {
+              std::sort(tmp, tmp + k);
+              cids[q].insert(cids[q].end(), tmp, tmp + k);
+            }
+            counts[6 * (size_t)b + 2 * d + side] = k;
+            if (k > 0 && T.bc[6 * b + 2 * d] == BC_PERIODIC && perr[q] < 0) perr[q] = b;
+          }
+        }
+      }
+    });
+    for (int q = 0; q < ncc; ++q)
+      if (perr[q] >= 0) return fail(-1, "block %s%lld touches another block along a periodic axis", "", perr[q]);
+    T.cand_off.assign(6 * (size_t)nb + 1, 0);
+    for (size_t k = 0; k < 6 * (size_t)nb; ++k) T.cand_off[k + 1] = T.cand_off[k] + counts[k];
+    T.cand_ids.clear();
+    T.cand_ids.reserve(T.cand_off[6 * (size_t)nb]);
+    for (int q = 0; q < ncc; ++q) T.cand_ids.insert(T.cand_ids.end(), cids[q].begin(), cids[q].end());
+  }
+
+  // ---- blocks and the offsets of their face-plane maps (serial) ----
   H.blocks.assign(nb, BlockDev{});
-  FlatMap facemap;   // emitting cell * 6 + slot -> face
-  long long cells_on_linked_planes = 0;
-  for (int b = 0; b < nb; ++b)
-    for (int d = 0; d < rank; ++d)
-      for (int side = 0; side < 2; ++side)
-        if (!T.cand[6 * b + 2 * d + side].empty()) cells_on_linked_planes += plane_size(&T.dims[3 * b], d);
-  facemap.reserve((size_t)cells_on_linked_planes + 16);
-  long long fine[4];
-  int nfine = 0;
+  long long nplane = 0;
   for (int b = 0; b < nb; ++b) {
     BlockDev& bk = H.blocks[b];
     for (int d = 0; d < 3; ++d) bk.n[d] = T.dims[3 * b + d];
@@ -305,167 +422,321 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
     bk.V = T.volume(b);
     for (int s = 0; s < 6; ++s) bk.repoff[s] = bk.lcoff[s] = -1;
     for (int d = 0; d < rank; ++d)
-      for (int side = 0; side < 2; ++side) {
-        const int slot = 2 * d + side;
-        if (T.cand[6 * b + slot].empty()) continue;
-        const int ps = plane_size(bk.n, d);
-        if ((long long)H.repmap.size() + ps > 0x7fffffffLL) return fail(-1, "mesh too large%s", "");
-        bk.repoff[slot] = (int)H.repmap.size();
-        H.repmap.resize(H.repmap.size() + ps, -1);
-        const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-        for (int v = 0; v < bk.n[e2]; ++v)
-          for (int u = 0; u < bk.n[e1]; ++u) {
-            long long ijk[3];
-            ijk[d] = side ? bk.n[d] - 1 : 0;
-            ijk[e1] = u;
-            ijk[e2] = v;
-            const long long g = T.index(b, ijk);
-            long long rep = -1;
-            const int kind = T.neighbor_at(b, ijk, d, side, &rep, fine, &nfine, nullptr);
-            if (kind == -1) return fail(-1, "block %s%lld touches a block more than one level apart", "", b);
-            if (kind < 0)
-              return fail(-1, "block %s%lld: a coarse face is not covered by one finer block (fine blocks must be "
-                          "aligned to coarse cells)", "", b);
-            H.repmap[bk.repoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] =
-                kind == NB_BOUNDARY ? -1 : (int)rep;
-            const bool emit = (side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER);
-            if (!emit) continue;
-            LinkFaceDev f{};
-            f.iL = (int)(side ? g : rep);
-            f.iR = (int)(side ? rep : g);
-            f.axis = d;
-            f.area = T.area(b, d);
-            facemap.put(g * 6 + slot, (int)H.faces.size());
-            H.faces.push_back(f);
-          }
-      }
+      for (int side = 0; side < 2; ++side)
+        if (T.has_cand(b, 2 * d + side)) {
+          bk.repoff[2 * d + side] = (int)nplane;
+          nplane += plane_size(bk.n, d);
+        }
+    if (nplane > 0x7fffffffLL) return fail(-1, "mesh too large%s", "");
   }
-  // representative neighbours of both cells of every link face
-  for (LinkFaceDev& f : H.faces) {
-    const int d = f.axis;
-    for (int which = 0; which < 2; ++which) {
-      const long long c = which ? f.iR : f.iL;
-      long long ijk[3];
-      const int b = T.locate(c, ijk);
-      int* nbr = which ? f.nR : f.nL;
-      for (int s = 0; s < 2; ++s) {
-        long long rep = -1;
-        const int kind = T.neighbor_at(b, ijk, d, s, &rep, nullptr, nullptr, nullptr);
-        if (kind < 0) return fail(-1, "invalid neighbour across a link face%s", "");
-        nbr[s] = kind == NB_BOUNDARY ? -1 - T.bc[6 * b + 2 * d + s] : (int)rep;
-      }
-    }
-  }
+  H.repmap.assign((size_t)nplane, -1);
+  // fmap runs beside repmap: the link face a face cell emits through that side, or -1
+  std::vector<int> fmap((size_t)nplane, -1);
 
-  // ---- link-cell records ----
-  // A slot lists at most 2^(rank-1) <= 4 faces: [0] count, [1..4] faces.
-  using SlotList = std::array<int, 5>;
-  FlatMap recof;   // cell -> record
-  recof.reserve(2 * H.faces.size() + 16);
-  std::vector<int> recblock;
-  std::vector<long long> reccell;             // record -> cell
-  std::vector<int> recgrpblock;               // record*6 + slot -> emitting block of a type-2 slot
-  std::vector<LinkCellDev> recs;
-  std::vector<SlotList> slotlists;            // record*6 + slot
+  // chunks: contiguous block ranges of similar face-plane work
+  int nchunk = std::min(4 * nthr, nb);
+  if (nb < 64 || nplane < 4096 || nthreads == 1) nchunk = 1;
+  std::vector<ChunkOut> ch(nchunk);
   {
-    // at most two records per link face, and one per cell on a linked face plane
-    const size_t nrec = std::min((size_t)cells_on_linked_planes, 2 * H.faces.size());
-    recs.reserve(nrec);
-    recblock.reserve(nrec);
-    reccell.reserve(nrec);
-    recgrpblock.reserve(6 * nrec);
-    slotlists.reserve(6 * nrec);
-  }
-  auto record_for = [&](long long g, int b) -> int {
-    const int found = recof.find(g);
-    if (found >= 0) return found;
-    const int r = (int)recs.size();
-    recof.put(g, r);
-    reccell.push_back(g);
-    LinkCellDev R{};
-    recs.push_back(R);
-    recblock.push_back(b);
-    for (int q = 0; q < 6; ++q) {
-      slotlists.push_back(SlotList{0, -1, -1, -1, -1});
-      recgrpblock.push_back(-1);
-    }
-    return r;
-  };
-  for (int b = 0; b < nb; ++b) {
-    const BlockDev& bk = H.blocks[b];
-    for (int d = 0; d < rank; ++d)
-      for (int side = 0; side < 2; ++side) {
-        const int slot = 2 * d + side;
-        if (bk.repoff[slot] < 0) continue;
-        const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-        for (int v = 0; v < bk.n[e2]; ++v)
-          for (int u = 0; u < bk.n[e1]; ++u) {
-            long long ijk[3];
-            ijk[d] = side ? bk.n[d] - 1 : 0;
-            ijk[e1] = u;
-            ijk[e2] = v;
-            const long long g = T.index(b, ijk);
-            long long rep = -1;
-            int nbb = -1;
-            const int kind = T.neighbor_at(b, ijk, d, side, &rep, fine, &nfine, &nbb);
-            if (kind == NB_BOUNDARY) continue;
-            int type = 0, emitter_block = -1;
-            SlotList fl{0, -1, -1, -1, -1};
-            auto face_of = [&](long long cell, int sl) -> int {
-              return facemap.find(cell * 6 + sl);
-            };
-            if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
-              type = 1;
-              fl[++fl[0]] = face_of(g, slot);
-            } else if (side == 0 && kind == NB_SAME) {
-              type = 2;
-              emitter_block = nbb;
-              fl[++fl[0]] = face_of(rep, 2 * d + 1);
-            } else if (kind == NB_FINER) {
-              type = 2;
-              emitter_block = nbb;
-              for (int i = 0; i < nfine; ++i) fl[++fl[0]] = face_of(fine[i], side ? 2 * d : 2 * d + 1);
-            }
-            for (int i = 1; i <= fl[0]; ++i)
-              if (fl[i] < 0)
-                return fail(-1, "block %s%lld: a link face has no emitting cell (mesh not 2:1 aligned)", "", b);
-            const int r = record_for(g, b);
-            recs[r].type[slot] = type;
-            slotlists[6 * r + slot] = fl;
-            recgrpblock[6 * r + slot] = emitter_block;
-            if (kind == NB_FINER) recs[r].finer |= 1 << slot;
-            for (int i = 1; i <= fl[0]; ++i) {   // each listed face must touch this cell
-              const LinkFaceDev& F = H.faces[fl[i]];
-              if (F.iL != g && F.iR != g) return fail(-1, "link face bookkeeping error at block %s%lld", "", b);
-            }
-          }
+    long long acc = 0;
+    int c = 0;
+    ch[0].b0 = 0;
+    for (int b = 0; b < nb; ++b) {
+      const BlockDev& bk = H.blocks[b];
+      long long w = 1;
+      for (int s = 0; s < 6; ++s)
+        if (bk.repoff[s] >= 0) w += plane_size(bk.n, s >> 1);
+      acc += w;
+      if (c + 1 < nchunk && acc * nchunk >= (long long)(c + 1) * (nplane + nb)) {
+        ch[c].b1 = b + 1;
+        ch[++c].b0 = b + 1;
       }
-  }
-  H.linkcells.resize(recs.size());
-  H.slot_faces.reserve(2 * H.faces.size());   // each link face is listed by both of its cells
-  for (size_t r = 0; r < recs.size(); ++r) {
-    LinkCellDev R = recs[r];
-    std::pair<int, int> grp[6];   // (emitting block, slot), at most one per slot
-    int ng = 0;
-    for (int s = 0; s < 6; ++s) {
-      const SlotList& sl = slotlists[6 * r + s];
-      R.start[s] = (int)H.slot_faces.size();
-      R.count[s] = sl[0];
-      for (int i = 1; i <= sl[0]; ++i) H.slot_faces.push_back(sl[i]);
-      if (R.type[s] == 2) grp[ng++] = {recgrpblock[6 * r + s], s};
     }
-    std::sort(grp, grp + ng);
-    R.ngrp = ng;
-    R.npre = 0;
-    for (int q = 0; q < 6; ++q) R.grp[q] = -1;
-    for (int q = 0; q < ng; ++q) {
-      if (q > 0 && grp[q].first == grp[q - 1].first)
-        return fail(-1, "a cell touches block %s%lld through two faces", "", grp[q].first);
-      R.grp[q] = grp[q].second;
-      if (grp[q].first < recblock[r]) R.npre++;
-    }
-    H.linkcells[r] = R;
+    ch[c].b1 = nb;
+    ch.resize(c + 1);
+    nchunk = c + 1;
   }
+  auto for_chunks = [&](const std::function<void(ChunkOut&)>& f) {
+    HostPool::run(nchunk, [&](int i) { f(ch[i]); });
+    for (ChunkOut& o : ch)
+      if (o.rc) return fail(o.rc, "%s", o.err.c_str());
+    return 0;
+  };
+
+  // ---- representative maps and link faces (per chunk) ----
+  // A face is emitted once, as the face walk emits it: same level from the hi
+  // side, coarse↔fine from the fine side (lib/HGBackend/src/HGBackend.jl:202-276),
+  // with the representative neighbours of both cells for the reconstruction.
+  int rc = for_chunks([&](ChunkOut& o) {
+    long long fine[4];
+    int nfine = 0;
+    for (int b = o.b0; b < o.b1 && !o.rc; ++b) {
+      const BlockDev& bk = H.blocks[b];
+      for (int d = 0; d < rank; ++d)
+        for (int side = 0; side < 2; ++side) {
+          const int slot = 2 * d + side;
+          if (bk.repoff[slot] < 0) continue;
+          const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+          for (int v = 0; v < bk.n[e2]; ++v)
+            for (int u = 0; u < bk.n[e1]; ++u) {
+              long long ijk[3];
+              ijk[d] = side ? bk.n[d] - 1 : 0;
+              ijk[e1] = u;
+              ijk[e2] = v;
+              const long long g = T.index(b, ijk);
+              long long rep = -1, nijk[3];
+              int nbb = -1;
+              const int kind = T.neighbor_at(b, ijk, d, side, &rep, fine, &nfine, &nbb, nijk);
+              if (kind == -1)
+                return (void)o.set_error(fail(-1, "block %s%lld touches a block more than one level apart", "", b));
+              if (kind < 0)
+                return (void)o.set_error(fail(-1, "block %s%lld: a coarse face is not covered by one finer block "
+                                              "(fine blocks must be aligned to coarse cells)", "", b));
+              const int pi = bk.repoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2]);
+              H.repmap[pi] = kind == NB_BOUNDARY ? -1 : (int)rep;
+              const bool emit = (side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER);
+              if (!emit) continue;
+              LinkFaceDev f{};
+              f.iL = (int)(side ? g : rep);
+              f.iR = (int)(side ? rep : g);
+              f.axis = d;
+              f.area = T.area(b, d);
+              for (int which = 0; which < 2; ++which) {
+                const bool own = (which == 0) == (side == 1);   // iL is this cell on the hi side
+                const int cb = own ? b : nbb;
+                const long long* q = own ? ijk : nijk;
+                int* nbr = which ? f.nR : f.nL;
+                for (int s = 0; s < 2; ++s) {
+                  long long r2 = -1;
+                  const int k2 = T.neighbor_at(cb, q, d, s, &r2, nullptr, nullptr, nullptr);
+                  if (k2 < 0) return (void)o.set_error(fail(-1, "invalid neighbour across a link face%s", ""));
+                  nbr[s] = k2 == NB_BOUNDARY ? -1 - T.bc[6 * cb + 2 * d + s] : (int)r2;
+                }
+              }
+              fmap[pi] = (int)o.faces.size();   // chunk-local until joined
+              o.faces.push_back(f);
+            }
+        }
+    }
+  });
+  if (rc) return rc;
+  std::vector<int> face_base(nchunk + 1, 0);
+  for (int c = 0; c < nchunk; ++c) face_base[c + 1] = face_base[c] + (int)ch[c].faces.size();
+  H.faces.resize(face_base[nchunk]);
+  HostPool::run(nchunk, [&](int c) {
+    const ChunkOut& o = ch[c];
+    std::copy(o.faces.begin(), o.faces.end(), H.faces.begin() + face_base[c]);
+    const int b = o.b0, e = o.b1;
+    if (e > b) {
+      // the chunk's plane maps are one contiguous run of fmap
+      int lo = -1, hi = -1;
+      for (int q = b; q < e; ++q)
+        for (int s = 0; s < 6; ++s)
+          if (H.blocks[q].repoff[s] >= 0) {
+            if (lo < 0) lo = H.blocks[q].repoff[s];
+            hi = H.blocks[q].repoff[s] + plane_size(H.blocks[q].n, s >> 1);
+          }
+      for (int i = lo; i >= 0 && i < hi; ++i)
+        if (fmap[i] >= 0) fmap[i] += face_base[c];
+    }
+  });
+
+  // the link face that cell q of block blk emits through side sl, or -1
+  auto face_at = [&](int blk, const long long q[3], int sl) -> int {
+    const BlockDev& ob = H.blocks[blk];
+    if (ob.repoff[sl] < 0) return -1;
+    const int d = sl >> 1;
+    if (q[d] != ((sl & 1) ? ob.n[d] - 1 : 0)) return -1;
+    return fmap[ob.repoff[sl] + plane_idx(ob.n, d, q[0], q[1], q[2])];
+  };
+
+  // ---- link-cell records, their per-face maps and the flux-register lists (per chunk) ----
+  // One record per cell touching link faces, created block by block in the
+  // order the block's face cells are visited (sides in slot order); a slot
+  // lists at most 2^(rank-1) <= 4 faces: [0] count, [1..4] faces.
+  using SlotList = std::array<int, 5>;
+  rc = for_chunks([&](ChunkOut& o) {
+    long long cmax = 1;
+    for (int b = o.b0; b < o.b1; ++b) cmax = std::max(cmax, T.off[b + 1] - T.off[b]);
+    std::vector<int> loc((size_t)cmax, -1);   // cell of the current block -> its record (block-local)
+    std::vector<long long> bcell;             // block-local record -> cell
+    std::vector<LinkCellDev> brec;
+    std::vector<SlotList> bslots;             // block-local record * 6 + slot
+    std::vector<int> bgrp;                    // block-local record * 6 + slot -> emitting block of a type-2 slot
+    std::vector<std::pair<long long, int>> bfiner;
+    o.regs.assign(T.maxlevel + 1, RegList{});
+    for (int b = o.b0; b < o.b1 && !o.rc; ++b) {
+      BlockDev& bk = H.blocks[b];
+      bcell.clear();
+      brec.clear();
+      bslots.clear();
+      bgrp.clear();
+      const int r0 = (int)o.recs.size();
+      for (int d = 0; d < rank; ++d)
+        for (int side = 0; side < 2; ++side) {
+          const int slot = 2 * d + side;
+          if (bk.repoff[slot] < 0) continue;
+          const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+          for (int v = 0; v < bk.n[e2]; ++v)
+            for (int u = 0; u < bk.n[e1]; ++u) {
+              long long ijk[3];
+              ijk[d] = side ? bk.n[d] - 1 : 0;
+              ijk[e1] = u;
+              ijk[e2] = v;
+              const int pi = bk.repoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2]);
+              if (H.repmap[pi] < 0) continue;   // a domain boundary
+              const long long g = T.index(b, ijk);
+              long long rep = -1, nijk[3];
+              int nbb = -1;
+              const int kind = T.neighbor_at(b, ijk, d, side, &rep, nullptr, nullptr, &nbb, nijk);
+              int type = 0, emitter_block = -1;
+              SlotList fl{0, -1, -1, -1, -1};
+              if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
+                type = 1;
+                fl[++fl[0]] = fmap[pi];
+              } else if (side == 0 && kind == NB_SAME) {
+                type = 2;
+                emitter_block = nbb;
+                fl[++fl[0]] = face_at(nbb, nijk, 2 * d + 1);
+              } else if (kind == NB_FINER) {
+                // the finer cells on the face in leaf order: the representative
+                // (lo corner) and its +1 steps along the face axes
+                type = 2;
+                emitter_block = nbb;
+                const int n2 = e2 < rank ? 2 : 1, n1 = e1 < rank ? 2 : 1;
+                for (int a2 = 0; a2 < n2; ++a2)
+                  for (int a1 = 0; a1 < n1; ++a1) {
+                    long long q[3] = {nijk[0], nijk[1], nijk[2]};
+                    q[e1] += a1;
+                    q[e2] += a2;
+                    fl[++fl[0]] = face_at(nbb, q, side ? 2 * d : 2 * d + 1);
+                  }
+              }
+              for (int i = 1; i <= fl[0]; ++i)
+                if (fl[i] < 0)
+                  return (void)o.set_error(
+                      fail(-1, "block %s%lld: a link face has no emitting cell (mesh not 2:1 aligned)", "", b));
+              int& r = loc[g - T.off[b]];
+              if (r < 0) {
+                r = (int)brec.size();
+                bcell.push_back(g);
+                brec.push_back(LinkCellDev{});
+                for (int q = 0; q < 6; ++q) {
+                  bslots.push_back(SlotList{0, -1, -1, -1, -1});
+                  bgrp.push_back(-1);
+                }
+              }
+              brec[r].type[slot] = type;
+              bslots[6 * r + slot] = fl;
+              bgrp[6 * r + slot] = emitter_block;
+              if (kind == NB_FINER) brec[r].finer |= 1 << slot;
+              for (int i = 1; i <= fl[0]; ++i) {   // each listed face must touch this cell
+                const LinkFaceDev& F = H.faces[fl[i]];
+                if (F.iL != g && F.iR != g)
+                  return (void)o.set_error(fail(-1, "link face bookkeeping error at block %s%lld", "", b));
+              }
+            }
+        }
+      // the block's records: face lists and groups
+      bfiner.clear();
+      for (size_t r = 0; r < brec.size(); ++r) {
+        LinkCellDev R = brec[r];
+        std::pair<int, int> grp[6];   // (emitting block, slot), at most one per slot
+        int ng = 0;
+        for (int s = 0; s < 6; ++s) {
+          const SlotList& sl = bslots[6 * r + s];
+          R.start[s] = (int)o.slot_faces.size();   // chunk-local until joined
+          R.count[s] = sl[0];
+          for (int i = 1; i <= sl[0]; ++i) o.slot_faces.push_back(sl[i]);
+          if (R.type[s] == 2) grp[ng++] = {bgrp[6 * r + s], s};
+        }
+        std::sort(grp, grp + ng);
+        R.ngrp = ng;
+        R.npre = 0;
+        for (int q = 0; q < 6; ++q) R.grp[q] = -1;
+        for (int q = 0; q < ng; ++q) {
+          if (q > 0 && grp[q].first == grp[q - 1].first)
+            return (void)o.set_error(fail(-1, "a cell touches block %s%lld through two faces", "", grp[q].first));
+          R.grp[q] = grp[q].second;
+          if (grp[q].first < b) R.npre++;
+        }
+        o.recs.push_back(R);
+        if (R.finer) bfiner.push_back({bcell[r], r0 + (int)r});
+      }
+      // link-cell maps of the block's faces (every record sits on one of them)
+      bk.haslinks = 0;
+      for (int d = 0; d < rank && !brec.empty(); ++d)
+        for (int side = 0; side < 2; ++side) {
+          const int slot = 2 * d + side;
+          if (bk.repoff[slot] < 0) continue;
+          const int ps = plane_size(bk.n, d);
+          const size_t base = o.lcmap.size();
+          bool any = false;
+          o.lcmap.resize(base + ps, -1);
+          const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+          for (int v = 0; v < bk.n[e2]; ++v)
+            for (int u = 0; u < bk.n[e1]; ++u) {
+              long long ijk[3];
+              ijk[d] = side ? bk.n[d] - 1 : 0;
+              ijk[e1] = u;
+              ijk[e2] = v;
+              const int r = loc[T.index(b, ijk) - T.off[b]];
+              if (r < 0) continue;
+              o.lcmap[base + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] = r0 + r;   // chunk-local
+              any = true;
+            }
+          if (!any) {
+            o.lcmap.resize(base);
+            continue;
+          }
+          bk.lcoff[slot] = (int)base;   // chunk-local
+          bk.haslinks = 1;
+        }
+      // flux-register lists: the coarse cells with finer neighbours, in cell order
+      std::sort(bfiner.begin(), bfiner.end());
+      RegList& L = o.regs[T.level[b]];
+      for (const auto& kv : bfiner) {
+        L.cell.push_back((int)kv.first);
+        L.lc.push_back(kv.second);   // chunk-local
+        L.block.push_back(b);
+      }
+      for (long long g : bcell) loc[g - T.off[b]] = -1;
+    }
+  });
+  if (rc) return rc;
+  std::vector<long long> rec_base(nchunk + 1, 0), sf_base(nchunk + 1, 0), lc_base(nchunk + 1, 0);
+  for (int c = 0; c < nchunk; ++c) {
+    rec_base[c + 1] = rec_base[c] + (long long)ch[c].recs.size();
+    sf_base[c + 1] = sf_base[c] + (long long)ch[c].slot_faces.size();
+    lc_base[c + 1] = lc_base[c] + (long long)ch[c].lcmap.size();
+  }
+  if (lc_base[nchunk] > 0x7fffffffLL || sf_base[nchunk] > 0x7fffffffLL) return fail(-1, "mesh too large%s", "");
+  H.linkcells.resize(rec_base[nchunk]);
+  H.slot_faces.resize(sf_base[nchunk]);
+  H.lcmap.resize(lc_base[nchunk]);
+  HostPool::run(nchunk, [&](int c) {
+    const ChunkOut& o = ch[c];
+    const int rb = (int)rec_base[c], sb = (int)sf_base[c], lb = (int)lc_base[c];
+    for (size_t r = 0; r < o.recs.size(); ++r) {
+      LinkCellDev R = o.recs[r];
+      for (int s = 0; s < 6; ++s) R.start[s] += sb;
+      H.linkcells[rb + r] = R;
+    }
+    std::copy(o.slot_faces.begin(), o.slot_faces.end(), H.slot_faces.begin() + sb);
+    for (size_t i = 0; i < o.lcmap.size(); ++i) H.lcmap[lb + i] = o.lcmap[i] >= 0 ? o.lcmap[i] + rb : -1;
+    for (int b = o.b0; b < o.b1; ++b)
+      for (int s = 0; s < 6; ++s)
+        if (H.blocks[b].lcoff[s] >= 0) H.blocks[b].lcoff[s] += lb;
+  });
+  H.regs.assign(T.maxlevel + 1, RegList{});
+  for (int c = 0; c < nchunk; ++c)
+    for (int l = 0; l <= T.maxlevel; ++l) {
+      const RegList& s = ch[c].regs[l];
+      RegList& L = H.regs[l];
+      L.cell.insert(L.cell.end(), s.cell.begin(), s.cell.end());
+      L.block.insert(L.block.end(), s.block.begin(), s.block.end());
+      for (int v : s.lc) L.lc.push_back(v + (int)rec_base[c]);
+    }
   // every link face must be gathered by both of its cells (fails when a fine
   // block is not aligned to the coarse cells it touches)
   {
@@ -475,68 +746,6 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
       if (seen[f] != 2)
         return fail(-1, "link face %s%lld is not shared by two cells: fine blocks must be aligned to coarse cells", "",
                     (long long)f);
-  }
-  // link-cell maps per block face
-  for (int b = 0; b < nb; ++b) {
-    BlockDev& bk = H.blocks[b];
-    bk.haslinks = 0;
-    for (int d = 0; d < rank; ++d)
-      for (int side = 0; side < 2; ++side) {
-        const int slot = 2 * d + side;
-        if (bk.repoff[slot] < 0) continue;
-        const int ps = plane_size(bk.n, d);
-        std::vector<int> pm(ps, -1);
-        bool any = false;
-        const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-        for (int v = 0; v < bk.n[e2]; ++v)
-          for (int u = 0; u < bk.n[e1]; ++u) {
-            long long ijk[3];
-            ijk[d] = side ? bk.n[d] - 1 : 0;
-            ijk[e1] = u;
-            ijk[e2] = v;
-            const int r = recof.find(T.index(b, ijk));
-            if (r < 0) continue;
-            pm[plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] = r;
-            any = true;
-          }
-        if (!any) continue;
-        if ((long long)H.lcmap.size() + ps > 0x7fffffffLL) return fail(-1, "mesh too large%s", "");
-        bk.lcoff[slot] = (int)H.lcmap.size();
-        H.lcmap.insert(H.lcmap.end(), pm.begin(), pm.end());
-        bk.haslinks = 1;
-      }
-  }
-  // a record may belong to a cell that is on a linked face only through
-  // another slot (edge/corner cells): make sure every record is reachable
-  for (int rec = 0; rec < (int)reccell.size(); ++rec) {
-    long long ijk[3];
-    const int b = T.locate(reccell[rec], ijk);
-    const BlockDev& bk = H.blocks[b];
-    bool found = false;
-    for (int d = 0; d < rank && !found; ++d)
-      for (int side = 0; side < 2 && !found; ++side) {
-        const int slot = 2 * d + side;
-        if (ijk[d] != (side ? bk.n[d] - 1 : 0) || bk.lcoff[slot] < 0) continue;
-        found = H.lcmap[bk.lcoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2])] == rec;
-      }
-    if (!found) return fail(-1, "unreachable link-cell record in block %s%lld", "", b);
-  }
-
-  // ---- flux-register lists (coarse cells with finer neighbours) ----
-  H.regs.assign(T.maxlevel + 1, RegList{});
-  {
-    std::vector<std::pair<long long, int>> order(reccell.size());
-    for (size_t r = 0; r < reccell.size(); ++r) order[r] = {reccell[r], (int)r};
-    std::sort(order.begin(), order.end());
-    for (auto& kv : order) {
-      const LinkCellDev& R = H.linkcells[kv.second];
-      if (!R.finer) continue;
-      const int b = recblock[kv.second];
-      RegList& L = H.regs[T.level[b]];
-      L.cell.push_back((int)kv.first);
-      L.lc.push_back(kv.second);
-      L.block.push_back(b);
-    }
   }
 
   // ---- tiles ----
@@ -612,6 +821,25 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32) {
   }
   H.cfl_list_off.push_back((int)H.cfl_blocks.size());
   return 0;
+}
+
+// FNV-1a over 8-byte words (a trailing partial word zero-padded) of each table.
+template <class V>
+static void digest_vec(uint64_t& h, const std::vector<V>& v) {
+  const size_t n = v.size() * sizeof(V);
+  h = (h ^ (uint64_t)n) * 1099511628211ULL;
+  const unsigned char* p = (const unsigned char*)v.data();
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, p + i, 8);
+    h = (h ^ w) * 1099511628211ULL;
+  }
+  if (i < n) {
+    uint64_t w = 0;
+    memcpy(&w, p + i, n - i);
+    h = (h ^ w) * 1099511628211ULL;
+  }
 }
 
 static int make_topo(Topo& T, int32_t rank, int32_t nblocks, int32_t maxlevel, const double* h0, const int32_t* level,
@@ -782,6 +1010,18 @@ struct vh_blocks {
   int* d_cfl_blocks = nullptr;
   long long* d_cfl_prefix = nullptr;
   double* d_ind = nullptr;   // per-block refinement indicator maximum
+  // Every table above (and the flux-register lists) lives in one device arena;
+  // an in-place regrid uploads the new mesh's tables into the spare arena while
+  // the remap still reads the current one. Both keep their capacity, as do the
+  // pinned staging of the upload and the spare host tables.
+  void* arena[2] = {nullptr, nullptr};
+  size_t arena_cap[2] = {0, 0};
+  int arena_cur = 0;
+  void* stage = nullptr;
+  size_t stage_cap = 0;
+  size_t state_cap = 0;      // bytes of U and of U1
+  Topo T2;
+  HostTables H2;
   std::vector<DevReg> regs;
   // state
   void* U = nullptr;
@@ -792,9 +1032,27 @@ struct vh_blocks {
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_np = -1;
+  struct BlockGrav* grav = nullptr;   // self-gravity (vh_blocks_enable_gravity)
 };
 
 static const int kCflWgs = 2048;
+
+// Self-gravity on a block mesh: the composite operator as CSR (built on the
+// host per mesh, as the reference builds its PoissonKAStruct), φ and the CG
+// vectors at the state's precision, and the step's g field.
+struct BlockGrav {
+  GravParams P;
+  GravBuffers B;
+  GravCsr A;
+  int* rowptr = nullptr;
+  int* colidx = nullptr;
+  double* coef = nullptr;
+  double* vol = nullptr;
+  int* glo = nullptr;
+  int* ghi = nullptr;
+  double* gsum = nullptr;
+  void* g = nullptr;       // 3·ncell values: g_x, g_y, g_z of the current φ
+};
 
 static BlocksDev blocks_dev(const vh_blocks* m) {
   BlocksDev B{};
@@ -811,6 +1069,10 @@ static BlocksDev blocks_dev(const vh_blocks* m) {
   B.igm1 = 1.0 / (m->gamma - 1.0);
   B.ntiles = 0;
   B.nfaces = (int)m->H.faces.size();
+  if (m->grav) {
+    B.gravg = m->grav->g;
+    B.gravG = m->grav->P.G;
+  }
   return B;
 }
 
@@ -1037,24 +1299,7 @@ static int alloc_partials(vh_blocks* m) {
   return 0;
 }
 
-static void free_tables(vh_blocks* m) {
-  (void)hipFree(m->d_blocks);
-  (void)hipFree(m->d_tiles);
-  (void)hipFree(m->d_repmap);
-  (void)hipFree(m->d_lcmap);
-  (void)hipFree(m->d_linkcells);
-  (void)hipFree(m->d_slot_faces);
-  (void)hipFree(m->d_faces);
-  (void)hipFree(m->d_flux);
-  (void)hipFree(m->d_cfl_blocks);
-  (void)hipFree(m->d_cfl_prefix);
-  (void)hipFree(m->d_ind);
-  for (DevReg& r : m->regs) {
-    (void)hipFree(r.cells);
-    (void)hipFree(r.lcs);
-    (void)hipFree(r.blk);
-    (void)hipFree(r.delta);
-  }
+static void forget_tables(vh_blocks* m) {
   m->regs.clear();
   m->d_blocks = nullptr;
   m->d_tiles = nullptr;
@@ -1066,36 +1311,248 @@ static void free_tables(vh_blocks* m) {
   m->d_ind = nullptr;
 }
 
-static int upload_tables(vh_blocks* m) {
-  const HostTables& H = m->H;
-  int rc = 0;
-  if ((rc = upload_vec(&m->d_blocks, H.blocks))) return rc;
-  if ((rc = upload_vec(&m->d_tiles, H.tiles))) return rc;
-  if ((rc = upload_vec(&m->d_repmap, H.repmap))) return rc;
-  if ((rc = upload_vec(&m->d_lcmap, H.lcmap))) return rc;
-  if ((rc = upload_vec(&m->d_linkcells, H.linkcells))) return rc;
-  if ((rc = upload_vec(&m->d_slot_faces, H.slot_faces))) return rc;
-  if ((rc = upload_vec(&m->d_faces, H.faces))) return rc;
-  if ((rc = upload_vec(&m->d_cfl_blocks, H.cfl_blocks))) return rc;
-  if ((rc = upload_vec(&m->d_cfl_prefix, H.cfl_prefix))) return rc;
-  VH_HIP(hipMalloc(&m->d_flux, m->esize * 5 * std::max<size_t>(1, H.faces.size())));
-  VH_HIP(hipMalloc(&m->d_ind, sizeof(double) * m->T.nb));
+static void free_tables(vh_blocks* m) {
+  for (int a = 0; a < 2; ++a) {
+    (void)hipFree(m->arena[a]);
+    m->arena[a] = nullptr;
+    m->arena_cap[a] = 0;
+  }
+  if (m->stage) (void)hipHostFree(m->stage);
+  m->stage = nullptr;
+  m->stage_cap = 0;
+  forget_tables(m);
+}
+
+// Uploads every device table of H into arena slot a (grown with headroom when
+// too small) through the pinned staging buffer in one copy, zeroes the flux-
+// register accumulators and points the context's tables there. extra/extra_bytes
+// (optional) is packed behind the tables; *extra_dev receives its address.
+// Stream-ordered: the caller synchronises before the staging is reused.
+static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, const void* extra = nullptr,
+                         size_t extra_bytes = 0, void** extra_dev = nullptr) {
+  struct Part {
+    const void* src;
+    size_t bytes;
+    size_t off;
+  };
+  std::vector<Part> parts;
+  size_t total = 0;
+  auto add = [&](const void* src, size_t bytes) -> size_t {
+    const size_t off = total;
+    parts.push_back({src, bytes, off});
+    total += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    return off;
+  };
+  const size_t o_blocks = add(H.blocks.data(), sizeof(BlockDev) * H.blocks.size());
+  const size_t o_tiles = add(H.tiles.data(), sizeof(TileDev) * H.tiles.size());
+  const size_t o_repmap = add(H.repmap.data(), sizeof(int) * H.repmap.size());
+  const size_t o_lcmap = add(H.lcmap.data(), sizeof(int) * H.lcmap.size());
+  const size_t o_lc = add(H.linkcells.data(), sizeof(LinkCellDev) * H.linkcells.size());
+  const size_t o_sf = add(H.slot_faces.data(), sizeof(int) * H.slot_faces.size());
+  const size_t o_faces = add(H.faces.data(), sizeof(LinkFaceDev) * H.faces.size());
+  const size_t o_cflb = add(H.cfl_blocks.data(), sizeof(int) * H.cfl_blocks.size());
+  const size_t o_cflp = add(H.cfl_prefix.data(), sizeof(long long) * H.cfl_prefix.size());
+  std::vector<size_t> o_reg(3 * H.regs.size());
+  for (size_t l = 0; l < H.regs.size(); ++l) {
+    o_reg[3 * l] = add(H.regs[l].cell.data(), sizeof(int) * H.regs[l].cell.size());
+    o_reg[3 * l + 1] = add(H.regs[l].lc.data(), sizeof(int) * H.regs[l].lc.size());
+    o_reg[3 * l + 2] = add(H.regs[l].block.data(), sizeof(int) * H.regs[l].block.size());
+  }
+  const size_t o_extra = extra_bytes ? add(extra, extra_bytes) : 0;
+  const size_t staged = total;   // everything above is copied; below are device-only buffers
+  const size_t o_flux = add(nullptr, m->esize * 5 * std::max<size_t>(1, H.faces.size()));
+  const size_t o_ind = add(nullptr, sizeof(double) * (size_t)nblocks);
+  std::vector<size_t> o_delta(H.regs.size());
+  const size_t zero_from = total;
+  for (size_t l = 0; l < H.regs.size(); ++l)
+    o_delta[l] = add(nullptr, m->esize * 5 * std::max<size_t>(1, H.regs[l].cell.size()));
+  const size_t zero_to = total;
+  if (total > m->arena_cap[a]) {
+    (void)hipFree(m->arena[a]);
+    m->arena[a] = nullptr;
+    m->arena_cap[a] = 0;
+    const size_t cap = total + total / 4;
+    VH_HIP(hipMalloc(&m->arena[a], cap));
+    m->arena_cap[a] = cap;
+  }
+  if (staged > m->stage_cap) {
+    if (m->stage) (void)hipHostFree(m->stage);
+    m->stage = nullptr;
+    m->stage_cap = 0;
+    const size_t cap = staged + staged / 4;
+    VH_HIP(hipHostMalloc(&m->stage, cap, 0));
+    m->stage_cap = cap;
+  }
+  char* st = (char*)m->stage;
+  for (const Part& p : parts)
+    if (p.src && p.bytes && p.off < staged) memcpy(st + p.off, p.src, p.bytes);
+  char* base = (char*)m->arena[a];
+  VH_HIP(hipMemcpyAsync(base, st, staged, hipMemcpyHostToDevice, m->stream));
+  const long long nz = (long long)((zero_to - zero_from) / sizeof(float));
+  if (nz > 0)
+    hipLaunchKernelGGL(zero_kernel<float>, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, m->stream,
+                       (float*)(base + zero_from), nz);
+  VH_HIP(hipGetLastError());
+  m->d_blocks = (BlockDev*)(base + o_blocks);
+  m->d_tiles = (TileDev*)(base + o_tiles);
+  m->d_repmap = (int*)(base + o_repmap);
+  m->d_lcmap = (int*)(base + o_lcmap);
+  m->d_linkcells = (LinkCellDev*)(base + o_lc);
+  m->d_slot_faces = (int*)(base + o_sf);
+  m->d_faces = (LinkFaceDev*)(base + o_faces);
+  m->d_cfl_blocks = (int*)(base + o_cflb);
+  m->d_cfl_prefix = (long long*)(base + o_cflp);
+  m->d_flux = base + o_flux;
+  m->d_ind = (double*)(base + o_ind);
   m->regs.assign(H.regs.size(), DevReg{});
   for (size_t l = 0; l < H.regs.size(); ++l) {
     DevReg& R = m->regs[l];
     R.n = (int)H.regs[l].cell.size();
-    if ((rc = upload_vec(&R.cells, H.regs[l].cell))) return rc;
-    if ((rc = upload_vec(&R.lcs, H.regs[l].lc))) return rc;
-    if ((rc = upload_vec(&R.blk, H.regs[l].block))) return rc;
-    const long long nv = 5LL * std::max(1, R.n);
-    VH_HIP(hipMalloc(&R.delta, m->esize * nv));
-    const unsigned g = (unsigned)((nv + 255) / 256);
-    if (m->dtype == VH_F64) hipLaunchKernelGGL(zero_kernel<double>, dim3(g), dim3(256), 0, m->stream, (double*)R.delta, nv);
-    else hipLaunchKernelGGL(zero_kernel<float>, dim3(g), dim3(256), 0, m->stream, (float*)R.delta, nv);
-    VH_HIP(hipGetLastError());
+    R.cells = (int*)(base + o_reg[3 * l]);
+    R.lcs = (int*)(base + o_reg[3 * l + 1]);
+    R.blk = (int*)(base + o_reg[3 * l + 2]);
+    R.delta = base + o_delta[l];
   }
-  VH_HIP(hipStreamSynchronize(m->stream));
+  if (extra_dev) *extra_dev = extra_bytes ? base + o_extra : nullptr;
+  m->arena_cur = a;
   return 0;
+}
+
+// ---- self-gravity (src/gravity.jl) ----
+
+// The composite operator A = −∇²·V of the mesh as CSR, from the face walk of
+// for_each_face (src/gravity.jl:512-562, _assemble_poisson_struct): per leaf in
+// leaf order, per axis the hi face when the neighbour is at the same level or
+// coarser and the lo face when it is coarser; each face (left i, right j, area
+// of the emitting leaf) is one edge with coef = area/d, d = ½(w_i + w_j),
+// pushed to row i then row j. The gravity stencil takes the face walk's
+// neighbours (a side no face reached is a Neumann boundary: the cell itself at
+// half width); the coarse cell of a level jump takes its representative
+// neighbours (src/gravity.jl:576-595).
+struct HostCsr {
+  std::vector<int> rowptr, colidx, glo, ghi;
+  std::vector<double> coef, vol, gsum;
+};
+
+static int build_poisson(const Topo& T, HostCsr& C) {
+  const long long n = T.off[T.nb];
+  const int rank = T.rank;
+  std::vector<int> ei, ej, lev(n);
+  std::vector<double> ec, dlo(3 * n, 0.0), dhi(3 * n, 0.0);
+  std::vector<char> marked(n, 0);
+  ei.reserve(rank * n);
+  ej.reserve(rank * n);
+  ec.reserve(rank * n);
+  C.vol.resize(n);
+  C.glo.resize(3 * n);
+  C.ghi.resize(3 * n);
+  C.gsum.assign(3 * n, 0.0);
+  for (int b = 0; b < T.nb; ++b)
+    for (long long c = T.off[b]; c < T.off[b + 1]; ++c) {
+      C.vol[c] = T.volume(b);
+      lev[c] = T.level[b];
+    }
+  for (long long c = 0; c < n; ++c)
+    for (int d = 0; d < 3; ++d) C.glo[d * n + c] = C.ghi[d * n + c] = (int)c;
+  for (int b = 0; b < T.nb; ++b) {
+    const int* nn = &T.dims[3 * b];
+    for (long long k = 0; k < nn[2]; ++k)
+      for (long long j = 0; j < nn[1]; ++j)
+        for (long long i = 0; i < nn[0]; ++i) {
+          const long long ijk[3] = {i, j, k};
+          const long long g = T.index(b, ijk);
+          for (int d = 0; d < rank; ++d) {
+            const double area = T.area(b, d);
+            for (int side = 1; side >= 0; --side) {
+              long long rep = -1;
+              int nbb = -1;
+              const int kind = T.neighbor_at(b, ijk, d, side, &rep, nullptr, nullptr, &nbb);
+              if (kind < 0) return fail(-1, "invalid neighbour while building the Poisson operator%s", "");
+              if (!(kind == NB_COARSER || (side == 1 && kind == NB_SAME))) continue;
+              const long long li = side ? g : rep, rj = side ? rep : g;
+              const int bi = side ? b : nbb, bj = side ? nbb : b;
+              const double fd = 0.5 * (T.width(bi, d) + T.width(bj, d));
+              ei.push_back((int)li);
+              ej.push_back((int)rj);
+              ec.push_back(area / fd);
+              C.ghi[d * n + li] = (int)rj;
+              dhi[3 * li + d] = fd;
+              C.glo[d * n + rj] = (int)li;
+              dlo[3 * rj + d] = fd;
+              if (lev[li] != lev[rj]) marked[lev[li] < lev[rj] ? li : rj] = 1;
+            }
+          }
+        }
+  }
+  C.rowptr.assign(n + 1, 0);
+  for (size_t e = 0; e < ei.size(); ++e) {
+    C.rowptr[ei[e] + 1]++;
+    C.rowptr[ej[e] + 1]++;
+  }
+  for (long long c = 0; c < n; ++c) C.rowptr[c + 1] += C.rowptr[c];
+  C.colidx.resize(C.rowptr[n]);
+  C.coef.resize(C.rowptr[n]);
+  std::vector<int> cur(C.rowptr.begin(), C.rowptr.end());
+  for (size_t e = 0; e < ei.size(); ++e) {
+    const int p = cur[ei[e]]++;
+    C.colidx[p] = ej[e];
+    C.coef[p] = ec[e];
+    const int q = cur[ej[e]]++;
+    C.colidx[q] = ei[e];
+    C.coef[q] = ec[e];
+  }
+  for (int b = 0; b < T.nb; ++b) {
+    const int* nn = &T.dims[3 * b];
+    for (long long k = 0; k < nn[2]; ++k)
+      for (long long j = 0; j < nn[1]; ++j)
+        for (long long i = 0; i < nn[0]; ++i) {
+          const long long ijk[3] = {i, j, k};
+          const long long c = T.index(b, ijk);
+          for (int d = 0; d < rank; ++d) {
+            const double hw = 0.5 * T.width(b, d);
+            double dl, dh;
+            if (!marked[c]) {
+              dl = dlo[3 * c + d] == 0.0 ? hw : dlo[3 * c + d];
+              dh = dhi[3 * c + d] == 0.0 ? hw : dhi[3 * c + d];
+            } else {
+              long long rep = -1;
+              int nbb = -1;
+              if (T.neighbor_at(b, ijk, d, 0, &rep, nullptr, nullptr, &nbb) == NB_BOUNDARY) {
+                C.glo[d * n + c] = (int)c;
+                dl = hw;
+              } else {
+                C.glo[d * n + c] = (int)rep;
+                dl = 0.5 * (T.width(b, d) + T.width(nbb, d));
+              }
+              if (T.neighbor_at(b, ijk, d, 1, &rep, nullptr, nullptr, &nbb) == NB_BOUNDARY) {
+                C.ghi[d * n + c] = (int)c;
+                dh = hw;
+              } else {
+                C.ghi[d * n + c] = (int)rep;
+                dh = 0.5 * (T.width(b, d) + T.width(nbb, d));
+              }
+            }
+            C.gsum[d * n + c] = dl + dh;
+          }
+        }
+  }
+  return 0;
+}
+
+static void free_grav(vh_blocks* m) {
+  BlockGrav* G = m->grav;
+  if (!G) return;
+  for (void* p : {(void*)G->rowptr, (void*)G->colidx, (void*)G->coef, (void*)G->vol, (void*)G->glo, (void*)G->ghi,
+                  (void*)G->gsum, G->g, G->B.phi, G->B.work, G->B.state, (void*)G->B.scal, (void*)G->B.partials})
+    (void)hipFree(p);
+  if (G->B.host_state) (void)hipHostFree(G->B.host_state);
+  delete G;
+  m->grav = nullptr;
+}
+
+// g of the current φ into the step's field (before a step with self-gravity).
+static void enqueue_grav_accel(vh_blocks* m) {
+  grav_csr_accel(m->stream, m->dtype, m->grav->A, m->grav->B.phi, m->grav->g);
 }
 
 extern "C" {
@@ -1118,6 +1575,39 @@ int vh_blocks_describe(int32_t rank, int32_t nblocks, int32_t maxlevel, const do
   counts6[3] = (int64_t)H.linkcells.size();
   counts6[4] = nreg;
   counts6[5] = (int64_t)H.slot_faces.size();
+  return 0;
+}
+
+int vh_blocks_table_digest(int32_t rank, int32_t nblocks, int32_t maxlevel, const double* h0, const int32_t* level,
+                           const int32_t* dims, const int64_t* origin, const int32_t* bc, int32_t zchunk,
+                           int32_t nthreads, uint64_t* digest) {
+  if (!digest) return fail(-1, "null argument%s", "");
+  if (nthreads < 0) return fail(-1, "nthreads must be >= 0%s", "");
+  Topo T;
+  int rc = make_topo(T, rank, nblocks, maxlevel, h0, level, dims, origin, bc);
+  if (rc) return rc;
+  HostTables H;
+  rc = build_tables(T, zchunk, H, 32, nthreads);
+  if (rc) return rc;
+  uint64_t h = 1469598103934665603ULL;
+  digest_vec(h, H.blocks);
+  digest_vec(h, H.repmap);
+  digest_vec(h, H.lcmap);
+  digest_vec(h, H.faces);
+  digest_vec(h, H.slot_faces);
+  digest_vec(h, H.linkcells);
+  for (const RegList& r : H.regs) {
+    digest_vec(h, r.cell);
+    digest_vec(h, r.lc);
+    digest_vec(h, r.block);
+  }
+  digest_vec(h, H.tiles);
+  digest_vec(h, H.tile_list_off);
+  digest_vec(h, H.block_tile_start);
+  digest_vec(h, H.cfl_blocks);
+  digest_vec(h, H.cfl_prefix);
+  digest_vec(h, H.cfl_list_off);
+  *digest = h;
   return 0;
 }
 
@@ -1150,7 +1640,8 @@ int vh_blocks_create(vh_blocks** out, int32_t rank, int32_t nblocks, int32_t max
     vh_blocks_destroy(m);
     return fail(-2, "device allocation of %s%lld bytes failed", "", (long long)(2 * bytes));
   }
-  rc = upload_tables(m);
+  m->state_cap = bytes;
+  rc = upload_tables(m, m->H, m->T.nb, 0);
   if (!rc) rc = alloc_partials(m);
   if (rc) { vh_blocks_destroy(m); return rc; }
   VH_HIP(hipMemsetAsync(m->clock, 0, 4 * sizeof(double), m->stream));
@@ -1164,6 +1655,7 @@ int vh_blocks_destroy(vh_blocks* m) {
   if (!m) return 0;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   drop_graph(m);
+  free_grav(m);
   free_tables(m);
   (void)hipFree(m->U);
   (void)hipFree(m->U1);
@@ -1212,6 +1704,7 @@ int vh_blocks_step(vh_blocks* m, double dt, int32_t level) {
   if (!m) return fail(-1, "null argument%s", "");
   if (level > m->T.maxlevel) return fail(-1, "level out of range%s (%lld)", "", level);
   VH_HIP(hipSetDevice(m->device));
+  if (m->grav) enqueue_grav_accel(m);   // φ (hence g) held over the step
   enqueue_set_dt(m->stream, m->clock, dt);
   enqueue_step(m, level < 0 ? -1 : level, dt, ActiveRegs());
   VH_HIP(hipGetLastError());
@@ -1231,8 +1724,17 @@ int vh_blocks_run(vh_blocks* m, double cfl, double tfinal, int64_t max_steps, in
   double t = *t_io;
   int rc = 0;
   if (subcycle) {
-    // evolve! with subcycling (src/driver.jl:581-605): root dt from level 0.
+    // evolve! with subcycling (src/driver.jl:581-605): root dt from level 0;
+    // with self-gravity one Poisson solve per root step, g held over every
+    // substep of every level (src/driver.jl:589-592)
     while (t < tfinal * (1 - 1e-12) && (max_steps <= 0 || k < max_steps)) {
+      if (m->grav) {
+        long long it = 0;
+        double rr = 0.0;
+        m->grav->B.rho = m->U;
+        if ((rc = grav_csr_solve(m->stream, m->dtype, m->grav->A, m->grav->P, m->grav->B, &it, &rr))) return rc;
+        enqueue_grav_accel(m);
+      }
       double dt = 0.0;
       if ((rc = compute_dt_sync(m, cfl, 0, &dt))) return rc;
       dt = std::fmin(dt, tfinal - t);
@@ -1250,7 +1752,22 @@ int vh_blocks_run(vh_blocks* m, double cfl, double tfinal, int64_t max_steps, in
     while (true) {
       if (max_steps > 0 && k >= max_steps) break;
       if (clip && !(t < tfinal * (1 - 1e-12))) break;
-      if ((rc = single_rate_step(m, cfl, tfinal, clip ? 1 : 0, np))) return rc;
+      if (m->grav) {
+        // solve_poisson! before each step (src/driver.jl:498), g held over the step;
+        // the CG loop synchronises, so these steps are enqueued eagerly
+        long long it = 0;
+        double rr = 0.0;
+        m->grav->B.rho = m->U;
+        if ((rc = grav_csr_solve(m->stream, m->dtype, m->grav->A, m->grav->P, m->grav->B, &it, &rr))) return rc;
+        enqueue_grav_accel(m);
+        const int saved = m->use_graph;
+        m->use_graph = 0;
+        rc = single_rate_step(m, cfl, tfinal, clip ? 1 : 0, np);
+        m->use_graph = saved;
+        if (rc) return rc;
+      } else if ((rc = single_rate_step(m, cfl, tfinal, clip ? 1 : 0, np))) {
+        return rc;
+      }
       np = list_wgs(m, 0);
       ++k;
       if (clip) {
@@ -1263,6 +1780,123 @@ int vh_blocks_run(vh_blocks* m, double cfl, double tfinal, int64_t max_steps, in
   }
   *t_io = t;
   if (steps) *steps = k;
+  return 0;
+}
+
+int vh_blocks_enable_gravity(vh_blocks* m, double G, int64_t maxiter, double tol, int32_t project_mean) {
+  if (!m) return fail(-1, "null argument%s", "");
+  if (!(G >= 0.0) || maxiter < 0 || !(tol > 0.0)) return fail(-1, "bad gravity parameters%s", "");
+  VH_HIP(hipSetDevice(m->device));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  drop_graph(m);    // captured steps hold the old kernel arguments
+  free_grav(m);
+  HostCsr C;
+  int rc = build_poisson(m->T, C);
+  if (rc) return rc;
+  BlockGrav* Gv = new BlockGrav();
+  m->grav = Gv;
+  const long long n = m->T.off[m->T.nb];
+  Gv->P.G = G;
+  Gv->P.maxiter = maxiter;
+  Gv->P.tol = tol;
+  Gv->P.project_mean = project_mean ? 1 : 0;
+  if ((rc = upload_vec(&Gv->rowptr, C.rowptr)) || (rc = upload_vec(&Gv->colidx, C.colidx)) ||
+      (rc = upload_vec(&Gv->coef, C.coef)) || (rc = upload_vec(&Gv->vol, C.vol)) ||
+      (rc = upload_vec(&Gv->glo, C.glo)) || (rc = upload_vec(&Gv->ghi, C.ghi)) || (rc = upload_vec(&Gv->gsum, C.gsum))) {
+    free_grav(m);
+    return rc;
+  }
+  hipError_t e = hipMalloc(&Gv->B.phi, m->esize * n);
+  if (e == hipSuccess) e = hipMalloc(&Gv->B.work, 4 * m->esize * n);
+  if (e == hipSuccess) e = hipMalloc(&Gv->g, 3 * m->esize * n);
+  if (e == hipSuccess) e = hipMalloc(&Gv->B.state, kGravStateBytes);
+  if (e == hipSuccess) e = hipMalloc(&Gv->B.scal, 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&Gv->B.partials, 4096 * sizeof(double));
+  if (e == hipSuccess) e = hipHostMalloc(&Gv->B.host_state, kGravStateBytes, 0);
+  if (e == hipSuccess) e = hipMemsetAsync(Gv->B.phi, 0, m->esize * n, m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(Gv->g, 0, 3 * m->esize * n, m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(Gv->B.scal, 0, 8 * sizeof(double), m->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(Gv->B.state, 0, kGravStateBytes, m->stream);
+  // Σ V accumulated cell by cell in leaf order in Float64 (the mean projections'
+  // denominator, src/gravity.jl:230-246, 310-320)
+  double sumv = 0.0;
+  for (long long i = 0; i < n; ++i) sumv += C.vol[i];
+  if (e == hipSuccess) e = hipMemcpyAsync(Gv->B.scal + 1, &sumv, sizeof(double), hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    free_grav(m);
+    return fail(-2, "HIP error %s while enabling gravity%s", hipGetErrorString(e));
+  }
+  Gv->A.n = n;
+  Gv->A.rowptr = Gv->rowptr;
+  Gv->A.colidx = Gv->colidx;
+  Gv->A.coef = Gv->coef;
+  Gv->A.vol = Gv->vol;
+  Gv->A.glo = Gv->glo;
+  Gv->A.ghi = Gv->ghi;
+  Gv->A.gsum = Gv->gsum;
+  Gv->A.rank = m->T.rank;
+  Gv->B.rho = m->U;
+  return 0;
+}
+
+int vh_blocks_disable_gravity(vh_blocks* m) {
+  if (!m) return fail(-1, "null argument%s", "");
+  VH_HIP(hipStreamSynchronize(m->stream));
+  drop_graph(m);
+  free_grav(m);
+  return 0;
+}
+
+static int need_grav(vh_blocks* m) {
+  if (!m) return fail(-1, "null argument%s", "");
+  if (!m->grav) return fail(-1, "gravity is not enabled on this mesh%s", "");
+  return 0;
+}
+
+int vh_blocks_solve_poisson(vh_blocks* m, int64_t* iters, double* relres) {
+  if (int rc = need_grav(m)) return rc;
+  VH_HIP(hipSetDevice(m->device));
+  long long it = 0;
+  double rr = 0.0;
+  m->grav->B.rho = m->U;
+  if (int rc = grav_csr_solve(m->stream, m->dtype, m->grav->A, m->grav->P, m->grav->B, &it, &rr)) return rc;
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return 0;
+}
+
+int vh_blocks_phi_upload(vh_blocks* m, const void* host) {
+  if (int rc = need_grav(m)) return rc;
+  if (!host) return fail(-1, "null argument%s", "");
+  VH_HIP(hipMemcpyAsync(m->grav->B.phi, host, m->esize * m->T.off[m->T.nb], hipMemcpyHostToDevice, m->stream));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int vh_blocks_phi_download(vh_blocks* m, void* host) {
+  if (int rc = need_grav(m)) return rc;
+  if (!host) return fail(-1, "null argument%s", "");
+  VH_HIP(hipMemcpyAsync(host, m->grav->B.phi, m->esize * m->T.off[m->T.nb], hipMemcpyDeviceToHost, m->stream));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int vh_blocks_apply_laplacian(vh_blocks* m, const void* dx, void* dout) {
+  if (int rc = need_grav(m)) return rc;
+  if (!dx || !dout) return fail(-1, "null argument%s", "");
+  grav_csr_apply(m->stream, m->dtype, m->grav->A, dx, dout);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int vh_blocks_gravity_accel(vh_blocks* m, void* dg) {
+  if (int rc = need_grav(m)) return rc;
+  if (!dg) return fail(-1, "null argument%s", "");
+  grav_csr_accel(m->stream, m->dtype, m->grav->A, m->grav->B.phi, dg);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(m->stream));
   return 0;
 }
 
@@ -1296,13 +1930,12 @@ int vh_blocks_set_launch(vh_blocks* m, int32_t zchunk, int32_t max_workgroups, i
   m->use_graph = use_graph ? 1 : 0;
   m->max_wgs = max_workgroups;
   if (zchunk != m->zchunk) {
-    HostTables H;
+    HostTables& H = m->H2;
     int rc = build_tables(m->T, zchunk, H, m->max_tx);
     if (rc) return rc;
-    free_tables(m);
-    m->H = std::move(H);
+    std::swap(m->H, H);
     m->zchunk = zchunk;
-    rc = upload_tables(m);
+    rc = upload_tables(m, m->H, m->T.nb, m->arena_cur);   // the stream is idle: the arena can be rewritten
     if (rc) return rc;
   }
   return alloc_partials(m);
@@ -1314,14 +1947,13 @@ int vh_blocks_set_tiling(vh_blocks* m, int32_t max_tile_x) {
   if (max_tile_x == m->max_tx) return 0;
   VH_HIP(hipSetDevice(m->device));
   VH_HIP(hipStreamSynchronize(m->stream));
-  HostTables H;
+  HostTables& H = m->H2;
   int rc = build_tables(m->T, m->zchunk, H, max_tile_x);
   if (rc) return rc;
   drop_graph(m);
-  free_tables(m);
-  m->H = std::move(H);
+  std::swap(m->H, H);
   m->max_tx = max_tile_x;
-  rc = upload_tables(m);
+  rc = upload_tables(m, m->H, m->T.nb, m->arena_cur);   // the stream is idle: the arena can be rewritten
   if (rc) return rc;
   return alloc_partials(m);
 }
@@ -1367,17 +1999,13 @@ int vh_blocks_indicator(vh_blocks* m, double* block_max, double* cell_g) {
   return 0;
 }
 
-int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_block) {
-  if (!dst || !src || !src_block) return fail(-1, "null argument%s", "");
-  if (dst == src) return fail(-1, "remap needs two different contexts%s", "");
-  const Topo& D = dst->T;
-  const Topo& S = src->T;
-  if (dst->dtype != src->dtype) return fail(-1, "remap between contexts of different dtypes%s", "");
-  if (dst->device != src->device) return fail(-1, "remap between contexts on different devices%s", "");
+// The injection map of a remap from mesh S onto mesh D (src_block[b] = the
+// block of S containing block b of D, at the same or a coarser level).
+static int build_remap(const Topo& D, const Topo& S, const int32_t* src_block, std::vector<RemapDev>& map) {
   if (D.rank != S.rank || D.maxlevel != S.maxlevel) return fail(-1, "remap between meshes of different rank or maxlevel%s", "");
   for (int d = 0; d < D.rank; ++d)
     if (D.h0[d] != S.h0[d]) return fail(-1, "remap between meshes of different base cell widths%s (axis %lld)", "", d);
-  std::vector<RemapDev> map(D.nb);
+  map.resize(D.nb);
   for (int b = 0; b < D.nb; ++b) {
     const int p = src_block[b];
     if (p < 0 || p >= S.nb) return fail(-1, "block %s%lld: source block out of range", "", b);
@@ -1398,20 +2026,113 @@ int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_blo
       map[b].o[d] = (int)o;
     }
   }
+  return 0;
+}
+
+int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_block) {
+  if (!dst || !src || !src_block) return fail(-1, "null argument%s", "");
+  if (dst == src) return fail(-1, "remap needs two different contexts%s", "");
+  const Topo& D = dst->T;
+  const Topo& S = src->T;
+  if (dst->dtype != src->dtype) return fail(-1, "remap between contexts of different dtypes%s", "");
+  if (dst->device != src->device) return fail(-1, "remap between contexts on different devices%s", "");
+  std::vector<RemapDev> map;
+  int rc = build_remap(D, S, src_block, map);
+  if (rc) return rc;
   VH_HIP(hipSetDevice(dst->device));
   VH_HIP(hipStreamSynchronize(src->stream));
   RemapDev* dmap = nullptr;
-  int rc = upload_vec(&dmap, map);
+  rc = upload_vec(&dmap, map);
   if (rc) {
     (void)hipFree(dmap);
     return rc;
   }
   regrid_remap(dst->stream, dst->dtype, D.nb, dst->d_blocks, src->d_blocks, dmap, src->U, S.off[S.nb], dst->U,
-               D.off[D.nb]);
+               D.off[D.nb], 5);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
   (void)hipFree(dmap);
   if (e != hipSuccess) return fail(-2, "HIP error %s in the remap%s", hipGetErrorString(e));
+  return 0;
+}
+
+int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const int32_t* dims, const int64_t* origin,
+                     const int32_t* bc, const int32_t* src_block) {
+  if (!m || !src_block) return fail(-1, "null argument%s", "");
+  Topo& N = m->T2;
+  HostTables& H = m->H2;
+  const double h0[3] = {m->T.h0[0], m->T.h0[1], m->T.h0[2]};
+  int rc = make_topo(N, m->T.rank, nblocks, m->T.maxlevel, h0, level, dims, origin, bc);
+  if (!rc) rc = build_tables(N, m->zchunk, H, m->max_tx);
+  std::vector<RemapDev> map;
+  if (!rc) rc = build_remap(N, m->T, src_block, map);
+  if (rc) return rc;
+  VH_HIP(hipSetDevice(m->device));
+  VH_HIP(hipStreamSynchronize(m->stream));   // the staging and the spare arena are free
+  drop_graph(m);                             // captured steps hold the old tables
+  const long long n_old = m->T.off[m->T.nb], n_new = N.off[N.nb];
+  const BlockDev* old_blocks = m->d_blocks;
+  const int old_arena = m->arena_cur;
+  // gravity: φ carried over by the same injection (the next solve's warm start)
+  void* phi_new = nullptr;
+  GravParams gp{};
+  const bool grav = m->grav != nullptr;
+  if (grav) {
+    gp = m->grav->P;
+    VH_HIP(hipMalloc(&phi_new, m->esize * std::max<long long>(1, n_new)));
+  }
+  void* dmap = nullptr;
+  rc = upload_tables(m, H, N.nb, 1 - old_arena, map.data(), sizeof(RemapDev) * map.size(), &dmap);
+  if (rc) {
+    (void)hipFree(phi_new);
+    return rc;   // the context keeps no valid tables: destroy it
+  }
+  // the state: remapped into U1 when it fits (then U and U1 swap), else into a
+  // new allocation with headroom
+  const size_t need = (size_t)5 * n_new * m->esize;
+  void* dst = m->U1;
+  bool fresh = false;
+  if (need > m->state_cap) {
+    const size_t cap = need + need / 4;
+    VH_HIP(hipMalloc(&dst, cap));
+    fresh = true;
+    m->state_cap = cap;
+  }
+  regrid_remap(m->stream, m->dtype, N.nb, m->d_blocks, old_blocks, (const RemapDev*)dmap, m->U, n_old, dst, n_new, 5);
+  if (grav)
+    regrid_remap(m->stream, m->dtype, N.nb, m->d_blocks, old_blocks, (const RemapDev*)dmap, m->grav->B.phi, n_old,
+                 phi_new, n_new, 1);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  if (e != hipSuccess) {
+    if (fresh) (void)hipFree(dst);
+    (void)hipFree(phi_new);
+    return fail(-2, "HIP error %s in the regrid remap%s", hipGetErrorString(e));
+  }
+  if (fresh) {
+    (void)hipFree(m->U);
+    (void)hipFree(m->U1);
+    m->U = dst;
+    m->U1 = nullptr;
+    if (hipMalloc(&m->U1, m->state_cap) != hipSuccess) return fail(-2, "device allocation of %s%lld bytes failed", "",
+                                                                     (long long)m->state_cap);
+  } else {
+    std::swap(m->U, m->U1);
+  }
+  std::swap(m->T, N);
+  std::swap(m->H, H);
+  rc = alloc_partials(m);
+  if (rc) return rc;
+  if (grav) {
+    rc = vh_blocks_enable_gravity(m, gp.G, gp.maxiter, gp.tol, gp.project_mean);
+    if (!rc) {
+      e = hipMemcpyAsync(m->grav->B.phi, phi_new, m->esize * n_new, hipMemcpyDeviceToDevice, m->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+      if (e != hipSuccess) rc = fail(-2, "HIP error %s carrying φ over%s", hipGetErrorString(e));
+    }
+    (void)hipFree(phi_new);
+    if (rc) return rc;
+  }
   return 0;
 }
 

@@ -18,7 +18,7 @@
 
 namespace vh {
 
-enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2 };
+enum : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTING = 2, BC_PARENT = 3 };
 
 template <typename T>
 struct Prim {
@@ -52,6 +52,7 @@ struct GridDev {
   double gravG;    // gravitational constant
   int gper;        // bit d: φ periodic along axis d
   int pad_;
+  const void* pg;  // ParentGhost states (5 Tf primitives per boundary face slot) or null
 };
 
 // ---- batched blocks (block-structured AMR) ----
@@ -115,8 +116,10 @@ struct BlocksDev {
   const int* slot_faces;
   const LinkFaceDev* faces;
   const void* fluxbuf;     // 5 values per link face (AoS)
+  const void* gravg;       // g = −∇φ of the step (3 SoA fields of the state type) with self-gravity, else null
   long long ncell;
   double gamma, igm1;
+  double gravG;            // gravitational constant (self-gravity)
   int ntiles;
   int nfaces;
 };

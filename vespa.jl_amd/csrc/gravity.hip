@@ -107,6 +107,31 @@ __global__ void __launch_bounds__(GNT) lap_kernel(const T* __restrict__ x, T* __
   }
 }
 
+// out = A·x on the CSR operator: per row Σ coef·(x_c − x_nb) in Float64 in row
+// order (_lap_csr_cpu!, src/gravity.jl:326-336), stored at T; partial (x·out)
+// per workgroup when partials is non-null. Skipped when the state is done.
+template <typename T>
+__global__ void __launch_bounds__(GNT) csr_lap_kernel(const T* __restrict__ x, T* __restrict__ out,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      const double* __restrict__ coef, long long n,
+                                                      double* __restrict__ partials, const CgState* st) {
+  __shared__ double red[GNT / 64];
+  if (st && st->done) return;
+  double dot = 0.0;
+  for (long long c = (long long)blockIdx.x * GNT + threadIdx.x; c < n; c += (long long)gridDim.x * GNT) {
+    const double xc = (double)x[c];
+    double s = 0.0;
+    for (int k = rowptr[c]; k < rowptr[c + 1]; ++k) s += coef[k] * (xc - (double)x[colidx[k]]);
+    const T o = (T)s;
+    out[c] = o;
+    dot += xc * (double)o;
+  }
+  if (partials) {
+    dot = gsum_block(dot, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = dot;
+  }
+}
+
 // Partials of Σ a·b (b null: Σ a·w with a constant weight w).
 template <typename T>
 __global__ void __launch_bounds__(GNT) dot_kernel(const T* __restrict__ a, const T* __restrict__ b, double w,
@@ -119,13 +144,44 @@ __global__ void __launch_bounds__(GNT) dot_kernel(const T* __restrict__ a, const
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// b = T(−4πG(ρ − ρ̄)·V) with ρ̄ = sums[0]/sums[1] (0 without the projection).
+// b = T(−4πG(ρ − ρ̄)·V) with ρ̄ = sums[0]/sums[1] (0 without the projection);
+// V per cell from vols when given, else the constant vol.
 template <typename T>
 __global__ void __launch_bounds__(GNT) rhs_kernel(const T* __restrict__ rho, T* __restrict__ b, long long n, double G,
-                                                  double vol, const double* __restrict__ rbar) {
+                                                  double vol, const double* __restrict__ vols,
+                                                  const double* __restrict__ rbar) {
   const double rb = rbar ? *rbar : 0.0;
   for (long long i = (long long)blockIdx.x * GNT + threadIdx.x; i < n; i += (long long)gridDim.x * GNT)
-    b[i] = (T)(-kFourPi * G * ((double)rho[i] - rb) * vol * 1.0);
+    b[i] = (T)(-kFourPi * G * ((double)rho[i] - rb) * (vols ? vols[i] : vol) * 1.0);
+}
+
+// Partials of Σ V_i·a_i with per-cell volumes.
+template <typename T>
+__global__ void __launch_bounds__(GNT) vdot_kernel(const T* __restrict__ a, const double* __restrict__ vols,
+                                                   long long n, double* __restrict__ partials) {
+  __shared__ double red[GNT / 64];
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * GNT + threadIdx.x; i < n; i += (long long)gridDim.x * GNT)
+    s += vols[i] * (double)a[i];
+  s = gsum_block(s, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// g_d = −(φ_hi − φ_lo)/T(d_lo + d_hi) per cell from the CSR stencil (_gax,
+// src/gravity.jl:403-410), 3 SoA fields; axes beyond the rank are 0.
+template <typename T>
+__global__ void __launch_bounds__(GNT) csr_accel_kernel(const T* __restrict__ phi, T* __restrict__ g,
+                                                        const int* __restrict__ glo, const int* __restrict__ ghi,
+                                                        const double* __restrict__ gsum, long long n, int rank) {
+  for (long long c = (long long)blockIdx.x * GNT + threadIdx.x; c < n; c += (long long)gridDim.x * GNT)
+    for (int d = 0; d < 3; ++d) {
+      if (d >= rank) {
+        g[d * n + c] = T(0);
+        continue;
+      }
+      const T wlo = phi[glo[d * n + c]], whi = phi[ghi[d * n + c]];
+      g[d * n + c] = -(whi - wlo) / T(gsum[d * n + c]);
+    }
 }
 
 // Single-workgroup sum of np partials into *out (fixed order per lane, then the tree).
@@ -273,13 +329,48 @@ void grav_accel(hipStream_t s, int dtype, const GridDev& G, int gper, const void
     hipLaunchKernelGGL(accel_kernel<float>, dim3(nb), dim3(GNT), 0, s, (const float*)phi, (float*)g, G, gper);
 }
 
-template <typename T>
-static int solve_t(hipStream_t s, const GridDev& G, const GravParams& P, GravBuffers& B, long long* iters,
+// The two operator forms of the CG driver: the uniform stencil (constant cell
+// volume) and the CSR composite operator of a block mesh (per-cell volumes).
+struct UniformOp {
+  const GridDev* G;
+  int gper;
+  double coef[3];
+  long long n() const { return G->ncell; }
+  const double* vols() const { return nullptr; }
+  double vol() const { return G->V; }
+  template <typename T>
+  void lap(hipStream_t s, int nb, const T* x, T* out, double* partials, const CgState* st) const {
+    hipLaunchKernelGGL(lap_kernel<T>, dim3(nb), dim3(GNT), 0, s, x, out, *G, gper, coef[0], coef[1], coef[2], partials,
+                       st);
+  }
+};
+struct CsrOp {
+  const GravCsr* A;
+  long long n() const { return A->n; }
+  const double* vols() const { return A->vol; }
+  double vol() const { return 0.0; }
+  template <typename T>
+  void lap(hipStream_t s, int nb, const T* x, T* out, double* partials, const CgState* st) const {
+    hipLaunchKernelGGL(csr_lap_kernel<T>, dim3(nb), dim3(GNT), 0, s, x, out, A->rowptr, A->colidx, A->coef, A->n,
+                       partials, st);
+  }
+};
+
+// Σ V·a into *out (constant or per-cell volume).
+template <typename T, typename Op>
+static void vsum(hipStream_t s, int nb, const Op& op, const T* a, double* partials, double* out) {
+  if (op.vols())
+    hipLaunchKernelGGL(vdot_kernel<T>, dim3(nb), dim3(GNT), 0, s, a, op.vols(), op.n(), partials);
+  else
+    hipLaunchKernelGGL(dot_kernel<T>, dim3(nb), dim3(GNT), 0, s, a, (const T*)nullptr, op.vol(), op.n(), partials);
+  hipLaunchKernelGGL(psum_kernel, dim3(1), dim3(GNT), 0, s, partials, nb, out);
+}
+
+template <typename T, typename Op>
+static int solve_t(hipStream_t s, const Op& op, const GravParams& P, GravBuffers& B, long long* iters,
                    double* relres) {
-  const long long n = G.ncell;
+  const long long n = op.n();
   const int nb = grid_blocks(n);
-  double coef[3] = {0, 0, 0};
-  for (int d = 0; d < G.rank; ++d) coef[d] = G.area[d] / (0.5 * (G.w[d] + G.w[d]));
   T* phi = (T*)B.phi;
   T* r = (T*)B.work;
   T* p = r + n;
@@ -290,15 +381,13 @@ static int solve_t(hipStream_t s, const GridDev& G, const GravParams& P, GravBuf
   double* scal = B.scal;
   // right-hand side (src/gravity.jl:230-246)
   if (P.project_mean) {
-    hipLaunchKernelGGL(dot_kernel<T>, dim3(nb), dim3(GNT), 0, s, rho, (const T*)nullptr, G.V, n, B.partials);
-    hipLaunchKernelGGL(psum_kernel, dim3(1), dim3(GNT), 0, s, B.partials, nb, scal + 0);
+    vsum<T>(s, nb, op, rho, B.partials, scal + 0);
     hipLaunchKernelGGL(ratio_kernel, dim3(1), dim3(1), 0, s, scal);
   }
-  hipLaunchKernelGGL(rhs_kernel<T>, dim3(nb), dim3(GNT), 0, s, rho, b, n, P.G, G.V,
+  hipLaunchKernelGGL(rhs_kernel<T>, dim3(nb), dim3(GNT), 0, s, rho, b, n, P.G, op.vol(), op.vols(),
                      P.project_mean ? (const double*)(scal + 2) : (const double*)nullptr);
   // r = b − Aφ, p = r (src/gravity.jl:280-289)
-  hipLaunchKernelGGL(lap_kernel<T>, dim3(nb), dim3(GNT), 0, s, (const T*)phi, Ap, G, P.gper, coef[0], coef[1], coef[2],
-                     (double*)nullptr, (const CgState*)nullptr);
+  op.template lap<T>(s, nb, (const T*)phi, Ap, (double*)nullptr, (const CgState*)nullptr);
   hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), dim3(GNT), 0, s, (const T*)b, (const T*)Ap, r, p, n);
   hipLaunchKernelGGL(dot_kernel<T>, dim3(nb), dim3(GNT), 0, s, (const T*)r, (const T*)r, 0.0, n, B.partials);
   hipLaunchKernelGGL(psum_kernel, dim3(1), dim3(GNT), 0, s, B.partials, nb, scal + 3);
@@ -310,8 +399,7 @@ static int solve_t(hipStream_t s, const GridDev& G, const GravParams& P, GravBuf
   for (long long done_it = 0; done_it < P.maxiter;) {
     const long long chunk = std::min<long long>(P.chunk, P.maxiter - done_it);
     for (long long k = 0; k < chunk; ++k) {
-      hipLaunchKernelGGL(lap_kernel<T>, dim3(nb), dim3(GNT), 0, s, (const T*)p, Ap, G, P.gper, coef[0], coef[1],
-                         coef[2], B.partials, (const CgState*)st);
+      op.template lap<T>(s, nb, (const T*)p, Ap, B.partials, (const CgState*)st);
       hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(GNT), 0, s, (const double*)B.partials, nb, st);
       hipLaunchKernelGGL(cg_update_kernel<T>, dim3(nb), dim3(GNT), 0, s, phi, r, (const T*)p, (const T*)Ap, n,
                          B.partials, (const CgState*)st);
@@ -327,9 +415,7 @@ static int solve_t(hipStream_t s, const GridDev& G, const GravParams& P, GravBuf
   }
   // volume-weighted mean projection (src/gravity.jl:310-320)
   if (P.project_mean) {
-    hipLaunchKernelGGL(dot_kernel<T>, dim3(nb), dim3(GNT), 0, s, (const T*)phi, (const T*)nullptr, G.V, n,
-                       B.partials);
-    hipLaunchKernelGGL(psum_kernel, dim3(1), dim3(GNT), 0, s, B.partials, nb, scal + 5);
+    vsum<T>(s, nb, op, (const T*)phi, B.partials, scal + 5);
     hipLaunchKernelGGL(sub_mean_kernel<T>, dim3(nb), dim3(GNT), 0, s, phi, n, (const double*)scal);
   }
   VH_HIP(hipGetLastError());
@@ -341,8 +427,34 @@ static int solve_t(hipStream_t s, const GridDev& G, const GravParams& P, GravBuf
 
 int grav_solve(hipStream_t s, int dtype, const GridDev& G, const GravParams& P, GravBuffers& B, long long* iters,
                double* relres) {
-  if (dtype == 0) return solve_t<double>(s, G, P, B, iters, relres);
-  return solve_t<float>(s, G, P, B, iters, relres);
+  UniformOp op{&G, P.gper, {0, 0, 0}};
+  for (int d = 0; d < G.rank; ++d) op.coef[d] = G.area[d] / (0.5 * (G.w[d] + G.w[d]));
+  if (dtype == 0) return solve_t<double>(s, op, P, B, iters, relres);
+  return solve_t<float>(s, op, P, B, iters, relres);
+}
+
+void grav_csr_apply(hipStream_t s, int dtype, const GravCsr& A, const void* x, void* out) {
+  const CsrOp op{&A};
+  const int nb = grid_blocks(A.n);
+  if (dtype == 0) op.lap<double>(s, nb, (const double*)x, (double*)out, nullptr, nullptr);
+  else op.lap<float>(s, nb, (const float*)x, (float*)out, nullptr, nullptr);
+}
+
+void grav_csr_accel(hipStream_t s, int dtype, const GravCsr& A, const void* phi, void* g) {
+  const int nb = grid_blocks(A.n);
+  if (dtype == 0)
+    hipLaunchKernelGGL(csr_accel_kernel<double>, dim3(nb), dim3(GNT), 0, s, (const double*)phi, (double*)g, A.glo,
+                       A.ghi, A.gsum, A.n, A.rank);
+  else
+    hipLaunchKernelGGL(csr_accel_kernel<float>, dim3(nb), dim3(GNT), 0, s, (const float*)phi, (float*)g, A.glo,
+                       A.ghi, A.gsum, A.n, A.rank);
+}
+
+int grav_csr_solve(hipStream_t s, int dtype, const GravCsr& A, const GravParams& P, GravBuffers& B, long long* iters,
+                   double* relres) {
+  const CsrOp op{&A};
+  if (dtype == 0) return solve_t<double>(s, op, P, B, iters, relres);
+  return solve_t<float>(s, op, P, B, iters, relres);
 }
 
 }  // namespace vh

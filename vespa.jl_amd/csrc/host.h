@@ -76,6 +76,27 @@ void grav_accel(hipStream_t s, int dtype, const GridDev& G, int gper, const void
 int grav_solve(hipStream_t s, int dtype, const GridDev& G, const GravParams& P, GravBuffers& B, long long* iters,
                double* relres);
 
+// The composite operator of a block mesh in CSR form (the reference's
+// PoissonKAStruct, src/gravity.jl:455-600), built on the host per mesh: row c
+// lists its face neighbours in the face walk's order with coef = area/d, and
+// the gravity stencil gives per cell and axis the lo/hi neighbour (the cell
+// itself at a Neumann side) and d_lo + d_hi. Device pointers.
+struct GravCsr {
+  long long n = 0;
+  const int* rowptr = nullptr;      // n + 1
+  const int* colidx = nullptr;
+  const double* coef = nullptr;
+  const double* vol = nullptr;      // per-cell volume
+  const int* glo = nullptr;         // 3·n, [axis][cell]
+  const int* ghi = nullptr;
+  const double* gsum = nullptr;     // 3·n, [axis][cell]
+  int rank = 1;
+};
+void grav_csr_apply(hipStream_t s, int dtype, const GravCsr& A, const void* x, void* out);
+void grav_csr_accel(hipStream_t s, int dtype, const GravCsr& A, const void* phi, void* g);
+int grav_csr_solve(hipStream_t s, int dtype, const GravCsr& A, const GravParams& P, GravBuffers& B, long long* iters,
+                   double* relres);
+
 }  // namespace vh
 
 namespace vh {
@@ -90,6 +111,6 @@ struct RemapDev {
 void regrid_indicator(hipStream_t s, int dtype, int nblocks, const BlockDev* blocks, const int* repmap, int rank,
                       const void* rho, double* cell_g, double* block_max);
 void regrid_remap(hipStream_t s, int dtype, int ndst, const BlockDev* dst_blocks, const BlockDev* src_blocks,
-                  const RemapDev* map, const void* Us, long long ns, void* Ud, long long nd);
+                  const RemapDev* map, const void* Us, long long ns, void* Ud, long long nd, int nf);
 
 }  // namespace vh

@@ -411,7 +411,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MUSCL_WAVES_F32 : 
       for (int q = 0; q < NF; ++q) {
         const T us = Us[q * n + c];
         if constexpr (STAGE2) o[q] = T(0.5) * (U0[q * n + c] + us + acc[q]);
-        else o[q] = us + acc[q];
+        else o[q] = U0[q * n + c] + acc[q];   // U0: the stage's base (Us itself unless bc! refilled a copy)
       }
       if constexpr (STAGE2 && NF == 6) {
         // dual_energy_sync! (lib/PPMKernels/src/muscl_grid.jl:97-107) on the new state
@@ -472,8 +472,9 @@ inline void muscl_launch_nf(bool stage2, bool rec, const MusclDev& G, hipStream_
   const unsigned nb = (unsigned)(G.tiles_x * G.tiles_y * G.chunks_z);
   T* fr = (T*)frec;
   if (!stage2) {
-    if (rec) muscl_stage_kernel<T, TX, TY, false, true, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, nullptr, (T*)Uout, fr, G);
-    else muscl_stage_kernel<T, TX, TY, false, false, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, nullptr, (T*)Uout, fr, G);
+    // stage 1 reads its base (U0, Us unless bc! refilled a copy) for every cell
+    if (rec) muscl_stage_kernel<T, TX, TY, false, true, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, G);
+    else muscl_stage_kernel<T, TX, TY, false, false, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, G);
   } else {
     if (rec) muscl_stage_kernel<T, TX, TY, true, true, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, G);
     else muscl_stage_kernel<T, TX, TY, true, false, NF><<<nb, TX * TY, 0, s>>>((const T*)Us, (const T*)U0, (T*)Uout, fr, G);

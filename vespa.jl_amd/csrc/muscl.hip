@@ -133,6 +133,13 @@ struct vh_muscl {
   double* partials = nullptr;
   hipStream_t stream = nullptr;
   bool state_ptr_shared = false;   // vh_muscl_state_device_ptr has handed out U
+  // bc! (vh_muscl_set_bc): the refill kind, the caller's hook, and the buffer
+  // holding the refilled copy of a step's input
+  int bc_kind = VH_MUSCL_BC_NONE;
+  vh_muscl_bc_fn bc_fn = nullptr;
+  void* bc_user = nullptr;
+  void* Ub = nullptr;
+  size_t ub_bytes = 0;
 };
 
 namespace {
@@ -148,18 +155,58 @@ void set_tiles(vh_muscl* m) {
 }
 
 void launch_stage(vh_muscl* m, bool stage2, bool rec, const void* Us, const void* U0, void* Uout) {
+  if (!U0) U0 = Us;   // both stages read U0 (stage 1: its base)
   if (m->mode == VH_FAST)
     fast::muscl_launch_stage(m->dtype, m->variant, m->G.nf, stage2, rec, m->G, m->stream, Us, U0, Uout, m->frec);
   else
     exact::muscl_launch_stage(m->dtype, m->variant, m->G.nf, stage2, rec, m->G, m->stream, Us, U0, Uout, m->frec);
 }
 
+void launch_fill_periodic(vh_muscl* m, void* state) {
+  const unsigned nb = (unsigned)std::min<long long>(4096, (m->G.ncell + 255) / 256);
+  if (m->dtype == VH_F64) fill_periodic_kernel<double><<<nb, 256, 0, m->stream>>>((double*)state, m->G);
+  else fill_periodic_kernel<float><<<nb, 256, 0, m->stream>>>((float*)state, m->G);
+}
+
+// bc! before a stage (lib/PPMKernels/src/muscl_grid.jl:316-322, 339, 350).
+int apply_bc(vh_muscl* m, void* state, int stage) {
+  if (m->bc_kind == VH_MUSCL_BC_PERIODIC) {
+    launch_fill_periodic(m, state);
+    VH_HIP(hipGetLastError());
+  } else if (m->bc_kind == VH_MUSCL_BC_CALLBACK) {
+    const int rc = m->bc_fn(m->bc_user, state, (void*)m->stream, stage);
+    if (rc) return fail(-3, "the bc! callback returned %s%lld", "", (long long)rc);
+  }
+  return 0;
+}
+
 // One step: U1 = U + dU(U), then U = ½((U + U1) + dU(U1)) written over U.
-void enqueue_step(vh_muscl* m, double dt, bool rec) {
+// With bc!, the input is copied to Ub and refilled there (U keeps the unfilled
+// input, the reference's D0), stage 1 writes U1 = U + dU(Ub), and U1 is
+// refilled in place before stage 2. A failing callback leaves U as it was.
+int enqueue_step(vh_muscl* m, double dt, bool rec) {
   const double dtdx = m->dtype == VH_F64 ? dt / m->dx : (double)((float)dt / (float)m->dx);
   m->G.dtdx = dtdx;
-  launch_stage(m, false, rec, m->U, nullptr, m->U1);
+  if (m->bc_kind == VH_MUSCL_BC_NONE) {
+    launch_stage(m, false, rec, m->U, m->U, m->U1);
+    launch_stage(m, true, rec, m->U1, m->U, m->U);
+    return 0;
+  }
+  const size_t bytes = (size_t)m->G.nf * m->G.ncell * m->esize;
+  if (m->ub_bytes < bytes) {
+    VH_HIP(hipStreamSynchronize(m->stream));
+    if (m->Ub) VH_HIP(hipFree(m->Ub));
+    m->Ub = nullptr;
+    m->ub_bytes = 0;
+    VH_HIP(hipMalloc(&m->Ub, bytes));
+    m->ub_bytes = bytes;
+  }
+  VH_HIP(hipMemcpyAsync(m->Ub, m->U, bytes, hipMemcpyDeviceToDevice, m->stream));
+  if (int rc = apply_bc(m, m->Ub, 0)) return rc;
+  launch_stage(m, false, rec, m->Ub, m->U, m->U1);
+  if (int rc = apply_bc(m, m->U1, 1)) return rc;
   launch_stage(m, true, rec, m->U1, m->U, m->U);
+  return 0;
 }
 
 }  // namespace
@@ -219,6 +266,7 @@ int vh_muscl_destroy(vh_muscl* m) {
   (void)hipFree(m->U1);
   (void)hipFree(m->frec);
   (void)hipFree(m->partials);
+  (void)hipFree(m->Ub);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return 0;
@@ -305,8 +353,21 @@ int vh_muscl_step(vh_muscl* m, double dt, int32_t record) {
   if (!m) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
   if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)3 * m->G.nf * m->G.ncell * m->esize));
-  enqueue_step(m, dt, record != 0);
+  if (int rc = enqueue_step(m, dt, record != 0)) return rc;
   VH_HIP(hipGetLastError());
+  return 0;
+}
+
+int vh_muscl_set_bc(vh_muscl* m, int32_t kind, vh_muscl_bc_fn fn, void* user) {
+  if (!m) return fail(-1, "null argument%s", "");
+  if (kind != VH_MUSCL_BC_NONE && kind != VH_MUSCL_BC_PERIODIC && kind != VH_MUSCL_BC_CALLBACK)
+    return fail(-1, "unknown bc kind%s %lld", "", kind);
+  if (kind == VH_MUSCL_BC_CALLBACK && !fn) return fail(-1, "VH_MUSCL_BC_CALLBACK needs a function%s", "");
+  VH_HIP(hipSetDevice(m->device));
+  VH_HIP(hipStreamSynchronize(m->stream));
+  m->bc_kind = kind;
+  m->bc_fn = kind == VH_MUSCL_BC_CALLBACK ? fn : nullptr;
+  m->bc_user = kind == VH_MUSCL_BC_CALLBACK ? user : nullptr;
   return 0;
 }
 
@@ -322,9 +383,7 @@ int vh_muscl_fluxrec_download(vh_muscl* m, void* frec) {
 int vh_muscl_fill_periodic(vh_muscl* m) {
   if (!m) return fail(-1, "null argument%s", "");
   VH_HIP(hipSetDevice(m->device));
-  const unsigned nb = (unsigned)std::min<long long>(4096, (m->G.ncell + 255) / 256);
-  if (m->dtype == VH_F64) fill_periodic_kernel<double><<<nb, 256, 0, m->stream>>>((double*)m->U, m->G);
-  else fill_periodic_kernel<float><<<nb, 256, 0, m->stream>>>((float*)m->U, m->G);
+  launch_fill_periodic(m, m->U);
   VH_HIP(hipGetLastError());
   return 0;
 }
@@ -395,15 +454,24 @@ int vh_muscl_time_steps(vh_muscl* m, double dt, int64_t nsteps, int32_t record, 
   if (!m || nsteps <= 0 || nsteps > 100000) return fail(-1, "bad argument%s (nsteps %lld)", "", nsteps);
   VH_HIP(hipSetDevice(m->device));
   if (record && !m->frec) VH_HIP(hipMalloc(&m->frec, (size_t)3 * m->G.nf * m->G.ncell * m->esize));
-  enqueue_step(m, dt, record != 0);   // untimed warm-up
+  if (int rc = enqueue_step(m, dt, record != 0)) return rc;   // untimed warm-up
   hipEvent_t e0, e1, s0, s1;
   VH_HIP(hipEventCreate(&e0));
   VH_HIP(hipEventCreate(&e1));
   VH_HIP(hipEventCreate(&s0));
   VH_HIP(hipEventCreate(&s1));
   VH_HIP(hipEventRecord(e0, m->stream));
-  for (int64_t k = 0; k < nsteps; ++k) enqueue_step(m, dt, record != 0);
+  int rc = 0;
+  for (int64_t k = 0; k < nsteps && !rc; ++k) rc = enqueue_step(m, dt, record != 0);
   VH_HIP(hipEventRecord(e1, m->stream));
+  if (rc) {
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
+    return rc;
+  }
   VH_HIP(hipStreamSynchronize(m->stream));
   float tot = 0;
   VH_HIP(hipEventElapsedTime(&tot, e0, e1));
@@ -411,7 +479,7 @@ int vh_muscl_time_steps(vh_muscl* m, double dt, int64_t nsteps, int32_t record, 
   double st2 = 0;
   const int k2 = (int)std::min<int64_t>(nsteps, 20);
   for (int k = 0; k < k2; ++k) {
-    launch_stage(m, false, record != 0, m->U, nullptr, m->U1);
+    launch_stage(m, false, record != 0, m->U, m->U, m->U1);
     VH_HIP(hipEventRecord(s0, m->stream));
     launch_stage(m, true, record != 0, m->U1, m->U, m->U);
     VH_HIP(hipEventRecord(s1, m->stream));

@@ -9,7 +9,8 @@
 //   remap_kernel      the conservative transfer of the state onto a new block
 //                     mesh (prolong! by injection, lib/RefMesh/src/RefMesh.jl:284-302):
 //                     every cell of a new block takes the conserved values of the
-//                     old cell that contains it, so Σ U·V is unchanged.
+//                     old cell that contains it, so Σ U·V is unchanged (nf
+//                     fields: the five conserved ones, or φ alone).
 //
 // One wavefront per block; each lane walks the block's cells with a stride of 64.
 // Both kernels read the state once and write their output once: HBM-bound
@@ -96,7 +97,7 @@ template <typename T>
 __global__ __launch_bounds__(kRegridLanes) void remap_kernel(const BlockDev* __restrict__ dst_blocks,
                                                              const BlockDev* __restrict__ src_blocks,
                                                              const RemapDev* __restrict__ map, const T* __restrict__ Us,
-                                                             long long ns, T* __restrict__ Ud, long long nd) {
+                                                             long long ns, T* __restrict__ Ud, long long nd, int nf) {
   const BlockDev& db = dst_blocks[blockIdx.x];
   const RemapDev r = map[blockIdx.x];
   const BlockDev& sb = src_blocks[r.src];
@@ -107,8 +108,7 @@ __global__ __launch_bounds__(kRegridLanes) void remap_kernel(const BlockDev* __r
     const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long long)nx * ny));
     const long long s = sb.off + ((long long)(r.o[2] + (k >> r.shift)) * sny + (r.o[1] + (j >> r.shift))) * snx +
                         (r.o[0] + (i >> r.shift));
-#pragma unroll
-    for (int f = 0; f < 5; ++f) Ud[f * nd + db.off + c] = Us[f * ns + s];
+    for (int f = 0; f < nf; ++f) Ud[f * nd + db.off + c] = Us[f * ns + s];
   }
 }
 
@@ -126,14 +126,14 @@ void regrid_indicator(hipStream_t s, int dtype, int nblocks, const BlockDev* blo
 }
 
 void regrid_remap(hipStream_t s, int dtype, int ndst, const BlockDev* dst_blocks, const BlockDev* src_blocks,
-                  const RemapDev* map, const void* Us, long long ns, void* Ud, long long nd) {
+                  const RemapDev* map, const void* Us, long long ns, void* Ud, long long nd, int nf) {
   if (ndst <= 0) return;
   if (dtype == VH_F64)
     remap_kernel<double><<<ndst, kRegridLanes, 0, s>>>(dst_blocks, src_blocks, map, (const double*)Us, ns,
-                                                       (double*)Ud, nd);
+                                                       (double*)Ud, nd, nf);
   else
     remap_kernel<float><<<ndst, kRegridLanes, 0, s>>>(dst_blocks, src_blocks, map, (const float*)Us, ns, (float*)Ud,
-                                                      nd);
+                                                      nd, nf);
 }
 
 }  // namespace vh

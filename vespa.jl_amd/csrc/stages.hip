@@ -27,26 +27,35 @@ __global__ void __launch_bounds__(256) face_flux_kernel(const T* __restrict__ WL
   out[0] = o.d; out[1] = o.mx; out[2] = o.my; out[3] = o.mz; out[4] = o.e;
 }
 
-template <typename T, int RANK, int TX, int TY, bool FLOOR64, bool PER, bool GRAV>
+template <typename T, int RANK, int TX, int TY, bool FLOOR64, bool PER, bool GRAV, bool PG = false>
 static void stage_tp(bool stage2, int blocks, hipStream_t s, const void* Us, const void* U0, void* Uout,
                      const double* dtp, double* partials, const GridDev& G) {
   if (stage2)
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64, PER, GRAV>), dim3(blocks), dim3(TX * TY), 0,
-                       s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, true, FLOOR64, PER, GRAV, PG>), dim3(blocks), dim3(TX * TY),
+                       0, s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
   else
-    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64, PER, GRAV>), dim3(blocks), dim3(TX * TY), 0,
-                       s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
+    hipLaunchKernelGGL((rk_stage_kernel<T, RANK, TX, TY, false, FLOOR64, PER, GRAV, PG>), dim3(blocks), dim3(TX * TY),
+                       0, s, (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, G);
 }
 
 // Grids periodic on every axis run the kernel with the boundary logic compiled
-// out; with self-gravity the gravity form runs (boundary logic kept).
+// out; with self-gravity the gravity form runs (boundary logic kept); a grid
+// with a ParentGhost side runs the form that reads their states (with or
+// without gravity); the rest the general form.
 template <typename T, int RANK, int TX, int TY, bool FLOOR64>
 static void stage_t(bool stage2, int blocks, hipStream_t s, const void* Us, const void* U0, void* Uout,
                     const double* dtp, double* partials, const GridDev& G) {
-  bool per = true;
-  for (int d = 0; d < RANK; ++d) per = per && G.bc[2 * d] == BC_PERIODIC && G.bc[2 * d + 1] == BC_PERIODIC;
-  if (G.phi) stage_tp<T, RANK, TX, TY, FLOOR64, false, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  bool per = true, parent = false;
+  for (int d = 0; d < RANK; ++d) {
+    per = per && G.bc[2 * d] == BC_PERIODIC && G.bc[2 * d + 1] == BC_PERIODIC;
+    parent = parent || G.bc[2 * d] == BC_PARENT || G.bc[2 * d + 1] == BC_PARENT;
+  }
+  if (G.phi && parent)
+    stage_tp<T, RANK, TX, TY, FLOOR64, false, true, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  else if (G.phi) stage_tp<T, RANK, TX, TY, FLOOR64, false, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
   else if (per) stage_tp<T, RANK, TX, TY, FLOOR64, true, false>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
+  else if (parent)
+    stage_tp<T, RANK, TX, TY, FLOOR64, false, false, true>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
   else stage_tp<T, RANK, TX, TY, FLOOR64, false, false>(stage2, blocks, s, Us, U0, Uout, dtp, partials, G);
 }
 
@@ -120,15 +129,24 @@ void launch_link_flux(int dtype, int nfaces, hipStream_t s, const void* U, const
                        (float*)F);
 }
 
+template <typename T, int RANK, bool FLOOR64, bool GRAV>
+static void block_stage_tg(bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
+                           const double* dtp, double* partials, const BlocksDev& B) {
+  if (stage2)
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, true, FLOOR64, GRAV>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
+                       (const T*)U0, (T*)Uout, dtp, partials, B);
+  else
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, false, FLOOR64, GRAV>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
+                       (const T*)U0, (T*)Uout, dtp, partials, B);
+}
+
+// With self-gravity (B.gravg set) the stage adds the gravity source of the
+// step's g field to every cell's flux sum.
 template <typename T, int RANK, bool FLOOR64>
 static void block_stage_t(bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
                           const double* dtp, double* partials, const BlocksDev& B) {
-  if (stage2)
-    hipLaunchKernelGGL((block_stage_kernel<T, RANK, true, FLOOR64>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
-                       (const T*)U0, (T*)Uout, dtp, partials, B);
-  else
-    hipLaunchKernelGGL((block_stage_kernel<T, RANK, false, FLOOR64>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
-                       (const T*)U0, (T*)Uout, dtp, partials, B);
+  if (B.gravg) block_stage_tg<T, RANK, FLOOR64, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+  else block_stage_tg<T, RANK, FLOOR64, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
 }
 
 void launch_block_stage(int dtype, int rank, bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0,

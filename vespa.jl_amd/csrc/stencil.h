@@ -183,15 +183,43 @@ __device__ __forceinline__ Cons<T> swap_xy(const Cons<T>& f, bool on) {
     arr[4] idx = (v).p;      \
   } while (0)
 
+// ParentGhost state (src/driver.jl:127-134) of the face on `side` of `axis`
+// (0-based) of the boundary cell with face-plane index idx: 5 primitives per
+// boundary-face slot in the boundary-flux register's slot order.
+template <typename T>
+__device__ __forceinline__ Prim<T> parent_ghost(const GridDev& G, int axis, int side, long long idx) {
+  const T* q = (const T*)G.pg + 5 * (G.bbase[2 * axis + side] + idx);
+  Prim<T> o;
+  o.r = q[0];
+  o.vx = q[1];
+  o.vy = q[2];
+  o.vz = q[3];
+  o.p = q[4];
+  return o;
+}
+
+// _boundary_ghost (src/driver.jl:127-134) at a domain-boundary side: the
+// ParentGhost state (PG: the kernel form that reads them), or ghost_state of w
+// (the cell's primitive for a slope, its face value for a face).
+template <bool PG, typename T>
+__device__ __forceinline__ Prim<T> bghost(const Prim<T>& w, const GridDev& G, int axis, int side, long long idx) {
+  const int bc = G.bc[2 * axis + side];
+  if constexpr (PG) {
+    if (bc == BC_PARENT) return parent_ghost<T>(G, axis, side, idx);
+  }
+  return ghost(w, bc, axis + 1);
+}
+
 // State of a column cell at plane index kk (any integer) with the z boundary
-// rule applied: periodic wrap, or the ghost of the nearest boundary cell.
-template <typename T, bool FLOOR64, bool PER>
-__device__ __forceinline__ Prim<T> column_w(const Cons<T>& u, const GridDev& G, int kk, T g) {
+// rule applied: periodic wrap, or the ghost of the nearest boundary cell (zidx:
+// the column's index in the z boundary planes).
+template <typename T, bool FLOOR64, bool PER, bool PG>
+__device__ __forceinline__ Prim<T> column_w(const Cons<T>& u, const GridDev& G, int kk, T g, long long zidx) {
   Prim<T> w = cons2prim<T, FLOOR64>(u, g, G.gamma);
   if constexpr (!PER) {
     int gh;
     (void)resolve(kk, G.n[2], G.bc[4], gh);
-    if (gh) w = ghost(w, G.bc[4 + gh - 1], 3);
+    if (gh) w = bghost<PG>(w, G, 2, gh - 1, zidx);
   }
   return w;
 }
@@ -211,6 +239,21 @@ __device__ __forceinline__ Cons<T> axis_face(int gpos, int n, int bclo, int bchi
   if (!PER && bclo != BC_PERIODIC) {
     if (gpos == 0) L = ghost(R, bclo, dim);
     else if (gpos == n) R = ghost(L, bchi, dim);
+  }
+  return hllc(L, R, g, igm1, dim);
+}
+
+// The same on the uniform grid, where a side may be ParentGhost: the outer
+// state is then the caller's state of the boundary cell (plane index idx along
+// grid axis `axis`), in the frame of the face (x and y exchanged with swap).
+template <bool PER, bool PG, typename T>
+__device__ __forceinline__ Cons<T> axis_face_g(int gpos, int n, int bclo, int bchi, Prim<T> L, Prim<T> R, T g,
+                                               T igm1, int dim, const GridDev& G, int axis, long long idx,
+                                               bool swap) {
+  if constexpr (!PG) return axis_face<PER>(gpos, n, bclo, bchi, L, R, g, igm1, dim);
+  if (!PER && bclo != BC_PERIODIC) {
+    if (gpos == 0) L = bclo == BC_PARENT ? swap_xy(parent_ghost<T>(G, axis, 0, idx), swap) : ghost(R, bclo, dim);
+    else if (gpos == n) R = bchi == BC_PARENT ? swap_xy(parent_ghost<T>(G, axis, 1, idx), swap) : ghost(L, bchi, dim);
   }
   return hllc(L, R, g, igm1, dim);
 }
@@ -239,7 +282,7 @@ __device__ __forceinline__ void bflux_add(const GridDev& G, int axis, int side, 
 // boundary-flux register), resolved at compile time. GRAV: self-gravity on
 // (G.phi set): the gravity source joins the flux sum and the free-fall limiter
 // the CFL maximum.
-template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64, bool PER, bool GRAV>
+template <typename T, int RANK, int TX, int TY, bool STAGE2, bool FLOOR64, bool PER, bool GRAV, bool PG = false>
 __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH_MIN_WAVES) rk_stage_kernel(const T* __restrict__ Us, const T* U0,
                                                           T* Uout, const double* __restrict__ dtp,
                                                           double* __restrict__ partials, GridDev G) {
@@ -287,11 +330,11 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     const Cons<T> u0 = load_cons(Us, n, column_src<PER>(G, col, kz0));
     const Cons<T> u1 = load_cons(Us, n, column_src<PER>(G, col, kz0 + 1));
     const Cons<T> u2 = load_cons(Us, n, column_src<PER>(G, col, kz0 + 2));
-    const Prim<T> wm2 = column_w<T, FLOOR64, PER>(um2, G, kz0 - 2, g);
-    const Prim<T> wm1 = column_w<T, FLOOR64, PER>(um1, G, kz0 - 1, g);
-    wz0 = column_w<T, FLOOR64, PER>(u0, G, kz0, g);
-    wz1 = column_w<T, FLOOR64, PER>(u1, G, kz0 + 1, g);
-    wz2 = column_w<T, FLOOR64, PER>(u2, G, kz0 + 2, g);
+    const Prim<T> wm2 = column_w<T, FLOOR64, PER, PG>(um2, G, kz0 - 2, g, col);
+    const Prim<T> wm1 = column_w<T, FLOOR64, PER, PG>(um1, G, kz0 - 1, g, col);
+    wz0 = column_w<T, FLOOR64, PER, PG>(u0, G, kz0, g, col);
+    wz1 = column_w<T, FLOOR64, PER, PG>(u1, G, kz0 + 1, g, col);
+    wz2 = column_w<T, FLOOR64, PER, PG>(u2, G, kz0 + 2, g, col);
     // The slope of the ghost plane kz0-1 is never used when kz0-1 is outside a
     // non-periodic domain: that face is a boundary face (lo state = ghost).
     const Prim<T> sm1 = slope(wm2, wm1, wz0);
@@ -299,7 +342,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     const int bz = G.bc[4];
     if (!PER && bz != BC_PERIODIC && kz0 == 0) {
       const Prim<T> R = face_value(wz0, sz0, T(-0.5));
-      fzlo = hllc(ghost(R, bz, 3), R, g, igm1, 3);
+      fzlo = hllc(bghost<PG>(R, G, 2, 0, col), R, g, igm1, 3);
       if (G.bflux && active) bflux_add(G, 2, 0, (long long)gy * nx + gx, fzlo, T(0.5 * dt), a2);
     } else {
       fzlo = hllc(face_value(wm1, sm1, T(0.5)), face_value(wz0, sz0, T(-0.5)), g, igm1, 3);
@@ -326,14 +369,14 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       ly = r < 2 ? r - 2 : TY + (r - 2);
     }
   };
-  // Stage a halo cell given its conserved state (ghost rule applied here).
-  auto stage_halo = [&](int lx, int ly, const Cons<T>& u) {
+  // Stage a halo cell of plane kk given its conserved state (ghost rule applied here).
+  auto stage_halo = [&](int lx, int ly, int kk, const Cons<T>& u) {
     int hgx, hgy;
-    (void)resolve_axis<PER>(x0 + lx, nx, G.bc[0], hgx);
-    (void)resolve_axis<PER>(y0 + ly, ny, G.bc[2], hgy);
+    const int sx = resolve_axis<PER>(x0 + lx, nx, G.bc[0], hgx);
+    const int sy = resolve_axis<PER>(y0 + ly, ny, G.bc[2], hgy);
     Prim<T> w = cons2prim<T, FLOOR64>(u, g, G.gamma);
-    if (hgx) w = ghost(w, G.bc[hgx - 1], 1);
-    if (hgy) w = ghost(w, G.bc[2 + hgy - 1], 2);
+    if (hgx) w = bghost<PG>(w, G, 0, hgx - 1, (long long)kk * ny + sy);
+    if (hgy) w = bghost<PG>(w, G, 1, hgy - 1, (long long)kk * nx + sx);
     VH_PUT5(S.w, [ly + L_::HY][lx + 2], w);
   };
   auto halo_src = [&](int lx, int ly, int kk) -> long long {
@@ -375,8 +418,8 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     }
     {
       Prim<T> w = wc;
-      if (ghx) w = ghost(w, G.bc[ghx - 1], 1);
-      if (ghy) w = ghost(w, G.bc[2 + ghy - 1], 2);
+      if (ghx) w = bghost<PG>(w, G, 0, ghx - 1, (long long)k * ny + syc);
+      if (ghy) w = bghost<PG>(w, G, 1, ghy - 1, (long long)k * nx + sxc);
       VH_PUT5(S.w, [ty + L_::HY][tx + 2], w);
       wc = w;
     }
@@ -387,7 +430,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       if (lt < NHX + NHY) {
         int lx, ly;
         halo_pos(lt, lx, ly);
-        stage_halo(lx, ly, uhalo);
+        stage_halo(lx, ly, k, uhalo);
       }
       // the next plane's halo cell of this lane, loaded while this plane runs
       const int hn = lane_item(k + 1);
@@ -400,7 +443,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       for (int h = lt; h < NHX + NHY; h += NT) {
         int lx, ly;
         halo_pos(h, lx, ly);
-        stage_halo(lx, ly, load_cons(Us, n, halo_src(lx, ly, k)));
+        stage_halo(lx, ly, k, load_cons(Us, n, halo_src(lx, ly, k)));
       }
     }
     __syncthreads();
@@ -449,7 +492,8 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     const int bx = G.bc[0], by = G.bc[2];
     if (gy < ny && gx <= nx) {
       const int sl = L_::xs(ty, tx);
-      const Cons<T> F = axis_face<PER>(gx, nx, bx, G.bc[1], lds_get5(ff, L_::FS, sl), xlo, g, igm1, 1);
+      const Cons<T> F = axis_face_g<PER, PG>(gx, nx, bx, G.bc[1], lds_get5(ff, L_::FS, sl), xlo, g, igm1, 1, G, 0,
+                                         (long long)k * ny + gy, false);
       lds_putf(ff, L_::FS, sl, F);
       if (!PER && G.bflux && bx != BC_PERIODIC && (gx == 0 || gx == nx))
         bflux_add(G, 0, gx == nx, (long long)k * ny + gy, F, T(0.5 * dt), a0);
@@ -457,7 +501,8 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     if constexpr (RANK > 1) {
       if (gx < nx && gy <= ny) {
         const int sl = L_::ys(ty, tx);
-        const Cons<T> F = axis_face<PER>(gy, ny, by, G.bc[3], lds_get5(ff, L_::FS, sl), ylo, g, igm1, 2);
+        const Cons<T> F = axis_face_g<PER, PG>(gy, ny, by, G.bc[3], lds_get5(ff, L_::FS, sl), ylo, g, igm1, 2, G, 1,
+                                           (long long)k * nx + gx, false);
         lds_putf(ff, L_::FS, sl, F);
         if (!PER && G.bflux && by != BC_PERIODIC && (gy == 0 || gy == ny))
           bflux_add(G, 1, gy == ny, (long long)k * nx + gx, F, T(0.5 * dt), a1);
@@ -479,7 +524,10 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
         const int sr = isy ? L_::ys(TY + 1, line) : L_::xs(line, TX + 1);
         const Prim<T> L = swap_xy(lds_get5(ff, L_::FS, sl), isy);
         const Prim<T> R = swap_xy(lds_get5(ff, L_::FS, sr), isy);
-        const Cons<T> F = swap_xy(axis_face<PER>(gp, na, isy ? by : bx, isy ? G.bc[3] : G.bc[1], L, R, g, igm1, 1),
+        const Cons<T> F = swap_xy(axis_face_g<PER, PG>(gp, na, isy ? by : bx, isy ? G.bc[3] : G.bc[1], L, R, g, igm1, 1,
+                                                   G, isy ? 1 : 0,
+                                                   isy ? (long long)k * nx + x0 + line : (long long)k * ny + y0 + line,
+                                                   isy),
                                   isy);
         lds_putf(ff, L_::FS, sl, F);
         if (!PER && G.bflux && (isy ? by : bx) != BC_PERIODIC && gp == na)
@@ -496,7 +544,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       const int bz = G.bc[4];
       const Prim<T> L = face_value(wz0, sz0, T(0.5));
       if (!PER && bz != BC_PERIODIC && k == nz - 1) {
-        fzhi = hllc(L, ghost(L, G.bc[5], 3), g, igm1, 3);
+        fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
         if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
       } else
         fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
@@ -586,7 +634,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       sz0 = sz1;
       wz0 = wz1;
       wz1 = wz2;
-      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64, PER>(unext, G, k + 3, g);
+      if (k + 1 < kz1) wz2 = column_w<T, FLOOR64, PER, PG>(unext, G, k + 3, g, col);
     }
   }
 

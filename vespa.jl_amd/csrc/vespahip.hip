@@ -128,7 +128,21 @@ struct vh_grid {
   hipStream_t stream;
   struct AdjWork* adj;   // reverse-mode work buffers (allocated on first use)
   struct GravWork* grav; // self-gravity state (vh_grid_enable_gravity)
+  void* pg;              // ParentGhost states (vh_grid_set_parent_ghost), G.pg points here
 };
+
+// A step with a ParentGhost side needs its states first.
+static int need_parent_states(const vh_grid* g) {
+  for (int s = 0; s < 2 * g->G.rank; ++s)
+    if (g->G.bc[s] == BC_PARENT && !g->G.pg)
+      return fail(-1, "a ParentGhost side needs its states (vh_grid_set_parent_ghost)%s", "");
+  return 0;
+}
+static bool has_parent(const vh_grid* g) {
+  for (int s = 0; s < 2 * g->G.rank; ++s)
+    if (g->G.bc[s] == BC_PARENT) return true;
+  return false;
+}
 
 // Self-gravity of a uniform-grid context (the reference's GravityField,
 // src/gravity.jl:30-45): φ, CG work vectors and solver settings.
@@ -446,7 +460,7 @@ int vh_grid_create(vh_grid** out, int32_t rank, const int64_t* dims, const doubl
       G.w[d] = (hi[d] - lo[d]) / (double)dims[d];
       for (int s = 0; s < 2; ++s) {
         const int b = bc[2 * d + s];
-        if (b < 0 || b > 2) { delete g; return fail(-1, "unknown boundary code%s %lld", "", b); }
+        if (b < 0 || b > 3) { delete g; return fail(-1, "unknown boundary code%s %lld", "", b); }
         G.bc[2 * d + s] = b;
       }
       if ((G.bc[2 * d] == BC_PERIODIC) != (G.bc[2 * d + 1] == BC_PERIODIC)) {
@@ -523,8 +537,23 @@ int vh_grid_destroy(vh_grid* g) {
   if (g->partials) (void)hipFree(g->partials);
   if (g->clock) (void)hipFree(g->clock);
   if (g->totals_partials) (void)hipFree(g->totals_partials);
+  if (g->pg) (void)hipFree(g->pg);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
+  return 0;
+}
+
+int vh_grid_set_parent_ghost(vh_grid* g, const void* host) {
+  if (!g || !host) return fail(-1, "null argument%s", "");
+  if (!has_parent(g)) return fail(-1, "no side of this grid is ParentGhost%s", "");
+  VH_HIP(hipSetDevice(g->device));
+  long long nb = 0;
+  for (int d = 0; d < g->G.rank; ++d) nb += 2 * (g->G.ncell / g->G.n[d]);
+  const size_t bytes = (size_t)5 * nb * g->esize;
+  VH_HIP(hipStreamSynchronize(g->stream));   // a running step may read the previous states
+  if (!g->pg) VH_HIP(hipMalloc(&g->pg, bytes));
+  VH_HIP(hipMemcpy(g->pg, host, bytes, hipMemcpyHostToDevice));
+  g->G.pg = g->pg;
   return 0;
 }
 
@@ -578,6 +607,7 @@ int vh_grid_compute_dt(vh_grid* g, double cfl, double* dt) {
 
 int vh_grid_step(vh_grid* g, double dt) {
   if (!g) return fail(-1, "null argument%s", "");
+  if (int rc = need_parent_states(g)) return rc;
   VH_HIP(hipSetDevice(g->device));
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, dt);
   enqueue_step(g);
@@ -594,6 +624,7 @@ int64_t vh_grid_bflux_count(const vh_grid* g) {
 
 int vh_grid_step_bflux(vh_grid* g, double dt, void* dbflux) {
   if (!g || !dbflux) return fail(-1, "null argument%s", "");
+  if (int rc = need_parent_states(g)) return rc;
   VH_HIP(hipSetDevice(g->device));
   hipLaunchKernelGGL(set_dt_kernel, dim3(1), dim3(1), 0, g->stream, g->clock, dt);
   g->G.bflux = dbflux;
@@ -609,6 +640,7 @@ int vh_grid_run(vh_grid* g, double cfl, double tfinal, int64_t max_steps, double
   if (!g || !t_io) return fail(-1, "null argument%s", "");
   const bool clip = tfinal > 0.0;
   if (!clip && max_steps <= 0) return fail(-1, "need tfinal > 0 or max_steps > 0%s", "");
+  if (int rc = need_parent_states(g)) return rc;
   VH_HIP(hipSetDevice(g->device));
   double clk[4] = {*t_io, 0.0, 0.0, 0.0};
   VH_HIP(hipMemcpyAsync(g->clock, clk, sizeof clk, hipMemcpyHostToDevice, g->stream));
@@ -769,6 +801,7 @@ int vh_grid_totals(vh_grid* g, double* out5) {
 
 int vh_grid_accumulate_flux(vh_grid* g, void* acc_host) {
   if (!g || !acc_host) return fail(-1, "null argument%s", "");
+  if (int rc = need_parent_states(g)) return rc;
   VH_HIP(hipSetDevice(g->device));
   // Stage 1 with dt = −V gives invV = Tf(−V)/(Tf(1)·Tf(V)) = −1 exactly, so
   // U1 = U + acc; acc is recovered as U1 − U (rounded; inspection only).
@@ -815,6 +848,7 @@ int vh_grid_get_launch(const vh_grid* g, int32_t* tile_x, int32_t* tile_y, int32
 int vh_grid_time_steps(vh_grid* g, double cfl, int64_t nsteps, double* ms_stage1, double* ms_stage2,
                        double* ms_step) {
   if (!g || nsteps <= 0 || nsteps > 100000) return fail(-1, "bad argument%s (nsteps %lld)", "", nsteps);
+  if (int rc = need_parent_states(g)) return rc;
   VH_HIP(hipSetDevice(g->device));
   // Three events per step, recorded asynchronously; one synchronisation at the end.
   const int64_t ne = 3 * nsteps;
@@ -998,6 +1032,7 @@ extern "C" {
 int vh_grid_step_vjp(vh_grid* g, const void* dU, double dt, const void* dWbar, void* dUbar) {
   if (!g || !dU || !dWbar || !dUbar) return fail(-1, "null argument%s", "");
   if (g->grav) return fail(-1, "the reverse mode does not cover self-gravity%s", "");
+  if (has_parent(g)) return fail(-1, "the reverse mode does not cover ParentGhost boundaries%s", "");
   VH_HIP(hipSetDevice(g->device));
   int rc = ensure_adj_work(g);
   if (rc) return rc;
@@ -1039,6 +1074,7 @@ int vh_adjoint_create(vh_adjoint** out, vh_grid* g, int64_t nsteps, int32_t ever
   if (nsteps < 1 || every < 1) return fail(-1, "need nsteps >= 1 and every >= 1%s", "");
   if (g->dtype != VH_F64) return fail(-1, "the reverse mode is Float64 only%s", "");
   if (g->grav) return fail(-1, "the reverse mode does not cover self-gravity%s", "");
+  if (has_parent(g)) return fail(-1, "the reverse mode does not cover ParentGhost boundaries%s", "");
   VH_HIP(hipSetDevice(g->device));
   int rc = ensure_adj_work(g);
   if (rc) return rc;

@@ -58,6 +58,20 @@ class Reflecting:
     code = _lib.BC_REFLECTING
 
 
+class ParentGhost:
+    """ParentGhost(ghost) (lib/MeshInterface/src/MeshInterface.jl:86-99): the
+    boundary ghost of a face on this side is ghost(axis, side, cell), a
+    primitive state (ρ, vx, vy, vz, p) — axis 1-based, side "lo" or "hi", cell
+    the 1-based index tuple of the boundary cell — used as the outer neighbour
+    of the cell's slope and as the outer state of its boundary face
+    (src/driver.jl:127-134). The Simulation evaluates it for every face of the
+    side before each step and hands the states to the device."""
+    code = _lib.BC_PARENT
+
+    def __init__(self, ghost):
+        self.ghost = ghost
+
+
 def _bc_code(bc) -> int:
     if isinstance(bc, str):
         return {"periodic": _lib.BC_PERIODIC, "outflow": _lib.BC_OUTFLOW,
@@ -67,6 +81,16 @@ def _bc_code(bc) -> int:
     if hasattr(bc, "code"):
         return bc.code
     raise ValueError(f"unknown boundary condition {bc!r}")
+
+
+def _parent_sides(bcs, rank: int) -> dict:
+    """{(axis 0-based, side 0/1): ParentGhost} of the sides whose BC is one."""
+    if isinstance(bcs, (list, tuple)) and len(bcs) == rank and all(
+            isinstance(p, (list, tuple)) and len(p) == 2 for p in bcs):
+        pairs = list(bcs)
+    else:
+        pairs = [(bcs, bcs)] * rank
+    return {(d, s): pairs[d][s] for d in range(rank) for s in range(2) if isinstance(pairs[d][s], ParentGhost)}
 
 
 def _bcs_array(bcs, rank: int):
@@ -260,6 +284,7 @@ class Simulation:
         hi = (C.c_double * 3)(*[mesh.domain[d][1] if d < rank else 1.0 for d in range(3)])
         bc = _bcs_array(prob.bcs, rank)
         self._bc_pairs = [(int(bc[2 * d]), int(bc[2 * d + 1])) for d in range(rank)]
+        self._parent = _parent_sides(prob.bcs, rank)   # {(axis, side): ParentGhost}
         h = C.c_void_p()
         check(self._lib.vh_grid_create(C.byref(h), rank, dims, lo, hi, bc, float(prob.gamma),
                                        _lib.VH_F64 if self.eltype == np.float64 else _lib.VH_F32))
@@ -353,10 +378,51 @@ def compute_dt(sim: Simulation) -> float:
     return dt.value
 
 
+def parent_ghost_states(sim: Simulation) -> np.ndarray:
+    """The ParentGhost closures of sim evaluated for every boundary face of their
+    sides: (bflux_count, 5) primitives at the state's precision in the
+    boundary-face slot order (other planes zero)."""
+    dims = tuple(sim.mesh.dims)
+    rank = len(dims)
+    n = int(np.prod(dims))
+    out = np.zeros((bflux_count(sim), 5), dtype=sim.eltype)
+    for (d, side), pg in sorted(sim._parent.items()):
+        base = sum(2 * (n // dims[e]) for e in range(d)) + (n // dims[d] if side else 0)
+        other = [e for e in range(rank) if e != d]
+        shape = [dims[e] for e in other]
+        for q in range(n // dims[d]):
+            cell = [0] * rank
+            cell[d] = 1 if side == 0 else dims[d]
+            r = q
+            for e, ne in zip(other, shape):
+                cell[e] = r % ne + 1
+                r //= ne
+            out[base + q] = np.asarray(pg.ghost(d + 1, "hi" if side else "lo", tuple(cell)), dtype=np.float64)
+    return out
+
+
+def set_parent_ghost(sim: Simulation, states: np.ndarray) -> None:
+    """Hand ParentGhost states (bflux_count × 5 primitives, boundary-face slot
+    order) to the device (vh_grid_set_parent_ghost)."""
+    a = np.ascontiguousarray(states, dtype=sim.eltype)
+    if a.shape != (bflux_count(sim), 5):
+        raise ValueError(f"states must be ({bflux_count(sim)}, 5)")
+    check(sim._lib.vh_grid_set_parent_ghost(sim._h, a.ctypes.data_as(C.c_void_p)))
+
+
+def _refresh_parent(sim: Simulation) -> None:
+    if sim._parent:
+        set_parent_ghost(sim, parent_ghost_states(sim))
+
+
 def step(sim: Simulation, dt: float, bflux: "BoundaryFluxRegister | None" = None) -> Simulation:
     """One SSP-RK2 step (src/driver.jl:333-359); advances sim.t and the step
     count. With a BoundaryFluxRegister, ½dt·F·area of every domain-boundary face
-    is added to it per RK stage (src/reflux.jl:105-132)."""
+    is added to it per RK stage (src/reflux.jl:105-132). ParentGhost sides take
+    their closures' states, evaluated before the step."""
+    if sim._parent and sim.flux is not None:
+        raise ValueError("ParentGhost boundaries run in the fused device path (sim.flux = None)")
+    _refresh_parent(sim)
     if sim.flux is not None:
         if bflux is not None:
             raise ValueError("the boundary-flux register is recorded by the fused device path (sim.flux = None)")
@@ -436,8 +502,9 @@ def bflux_plane(rank: int, dim: int, start, stop, g0, flux_off: int, comp: int, 
 
 def run_steps(sim: Simulation, nsteps: int) -> Simulation:
     """nsteps steps with the CFL dt reduced on the device before each one (no
-    clip, no host synchronisation inside the loop)."""
-    if sim.flux is not None:
+    clip, no host synchronisation inside the loop; with ParentGhost sides the
+    closures are evaluated before each step)."""
+    if sim.flux is not None or sim._parent:
         for _ in range(int(nsteps)):
             step(sim, compute_dt(sim))
         return sim
@@ -454,7 +521,7 @@ def evolve(sim: Simulation, max_steps: int = 0, callback=None, callback_every: i
     tfinal − t), final step clipped to land on tfinal."""
     if callback is not None:
         callback(sim, "init")
-    if callback is None and sim.flux is None:
+    if callback is None and sim.flux is None and not sim._parent:
         t = C.c_double(sim.t)
         k = C.c_int64()
         check(sim._lib.vh_grid_run(sim._h, float(sim.problem.cfl), float(sim.problem.tfinal),

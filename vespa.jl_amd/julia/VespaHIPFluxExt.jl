@@ -29,7 +29,7 @@ module VespaHIPFluxExt
 
 using Vespa
 using Vespa: AbstractFluxBackend, adiabatic_index
-using MeshInterface: Outflow, Periodic, Reflecting, bc_on
+using MeshInterface: Outflow, ParentGhost, Periodic, Reflecting, bc_on
 
 const libvespahip = get(ENV, "VESPAHIP_LIB", joinpath(@__DIR__, "..", "libvespahip.so"))
 
@@ -71,6 +71,7 @@ end
 _bc_code(::Periodic) = Int32(0)
 _bc_code(::Outflow) = Int32(1)
 _bc_code(::Reflecting) = Int32(2)
+_bc_code(::ParentGhost) = Int32(3)
 _bc_code(bc) = error("vespahip: boundary condition $(typeof(bc)) is not supported on the device path")
 
 "Device-resident state of a `Simulation` on a RefMesh `UniformMesh`."
@@ -140,7 +141,35 @@ step!(sim, dt) on the device (src/driver.jl:333-359). Advances `sim.t` and
 are stale until `hip_download!(sim, g)` (keeping the state resident is what
 removes the per-step PCIe round trip). `hip_evolve!` downloads at its end.
 """
+# The ParentGhost closures of sim (src/driver.jl:127-134) evaluated for every
+# boundary face of their sides, in the boundary-flux slot order, handed to the device.
+function _refresh_parent!(sim, g::HIPGrid)
+    N = g.rank
+    b = sim.backend
+    dims = ntuple(d -> b.dims[d], N)
+    any(bc_on(sim.bcs, d, s) isa ParentGhost for d in 1:N for s in (:lo, :hi)) || return g
+    states = zeros(g.eltype, 5 * hip_bflux_count(g))
+    slot = 0
+    for d in 1:N, s in (:lo, :hi)
+        bc = bc_on(sim.bcs, d, s)
+        other = [e for e in 1:N if e != d]
+        plane = CartesianIndices(ntuple(i -> 1:dims[other[i]], N - 1))
+        for J in plane
+            if bc isa ParentGhost
+                I = ntuple(e -> e == d ? (s === :lo ? 1 : dims[d]) : J[findfirst(==(e), other)], N)
+                W = bc.ghost(d, s, CartesianIndex(I))
+                for f in 1:5
+                    states[5 * slot + f] = W[f]
+                end
+            end
+            slot += 1
+        end
+    end
+    return hip_set_parent_ghost!(g, states)
+end
+
 function hip_step!(sim, g::HIPGrid, dt::Real)
+    _refresh_parent!(sim, g)
     _check(ccall((:vh_grid_step, libvespahip), Cint, (Ptr{Cvoid}, Float64), g.ptr, Float64(dt)))
     sim.t += dt
     sim.step += 1
@@ -253,6 +282,25 @@ function hip_remap!(dst::HIPBlocks, src::HIPBlocks, src_block::AbstractVector{<:
     return dst
 end
 
+"""
+    hip_regrid!(m::HIPBlocks, level, dims, origin, bc, src_block)
+
+regrid! in place (src/driver.jl:440-462): `m` moves onto the new block table
+(same rank, maxlevel, h0, γ), block b taking the cells of current block
+src_block[b] (1-based: the same block or its ancestor) by injection
+(lib/RefMesh/src/RefMesh.jl:284-302); the device buffers are reused.
+"""
+function hip_regrid!(m::HIPBlocks, level::AbstractVector, dims::AbstractMatrix, origin::AbstractMatrix,
+                     bc::AbstractMatrix, src_block::AbstractVector{<:Integer})
+    lv = Int32.(level); dm = Int32.(dims); og = Int64.(origin); bcs = Int32.(bc)
+    sb = Int32.(src_block) .- Int32(1)
+    _check(ccall((:vh_blocks_regrid, libvespahip), Cint,
+                 (Ptr{Cvoid}, Int32, Ptr{Int32}, Ptr{Int32}, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}),
+                 m.ptr, Int32(length(lv)), lv, dm, og, bcs, sb))
+    m.ncell = Int(ccall((:vh_blocks_ncell, libvespahip), Int64, (Ptr{Cvoid},), m.ptr))
+    return m
+end
+
 # ------------------------------------------------------------------ PPMKernels MUSCL
 
 """
@@ -306,10 +354,46 @@ hip_set_mode!(m::HIPMuscl, fast::Bool) =
 hip_fill_periodic!(m::HIPMuscl) = (_check(ccall((:vh_muscl_fill_periodic, libvespahip), Cint, (Ptr{Cvoid},),
                                                 m.ptr)); m)
 
+"""
+    hip_set_bc!(m::HIPMuscl, bc)
+
+bc! of muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:316-322, 339, 350), run
+before each RK stage: `nothing`, `:periodic` (fill_periodic! on the device), or a
+C function pointer `fn` (from `@cfunction(f, Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32))`)
+called as `f(user, state, stream, stage)` with the device state of the stage and the
+library's HIP stream; `user` is passed through (`hip_set_bc!(m, fn, user)`).
+"""
+function hip_set_bc!(m::HIPMuscl, bc, user::Ptr{Cvoid} = C_NULL)
+    kind, fn = bc === nothing ? (Int32(0), C_NULL) :
+               bc === :periodic ? (Int32(1), C_NULL) :
+               bc isa Ptr{Cvoid} ? (Int32(2), bc) : throw(ArgumentError("bc must be nothing, :periodic or a Ptr{Cvoid}"))
+    _check(ccall((:vh_muscl_set_bc, libvespahip), Cint, (Ptr{Cvoid}, Int32, Ptr{Cvoid}, Ptr{Cvoid}),
+                 m.ptr, kind, fn, kind == 2 ? user : C_NULL))
+    return m
+end
+
 function hip_max_wavespeed(m::HIPMuscl)
     s = Ref{Float64}(0.0)
     _check(ccall((:vh_muscl_max_wavespeed, libvespahip), Cint, (Ptr{Cvoid}, Ref{Float64}), m.ptr, s))
     return s[]
+end
+
+# ------------------------------------------------------------------ ParentGhost
+
+"""
+    hip_set_parent_ghost!(g::HIPGrid, states::Vector) -> g
+
+The states of the grid's ParentGhost sides (`VH_BC_PARENT`; MeshInterface's
+`ParentGhost(ghost)`, src/driver.jl:127-134): `ghost(axis, side, cell)` evaluated for
+every boundary face, 5 primitives (ρ, vx, vy, vz, p) of the state type per face in the
+slot order of the boundary-flux register (`hip_bflux_count(g)` faces). Held for every
+step until the next call; call it before each step when the parent data change.
+"""
+function hip_set_parent_ghost!(g::HIPGrid, states::Vector)
+    length(states) == 5 * hip_bflux_count(g) || error("states must hold 5 × hip_bflux_count(g) values")
+    GC.@preserve states _check(ccall((:vh_grid_set_parent_ghost, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}),
+                                     g.ptr, states))
+    return g
 end
 
 # ------------------------------------------------------------------ boundary-flux register
@@ -437,6 +521,35 @@ function hip_phi(g::HIPGrid)
 end
 
 """
+    hip_enable_gravity!(m::HIPBlocks; G=1.0, maxiter=500, tol, project_mean=true)
+
+enable_gravity! (src/gravity.jl:63-80) on a block mesh: the composite Poisson
+operator of the mesh (coarse↔fine rows, src/gravity.jl:455-600) built once, φ = 0.
+`hip_evolve!` (single rate) then solves before each step; every stage adds the
+gravity source and the device dt folds in the free-fall limiter.
+"""
+function hip_enable_gravity!(m::HIPBlocks; G::Real = 1.0, maxiter::Integer = 500,
+                             tol::Real = (m.eltype === Float32 ? 1.0e-5 : 1.0e-8), project_mean::Bool = true)
+    _check(ccall((:vh_blocks_enable_gravity, libvespahip), Cint, (Ptr{Cvoid}, Float64, Int64, Float64, Int32),
+                 m.ptr, Float64(G), Int64(maxiter), Float64(tol), Int32(project_mean)))
+    return m
+end
+
+function hip_solve_poisson!(m::HIPBlocks)
+    it = Ref{Int64}(0)
+    rr = Ref{Float64}(0.0)
+    _check(ccall((:vh_blocks_solve_poisson, libvespahip), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Float64}), m.ptr, it, rr))
+    return (it[], rr[])
+end
+
+"φ of the block mesh as a flat host vector in leaf order."
+function hip_phi(m::HIPBlocks)
+    out = Vector{m.eltype}(undef, m.ncell)
+    GC.@preserve out _check(ccall((:vh_blocks_phi_download, libvespahip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), m.ptr, out))
+    return out
+end
+
+"""
     HIPRollout(g::HIPGrid, nsteps; every = 20)
 
 Checkpointed rollout of `nsteps` steps with the device CFL dt: `hip_forward!` keeps the
@@ -528,8 +641,8 @@ end
 # with ChainRulesCore).
 
 export HIPFlux, HIPGrid, hip_step!, hip_evolve!, hip_set_mode!, hip_upload!, hip_download!, hip_conserved_totals,
-       HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
-       hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_max_wavespeed, hip_step_bflux!, hip_bflux_count,
+       HIPBlocks, hip_download, hip_indicator, hip_remap!, hip_regrid!, hip_step_vjp, hip_enable_gravity!, hip_solve_poisson!,
+       hip_phi, HIPMuscl, hip_muscl_step!, hip_fill_periodic!, hip_set_bc!, hip_max_wavespeed, hip_step_bflux!, hip_bflux_count, hip_set_parent_ghost!,
        hip_fluxrec, hip_flux_register, HIPRollout, hip_forward!, hip_backward, hip_step_state, hip_rollout_state
 
 end # module

@@ -20,6 +20,9 @@ from . import _lib
 from ._lib import check, load_library
 
 
+_BC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32)
+
+
 class MusclGrid:
     def __init__(self, dims, ng: int, gamma: float, theta: float = 1.5, dx: float = 1.0, small_rho: float = 1e-10,
                  eltype=np.float64, dual: bool = False, eta1: float = 1e-3):
@@ -93,6 +96,42 @@ class MusclGrid:
     def fill_periodic(self):
         """fill_periodic!(dims, ng, D, S1, S2, S3, Tau) (lib/PPMKernels/src/muscl_grid.jl:664-670)."""
         check(self._lib.vh_muscl_fill_periodic(self._h))
+
+    def set_bc(self, bc=None):
+        """bc! of muscl_step_3d! (lib/PPMKernels/src/muscl_grid.jl:316-322, 339, 350),
+        run before each RK stage: None (bc! = nothing), "periodic" (fill_periodic!
+        on the device), or a host function bc(V, stage) that refills the ghost
+        zones of V, the stage's input as an (nf, nz, ny, nx) array, in place (the
+        state crosses PCIe twice per stage; a device-side fill belongs in the C
+        ABI's callback, which receives the device pointer and the stream)."""
+        if bc is None:
+            check(self._lib.vh_muscl_set_bc(self._h, 0, None, None))
+            self._bc_keep = None
+        elif isinstance(bc, str):
+            if bc != "periodic":
+                raise ValueError(f"unknown bc {bc!r}")
+            check(self._lib.vh_muscl_set_bc(self._h, 1, None, None))
+            self._bc_keep = None
+        else:
+            nf, shape = self.nf, (self.nf,) + self.dims[::-1]
+            nbytes = nf * self.ncell * self.eltype.itemsize
+            lib, h, dt = self._lib, self._h, self.eltype
+
+            def hook(_user, state, _stream, stage):
+                try:
+                    lib.vh_muscl_sync(h)
+                    host = np.empty(shape, dtype=dt)
+                    if lib.vh_buffer_download(host.ctypes.data_as(C.c_void_p), state, nbytes):
+                        return 2
+                    bc(host, int(stage))
+                    host = np.ascontiguousarray(host, dtype=dt)
+                    return 3 if lib.vh_buffer_upload(state, host.ctypes.data_as(C.c_void_p), nbytes) else 0
+                except Exception:   # the error surfaces as the step's failure
+                    return 1
+
+            cb = _BC_FN(hook)
+            self._bc_keep = cb
+            check(self._lib.vh_muscl_set_bc(self._h, 2, C.cast(cb, C.c_void_p), None))
 
     def max_wavespeed(self) -> float:
         """max_wavespeed (lib/PPMKernels/src/muscl_grid.jl:689-695)."""
@@ -195,14 +234,20 @@ def total_field(f, dims, ng, dx):
 
 
 def muscl_step_3d(D, S1, S2, S3, Tau, dims, ng, *, dt, gamma, theta=1.5, dx=1.0, small_rho=1e-10,
-                  ge=None, eta1=1e-3, fluxrec=None):
-    """muscl_step_3d!(D, S1, S2, S3, Tau, dims, ng; dt, gamma, theta, dx, small_rho, ge,
-    eta1, fluxrec) on host arrays, updated in place through a device context. fluxrec:
-    None or a (3, nf, N) array filled with the recorded fluxes (nf = 6 with ge)."""
+                  bc=None, ge=None, eta1=1e-3, fluxrec=None):
+    """muscl_step_3d!(D, S1, S2, S3, Tau, dims, ng; dt, gamma, theta, dx, small_rho, bc!,
+    ge, eta1, fluxrec) on host arrays, updated in place through a device context.
+    bc: None, "periodic", or bc!(D, S1, S2, S3, Tau[, ge]) on flat host field
+    arrays, called before each stage. fluxrec: None or a (3, nf, N) array filled
+    with the recorded fluxes (nf = 6 with ge)."""
     fields = (D, S1, S2, S3, Tau) + ((ge,) if ge is not None else ())
     g = MusclGrid(dims, ng, gamma, theta, dx, small_rho, eltype=np.asarray(D).dtype, dual=ge is not None,
                   eta1=eta1)
     try:
+        if callable(bc):
+            g.set_bc(lambda V, stage: bc(*[f.reshape(-1) for f in V]))
+        else:
+            g.set_bc(bc)
         g.upload(np.stack([np.ravel(f) for f in fields]))
         g.step(dt, fluxrec is not None)
         U = g.state()
