@@ -233,6 +233,12 @@ class HostPool {
   }
 
  private:
+  // A build issues a few jobs back to back: threads spin this many pauses
+  // (tens of microseconds) for the next job or the end of the current one
+  // before they sleep, which saves a wake-up per job.
+  static constexpr int kSpin = 1 << 14;
+  static void pause() { __builtin_ia32_pause(); }
+
   explicit HostPool(int workers) : pid_(getpid()) {
     for (int i = 0; i < workers; ++i) std::thread([this] { loop(); }).detach();
     nworkers_ = workers;
@@ -240,43 +246,52 @@ class HostPool {
   void loop() {
     long long seen = 0;
     for (;;) {
-      const std::function<void(int)>* f;
-      int n;
-      {
-        std::unique_lock<std::mutex> lk(mu_);
-        cv_.wait(lk, [&] { return gen_ != seen; });
-        seen = gen_;
-        f = job_;
-        n = njob_;
+      long long g = gen_.load(std::memory_order_acquire);
+      for (int i = 0; g == seen && i < kSpin; ++i) {
+        pause();
+        g = gen_.load(std::memory_order_acquire);
       }
+      if (g == seen) {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_.load(std::memory_order_acquire) != seen; });
+        g = gen_.load(std::memory_order_acquire);
+      }
+      seen = g;
+      const std::function<void(int)>* f = job_;
+      const int n = njob_;
       for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
-      std::lock_guard<std::mutex> lk(mu_);
-      if (--active_ == 0) done_.notify_one();
+      if (active_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        std::lock_guard<std::mutex> lk(mu_);
+        done_.notify_one();
+      }
     }
   }
   void go(int n, const std::function<void(int)>& f) {
     std::lock_guard<std::mutex> run_lk(run_mu_);   // one job at a time
+    job_ = &f;
+    njob_ = n;
+    next_.store(0);
+    active_.store(nworkers_);
     {
       std::lock_guard<std::mutex> lk(mu_);
-      job_ = &f;
-      njob_ = n;
-      next_.store(0);
-      active_ = nworkers_;
-      ++gen_;
+      gen_.fetch_add(1, std::memory_order_release);   // publishes the job to spinning and sleeping workers
     }
     cv_.notify_all();
     for (int i; (i = next_.fetch_add(1)) < n;) f(i);
-    std::unique_lock<std::mutex> lk(mu_);
-    done_.wait(lk, [&] { return active_ == 0; });
+    for (int i = 0; active_.load(std::memory_order_acquire) != 0 && i < kSpin; ++i) pause();
+    if (active_.load(std::memory_order_acquire) != 0) {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_.wait(lk, [&] { return active_.load(std::memory_order_acquire) == 0; });
+    }
   }
   pid_t pid_;
   int nworkers_ = 0;
   std::mutex run_mu_, mu_;
   std::condition_variable cv_, done_;
   const std::function<void(int)>* job_ = nullptr;
-  int njob_ = 0, active_ = 0;
-  long long gen_ = 0;
-  std::atomic<int> next_{0};
+  int njob_ = 0;
+  std::atomic<long long> gen_{0};
+  std::atomic<int> next_{0}, active_{0};
 };
 
 // Output of one chunk (a contiguous run of blocks) of the parallel table
@@ -1323,8 +1338,8 @@ static void free_tables(vh_blocks* m) {
   forget_tables(m);
 }
 
-// Uploads every device table of H into arena slot a (grown with headroom when
-// too small) through the pinned staging buffer in one copy, zeroes the flux-
+// Uploads every device table of H into arena slot a (allocated at twice the
+// need when too small) through the pinned staging buffer in one copy, zeroes the flux-
 // register accumulators and points the context's tables there. extra/extra_bytes
 // (optional) is packed behind the tables; *extra_dev receives its address.
 // Stream-ordered: the caller synchronises before the staging is reused.
@@ -1371,7 +1386,7 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     (void)hipFree(m->arena[a]);
     m->arena[a] = nullptr;
     m->arena_cap[a] = 0;
-    const size_t cap = total + total / 4;
+    const size_t cap = 2 * total;   // regrids grow the mesh: keep room for the next ones
     VH_HIP(hipMalloc(&m->arena[a], cap));
     m->arena_cap[a] = cap;
   }
@@ -1379,7 +1394,7 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     if (m->stage) (void)hipHostFree(m->stage);
     m->stage = nullptr;
     m->stage_cap = 0;
-    const size_t cap = staged + staged / 4;
+    const size_t cap = 2 * staged;
     VH_HIP(hipHostMalloc(&m->stage, cap, 0));
     m->stage_cap = cap;
   }
@@ -2093,7 +2108,7 @@ int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const 
   void* dst = m->U1;
   bool fresh = false;
   if (need > m->state_cap) {
-    const size_t cap = need + need / 4;
+    const size_t cap = 2 * need;
     VH_HIP(hipMalloc(&dst, cap));
     fresh = true;
     m->state_cap = cap;
