@@ -20,12 +20,15 @@
 //
 // * fast (VH_FAST_PROFILE = 1, built with FMA contraction): the same formulas with
 //   a*b+c fused, E = p·(1/(γ−1)), sound speeds c = (γp)·rsqrt(γpρ) refined by
-//   Newton steps, divisions through a refined reciprocal, and the star state
-//   rearranged so that one reciprocal 1/(S_K − S*) replaces the three
-//   divisions by ρ, by ρ(S_K − u_n) and by (S_K − S*):
-//     F*_K = F_K + S_K·(U*_K − U_K),   U*_K = r·(ρ, ρv*, ·) with r = (S_K − u_n)/(S_K − S*),
-//     ρ_*E_* = r·E + ρ·r·(S* − u_n)·(S* + p/(ρ(S_K − u_n)))
-//            = [(S_K − u_n)·E + (S* − u_n)·(ρ(S_K − u_n)·S* + p)] / (S_K − S*).
+//   Newton steps, divisions through a refined reciprocal, and the star flux
+//   written through the star state itself, so that one reciprocal 1/(S_K − S*)
+//   replaces the three divisions by ρ, by ρ(S_K − u_n) and by (S_K − S*) and the
+//   physical flux F_K is formed only on supersonic faces. With q = ρ(S_K − u_n),
+//   ρ* = q/(S_K − S*), p* = p + q(S* − u_n) and
+//     ρ_*E_* = [(S_K − u_n)·E + (S* − u_n)·(q·S* + p)] / (S_K − S*),
+//   the Rankine-Hugoniot form F*_K = F_K + S_K·(U*_K − U_K) equals
+//     F*_K = (ρ*S*,  ρ*S*·S* + p* (normal),  ρ*S*·v_t (tangential),  (ρ_*E_* + p*)·S*),
+//   which costs about ten fewer operations per face.
 //   Results agree with the exact profile to a few ulps per face.
 #pragma once
 
@@ -210,8 +213,8 @@ __device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T 
   const bool supL = SL >= T(0);
   const bool supR = !supL && (SR <= T(0));
 #if VH_FAST_PROFILE
-  const T Ss = fast_div(pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR),
-                        rL * (SL - unL) - rR * (SR - unR));
+  const T aL = rL * (SL - unL), aR = rR * (SR - unR);
+  const T Ss = fast_div(vfma(aL, unL, pR - pL) - aR * unR, aL - aR);
 #else
   const T Ss = (pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR)) /
                (rL * (SL - unL) - rR * (SR - unR));
@@ -222,6 +225,28 @@ __device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T 
   const T SK = useL ? SL : SR;
   const T un = useL ? unL : unR;
   const T E = energy(W, g, igm1);
+#if VH_FAST_PROFILE
+  Cons<T> o;
+  if (supL || supR) {
+    o.d = W.r * un;
+    o.mx = o.d * W.vx + (dim == 1 ? W.p : T(0));
+    o.my = o.d * W.vy + (dim == 2 ? W.p : T(0));
+    o.mz = o.d * W.vz + (dim == 3 ? W.p : T(0));
+    o.e = (E + W.p) * un;
+    return o;
+  }
+  const T sku = SK - un, su = Ss - un;
+  const T q = W.r * sku;
+  const T inv = fast_rcp(SK - Ss);
+  const T ps = vfma(q, su, W.p);
+  const T es = vfma(sku, E, su * vfma(q, Ss, W.p)) * inv;
+  o.d = (q * inv) * Ss;
+  o.mx = dim == 1 ? vfma(o.d, Ss, ps) : o.d * W.vx;
+  o.my = dim == 2 ? vfma(o.d, Ss, ps) : o.d * W.vy;
+  o.mz = dim == 3 ? vfma(o.d, Ss, ps) : o.d * W.vz;
+  o.e = (es + ps) * Ss;
+  return o;
+#else
   const T m1 = W.r * W.vx, m2 = W.r * W.vy, m3 = W.r * W.vz;
   Cons<T> F;
   F.d = W.r * un;
@@ -236,15 +261,9 @@ __device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T 
   const T vsx = dim == 1 ? Ss : W.vx;
   const T vsy = dim == 2 ? Ss : W.vy;
   const T vsz = dim == 3 ? Ss : W.vz;
-#if VH_FAST_PROFILE
-  const T inv = fast_rcp(SK - Ss);
-  const T factor = q * inv;
-  const T fE = ((SK - un) * E + (Ss - un) * (q * Ss + W.p)) * inv;
-#else
   const T factor = q / (SK - Ss);
   const T Es = E / W.r + (Ss - un) * (Ss + W.p / q);
   const T fE = factor * Es;
-#endif
   Cons<T> o;
   o.d = F.d + SK * (factor - W.r);
   o.mx = F.mx + SK * (factor * vsx - m1);
@@ -252,6 +271,7 @@ __device__ __forceinline__ Cons<T> hllc(const Prim<T>& WL, const Prim<T>& WR, T 
   o.mz = F.mz + SK * (factor * vsz - m3);
   o.e = F.e + SK * (fE - E);
   return o;
+#endif
 }
 
 template <typename T>

@@ -42,11 +42,11 @@
 #ifndef VH_MIN_WAVES_F32
 #define VH_MIN_WAVES_F32 4
 #endif
-// 1: issue the loads of the cell's own state at the start of the plane (more
-// latency hidden, more registers live across the plane); 0: load it in the
-// update phase.
+// 1: in the periodic form, issue the loads of the cell's own state at the start
+// of the plane (more latency hidden, 10-20 more registers live across the
+// plane, which the boundary forms would spill); 0: load it in the update phase.
 #ifndef VH_EARLY_OWN
-#define VH_EARLY_OWN 0
+#define VH_EARLY_OWN 1
 #endif
 // 1: load the halo cells of the next plane while the current one is processed
 // (tiles whose halo fits one cell per lane); 0: load them when they are staged.
@@ -402,12 +402,13 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     if constexpr (RANK == 3) {
       if (k + 1 < kz1) unext = load_cons(Us, n, column_src<PER>(G, col, k + 3));
     }
-#if VH_EARLY_OWN
-    if (active) {
-      uown = load_cons(Us, n, c);
-      if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
+    constexpr bool EARLY = VH_EARLY_OWN && PER;
+    if constexpr (EARLY) {
+      if (active) {
+        uown = load_cons(Us, n, c);
+        if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
+      }
     }
-#endif
 
     // ---- P0: stage the plane's primitive state in LDS ----
     Prim<T> wc;
@@ -553,10 +554,10 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
 
     // ---- PC: gather the six face fluxes in the reference order and update ----
     if (active) {
-#if !VH_EARLY_OWN
-      uown = load_cons(Us, n, c);
-      if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
-#endif
+      if constexpr (!EARLY) {
+        uown = load_cons(Us, n, c);
+        if constexpr (STAGE2) u0own = load_cons_stream(U0, n, c);
+      }
       Cons<T> fxlo, fxhi, fylo, fyhi;
       fxlo = lds_getf(ff, L_::FS, L_::xs(ty, tx));
       fxhi = lds_getf(ff, L_::FS, L_::xs(ty, tx + 1));
