@@ -151,7 +151,7 @@ def test_describe_counts_of_refined_quadtrees(rounds, counts):
 @pytest.mark.parametrize("name,zchunk,digest", [
     ("ring3", 0, 0xf9056b64bf2defc7),
     ("ring8", 0, 0xff84b41d15663f51),
-    ("pairs300_3d", 0, 0xd9a2bf2a2e96a910),
+    ("pairs300_3d", 0, 0x82220695a4d1ae80),
     ("pairs300_3d", 8, 0xbe8548bde1bc6ebc),
     ("pairs1000_2d", 0, 0xf95618b0ccd893a3),
 ])
@@ -159,7 +159,11 @@ def test_threaded_builder_tables_equal_serial_ones(name, zchunk, digest):
     """The builder splits its work over host threads (contiguous block chunks,
     joined in order). Every derived table must be the serial walk's: the digests
     were recorded with the earlier single-threaded builder (hash-map lookups),
-    and every thread count reproduces them."""
+    and every thread count reproduces them. Exception: the 3D cost-sorted tile
+    lists are grouped by tile shape class since the stage kernel has a
+    compile-time form per shape; that moved the pairs300_3d/zchunk 0 digest
+    (from 0xd9a2bf2a2e96a910) and left the other four, whose lists hold one class
+    (2D) or were already grouped (zchunk 8), as recorded."""
     m = {"ring3": lambda: _ring_quadtree(3), "ring8": lambda: _ring_quadtree(8),
          "pairs300_3d": lambda: BM.random_block_pairs(300, seed=1),
          "pairs1000_2d": lambda: BM.random_block_pairs(1000, seed=3, rank=2)}[name]()

@@ -47,20 +47,18 @@ constexpr int BFS = 592;     // max TY*(TX+2) + (TY+2)*TX (TX, TY <= 32, TX*TY <
 //   y slot (r, c) = TY*(TX+2) + r*TX + c, r = 0..TY: hi face state of cell r-1
 //     of column c, then the flux of y face r; r = TY+1: lo state of halo cell TY.
 // The final workgroup reduction reuses the start of w (after a barrier).
-template <typename T>
+// Compile-time tile shapes (TX×TY, one lane per cell) have their exact LDS
+// footprint; the runtime shape (CTX = 0) the maximum above.
+template <typename T, int CTX = 0, int CTY = 0>
 struct alignas(16) BlockLDS {
-  T w[5][BW];
-  T f[5][BFS];
+  static constexpr int W = CTX ? (CTY + 4) * (CTX + 4) : BW;
+  static constexpr int F = CTX ? CTY * (CTX + 2) + (CTY + 2) * CTX : BFS;
+  T w[5][W];
+  T f[5][F];
 };
 
-template <typename T>
-__device__ __forceinline__ Prim<T> bl_getw(const BlockLDS<T>& S, int i) {
-  Prim<T> w;
-  w.r = S.w[0][i]; w.vx = S.w[1][i]; w.vy = S.w[2][i]; w.vz = S.w[3][i]; w.p = S.w[4][i];
-  return w;
-}
-template <typename T>
-__device__ __forceinline__ void bl_putw(BlockLDS<T>& S, int i, const Prim<T>& w) {
+template <typename L, typename T>
+__device__ __forceinline__ void bl_putw(L& S, int i, const Prim<T>& w) {
   S.w[0][i] = w.r; S.w[1][i] = w.vx; S.w[2][i] = w.vy; S.w[3][i] = w.vz; S.w[4][i] = w.p;
 }
 
@@ -176,11 +174,19 @@ __device__ __forceinline__ void block_grav_add(Cons<T>& acc, const Cons<T>& U, c
   acc.e = acc.e + (-esV);
 }
 
-template <typename T, int RANK, bool STAGE2, bool FLOOR64, bool GRAV = false>
-__global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 : VH_BLOCK_MIN_WAVES) block_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
+// CTX×CTY > 0: every tile of the launch has that shape (one lane per cell; the
+// index arithmetic folds at compile time); 0: the tile shape is read per tile
+// and a workgroup of BNT lanes covers it.
+template <typename T, int RANK, bool STAGE2, bool FLOOR64, bool GRAV = false, int CTX = 0, int CTY = 0>
+__global__ void __launch_bounds__(CTX ? CTX * CTY : BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 : VH_BLOCK_MIN_WAVES) block_stage_kernel(const T* __restrict__ Us, const T* U0, T* Uout,
                                                                        const double* __restrict__ dtp,
                                                                        double* __restrict__ partials, BlocksDev B) {
-  __shared__ BlockLDS<T> S;
+  constexpr int NT = CTX ? CTX * CTY : BNT;   // lanes per workgroup
+  constexpr int NW = NT / 64;                 // waves per workgroup
+  static_assert(NT % 64 == 0 && NT <= BNT, "tile shape");
+  using LDS = BlockLDS<T, CTX, CTY>;
+  constexpr int LWP = LDS::W, LFP = LDS::F;   // field pitches of w and f
+  __shared__ LDS S;
   const T g = T(B.gamma);
   const T igm1 = T(B.igm1);
   const int tid = threadIdx.x;
@@ -200,9 +206,9 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
     bg.off = bk->off;
     const int nx = bg.nx, ny = bg.ny, nz = bg.nz;
     const int bxl = bk->bc[0], bxh = bk->bc[1], byl = bk->bc[2], byh = bk->bc[3], bzl = bk->bc[4], bzh = bk->bc[5];
-    const int TX = tl.tx, TY = tl.ty;
+    const int TX = CTX ? CTX : tl.tx, TY = CTX ? CTY : tl.ty;
     const int WX = TX + 4;
-    const bool lane = tid < TX * TY;
+    const bool lane = CTX ? true : tid < TX * TY;
     const int tx = lane ? tid % TX : 0, ty = lane ? tid / TX : 0;
     const int x0 = tl.x0, y0 = tl.y0, kz0 = tl.z0, kz1 = tl.z1;
     const T invV = T(dt) / (T(1.0) * T(bk->V));
@@ -270,10 +276,10 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
       }
       // Halo, rim and edge work goes to lanes counted from a wave that moves
       // by one per plane, so every SIMD gets its share.
-      const int lt = (tid + 64 * ((k + t) & 3)) & (BNT - 1);
+      const int lt = NW == 1 ? tid : (tid + 64 * ((k + t) % NW)) % NT;
       const int NHX = 4 * TY;
       const int NHY = (RANK > 1) ? 4 * TX : 0;
-      for (int h = lt; h < NHX + NHY; h += BNT) {
+      for (int h = lt; h < NHX + NHY; h += NT) {
         int lx, ly;
         Prim<T> w;
         if (h < NHX) {
@@ -307,20 +313,20 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
       Prim<T> xlo, ylo;
       if (lane) {
         const int o = (ty + 2) * WX + tx + 2;
-        const Prim<T> sx = slope(lds_get5(wf, BW, o - 1), wc, lds_get5(wf, BW, o + 1));
+        const Prim<T> sx = slope(lds_get5(wf, LWP, o - 1), wc, lds_get5(wf, LWP, o + 1));
         xlo = face_value(wc, sx, T(-0.5));
-        lds_put5(ff, BFS, ty * (TX + 2) + tx + 1, face_value(wc, sx, T(0.5)));
+        lds_put5(ff, LFP, ty * (TX + 2) + tx + 1, face_value(wc, sx, T(0.5)));
         if constexpr (RANK > 1) {
-          const Prim<T> sy = slope(lds_get5(wf, BW, o - WX), wc, lds_get5(wf, BW, o + WX));
+          const Prim<T> sy = slope(lds_get5(wf, LWP, o - WX), wc, lds_get5(wf, LWP, o + WX));
           ylo = face_value(wc, sy, T(-0.5));
-          lds_put5(ff, BFS, NXS + (ty + 1) * TX + tx, face_value(wc, sy, T(0.5)));
+          lds_put5(ff, LFP, NXS + (ty + 1) * TX + tx, face_value(wc, sy, T(0.5)));
         }
       }
       // Halo rim through one code path: x hi state of cell -1 and lo state of
       // cell TX per row, y hi state of cell -1 and lo state of cell TY per column.
       const int NRX = 2 * TY;
       const int NRY = (RANK > 1) ? 2 * TX : 0;
-      for (int h = lt; h < NRX + NRY; h += BNT) {
+      for (int h = lt; h < NRX + NRY; h += NT) {
         const bool isy = h >= NRX;
         const int q = isy ? h - NRX : h;
         const int line = q >> 1;
@@ -334,28 +340,28 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
           dst = NXS + (hi ? TY + 1 : 0) * TX + line;
         }
         const int d = isy ? WX : 1;
-        const Prim<T> wm = lds_get5(wf, BW, ic);
-        const Prim<T> sl = slope(lds_get5(wf, BW, ic - d), wm, lds_get5(wf, BW, ic + d));
-        lds_put5(ff, BFS, dst, face_value(wm, sl, hi ? T(-0.5) : T(0.5)));
+        const Prim<T> wm = lds_get5(wf, LWP, ic);
+        const Prim<T> sl = slope(lds_get5(wf, LWP, ic - d), wm, lds_get5(wf, LWP, ic + d));
+        lds_put5(ff, LFP, dst, face_value(wm, sl, hi ? T(-0.5) : T(0.5)));
       }
       __syncthreads();
 
       // ---- PB: face fluxes (each face of the tile once) ----
       if (lane && gy < ny && gx <= nx) {
         const int ix = ty * (TX + 2) + tx;
-        lds_putf(ff, BFS, ix, axis_face<false>(gx, nx, bxl, bxh, lds_get5(ff, BFS, ix), xlo, g, igm1, 1));
+        lds_putf(ff, LFP, ix, axis_face<false>(gx, nx, bxl, bxh, lds_get5(ff, LFP, ix), xlo, g, igm1, 1));
       }
       if constexpr (RANK > 1) {
         if (lane && gx < nx && gy <= ny) {
           const int iy = NXS + ty * TX + tx;
-          lds_putf(ff, BFS, iy, axis_face<false>(gy, ny, byl, byh, lds_get5(ff, BFS, iy), ylo, g, igm1, 2));
+          lds_putf(ff, LFP, iy, axis_face<false>(gy, ny, byl, byh, lds_get5(ff, LFP, iy), ylo, g, igm1, 2));
         }
       }
       // Tile hi-edge faces (x face TX of each row, y face TY of each column)
       // through one code path, a y face in the frame of an x face.
       const int NEX = TY;
       const int NEY = (RANK > 1) ? TX : 0;
-      for (int h = lt; h < NEX + NEY; h += BNT) {
+      for (int h = lt; h < NEX + NEY; h += NT) {
         const bool isy = h >= NEX;
         const int line = isy ? h - NEX : h;
         const int gp = isy ? y0 + TY : x0 + TX;
@@ -364,9 +370,9 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
         if (inside && gp <= na) {
           const int sl = isy ? NXS + TY * TX + line : line * (TX + 2) + TX;
           const int sr = isy ? sl + TX : sl + 1;
-          const Prim<T> L = swap_xy(lds_get5(ff, BFS, sl), isy);
-          const Prim<T> R = swap_xy(lds_get5(ff, BFS, sr), isy);
-          lds_putf(ff, BFS, sl,
+          const Prim<T> L = swap_xy(lds_get5(ff, LFP, sl), isy);
+          const Prim<T> R = swap_xy(lds_get5(ff, LFP, sr), isy);
+          lds_putf(ff, LFP, sl,
                    swap_xy(axis_face<false>(gp, na, isy ? byl : bxl, isy ? byh : bxh, L, R, g, igm1, 1), isy));
         }
       }
@@ -391,10 +397,10 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
         const int ix = ty * (TX + 2) + tx;
         const int iy = NXS + ty * TX + tx;
         auto flux = [&](int s) -> Cons<T> {
-          if (s == 0) return lds_getf(ff, BFS, ix);
-          if (s == 1) return lds_getf(ff, BFS, ix + 1);
-          if (s == 2) return lds_getf(ff, BFS, iy);
-          if (s == 3) return lds_getf(ff, BFS, iy + TX);
+          if (s == 0) return lds_getf(ff, LFP, ix);
+          if (s == 1) return lds_getf(ff, LFP, ix + 1);
+          if (s == 2) return lds_getf(ff, LFP, iy);
+          if (s == 3) return lds_getf(ff, LFP, iy + TX);
           if (s == 4) return fzlo;
           return fzhi;
         };
@@ -510,7 +516,7 @@ __global__ void __launch_bounds__(BNT, sizeof(T) == 4 ? VH_BLOCK_MIN_WAVES_F32 :
   }
 
   if constexpr (STAGE2) {
-    const double m = block_max<BNT>(cfl_max, reinterpret_cast<double*>(&S.w[0][0]));
+    const double m = block_max<NT>(cfl_max, reinterpret_cast<double*>(&S.w[0][0]));
     if (tid == 0) partials[blockIdx.x] = m;
   }
 }

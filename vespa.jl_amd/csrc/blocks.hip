@@ -187,6 +187,9 @@ struct HostTables {
   // tiles: [block order][all, cost-sorted][level 0 .. maxlevel, cost-sorted]
   std::vector<TileDev> tiles;
   std::vector<int> tile_list_off;        // list l: [tile_list_off[l], tile_list_off[l+1])
+  // list l, shape class c (block_shape_class): tiles [tile_list_off[l] + tile_class_off[l*(K+1) + c],
+  // tile_list_off[l] + tile_class_off[l*(K+1) + c + 1]), K = kBlockClasses
+  std::vector<int> tile_class_off;
   std::vector<int> block_tile_start;     // per block, in the block-order list (nb + 1)
   // CFL block lists: [all][level 0 .. maxlevel]
   std::vector<int> cfl_blocks;
@@ -804,18 +807,29 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   }
   H.block_tile_start[nb] = (int)base.size();
   const int nt = (int)base.size();
+  // A list holds its tiles grouped by shape class (one launch per class), each
+  // group in decreasing cost.
+  std::vector<int> cls(nt);
+  for (int i = 0; i < nt; ++i) cls[i] = block_shape_class(rank, base[i].tx, base[i].ty);
   auto sorted_list = [&](int lev) {
     std::vector<int> idx;
     for (int i = 0; i < nt; ++i)
       if (lev < 0 || T.level[base[i].block] == lev) idx.push_back(i);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return cls[a] != cls[b] ? cls[a] < cls[b] : cost[a] > cost[b]; });
     return idx;
   };
   H.tiles = base;
   H.tile_list_off.clear();
+  H.tile_class_off.clear();
   H.tile_list_off.push_back(nt);   // list 0: all, starts after the block-order copy
   for (int l = -1; l <= T.maxlevel; ++l) {
-    for (int i : sorted_list(l)) H.tiles.push_back(base[i]);
+    const std::vector<int> idx = sorted_list(l);
+    for (int c = 0, q = 0; c <= kBlockClasses; ++c) {
+      while (c < kBlockClasses && q < (int)idx.size() && cls[idx[q]] < c) ++q;
+      H.tile_class_off.push_back(c < kBlockClasses ? q : (int)idx.size());
+    }
+    for (int i : idx) H.tiles.push_back(base[i]);
     H.tile_list_off.push_back((int)H.tiles.size());
   }
 
@@ -1106,11 +1120,24 @@ static int upload_vec(V** dst, const std::vector<V>& src) {
   return 0;
 }
 
-// Tile list l: 0 all blocks, 1 + L level L.
-static int list_wgs(const vh_blocks* m, int l) {
-  const int n = m->H.tile_list_off[l + 1] - m->H.tile_list_off[l];
-  const int cap = m->max_wgs > 0 ? m->max_wgs : 3 * m->ncu;
+// Tile list l: 0 all blocks, 1 + L level L. A stage launches one persistent
+// kernel per tile shape class of the list; max_wgs (default 3 per CU) caps the
+// workgroups of 256 lanes, a class with smaller workgroups gets proportionally
+// more.
+static int class_tiles(const vh_blocks* m, int l, int c) {
+  const int* o = m->H.tile_class_off.data() + (size_t)l * (kBlockClasses + 1);
+  return o[c + 1] - o[c];
+}
+static int class_wgs(const vh_blocks* m, int l, int c) {
+  const int n = class_tiles(m, l, c);
+  const int cap = (m->max_wgs > 0 ? m->max_wgs : 3 * m->ncu) * (256 / block_class_lanes(c));
   return n < cap ? n : cap;
+}
+// Workgroups of a stage launch over list l (= the CFL partials its stage 2 writes).
+static int list_wgs(const vh_blocks* m, int l) {
+  int np = 0;
+  for (int c = 0; c < kBlockClasses; ++c) np += class_wgs(m, l, c);
+  return np;
 }
 
 static void enqueue_link(vh_blocks* m, const void* Us) {
@@ -1120,14 +1147,22 @@ static void enqueue_link(vh_blocks* m, const void* Us) {
 }
 
 static void enqueue_tiles(vh_blocks* m, bool stage2, int l, const void* Us, const void* U0, void* Uout) {
-  BlocksDev B = blocks_dev(m);
-  B.tiles = m->d_tiles + m->H.tile_list_off[l];
-  B.ntiles = m->H.tile_list_off[l + 1] - m->H.tile_list_off[l];
-  const int wgs = list_wgs(m, l);
   const double* dtp = m->clock + 1;
-  if (m->mode == VH_FAST) fast::launch_block_stage(m->dtype, m->T.rank, stage2, wgs, m->stream, Us, U0, Uout, dtp, m->partials, B);
-  else exact::launch_block_stage(m->dtype, m->T.rank, stage2, wgs, m->stream, Us, U0, Uout, dtp, m->partials, B);
-  if (l == 0 && stage2) m->stage_wgs = wgs;
+  const int* co = m->H.tile_class_off.data() + (size_t)l * (kBlockClasses + 1);
+  int np = 0;
+  for (int c = 0; c < kBlockClasses; ++c) {
+    const int wgs = class_wgs(m, l, c);
+    if (wgs == 0) continue;
+    BlocksDev B = blocks_dev(m);
+    B.tiles = m->d_tiles + m->H.tile_list_off[l] + co[c];
+    B.ntiles = co[c + 1] - co[c];
+    if (m->mode == VH_FAST)
+      fast::launch_block_stage(m->dtype, m->T.rank, c, stage2, wgs, m->stream, Us, U0, Uout, dtp, m->partials + np, B);
+    else
+      exact::launch_block_stage(m->dtype, m->T.rank, c, stage2, wgs, m->stream, Us, U0, Uout, dtp, m->partials + np, B);
+    np += wgs;
+  }
+  if (l == 0 && stage2) m->stage_wgs = np;
 }
 
 // One launch per block (the comparison loop): tiles of block b only.
@@ -1139,9 +1174,9 @@ static void enqueue_tiles_per_block(vh_blocks* m, bool stage2, const void* Us, c
     B.tiles = m->d_tiles + t0;
     B.ntiles = t1 - t0;
     if (m->mode == VH_FAST)
-      fast::launch_block_stage(m->dtype, m->T.rank, stage2, t1 - t0, m->stream, Us, U0, Uout, dtp, m->partials + t0, B);
+      fast::launch_block_stage(m->dtype, m->T.rank, 0, stage2, t1 - t0, m->stream, Us, U0, Uout, dtp, m->partials + t0, B);
     else
-      exact::launch_block_stage(m->dtype, m->T.rank, stage2, t1 - t0, m->stream, Us, U0, Uout, dtp, m->partials + t0, B);
+      exact::launch_block_stage(m->dtype, m->T.rank, 0, stage2, t1 - t0, m->stream, Us, U0, Uout, dtp, m->partials + t0, B);
   }
   if (stage2) m->stage_wgs = m->H.block_tile_start[m->T.nb];
 }
@@ -1304,7 +1339,7 @@ static int evolve_level(vh_blocks* m, int level, double dt_target, double cfl, b
 static int alloc_partials(vh_blocks* m) {
   int need = kCflWgs;
   need = std::max(need, m->H.block_tile_start[m->T.nb]);
-  need = std::max(need, m->max_wgs > 0 ? m->max_wgs : 3 * m->ncu);
+  need = std::max(need, list_wgs(m, 0));
   if (need > m->npartials) {
     if (m->partials) VH_HIP(hipFree(m->partials));
     m->partials = nullptr;

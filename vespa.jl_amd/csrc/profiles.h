@@ -25,6 +25,19 @@ inline TileShape tile_for(int rank, int variant) {
   return kTiles1[variant];
 }
 
+// Tile shapes the rank-3 block stage kernel is compiled for (class 1..4; one
+// lane per cell). Tiles of any other shape, of rank 1 and 2 meshes and of meshes
+// with self-gravity run the runtime-shape kernel (class 0, 256 lanes).
+static const TileShape kBlockShapes[] = {{32, 8}, {16, 16}, {24, 8}, {8, 8}};
+constexpr int kBlockClasses = 5;
+inline int block_shape_class(int rank, int tx, int ty) {
+  if (rank != 3) return 0;
+  for (int c = 0; c < 4; ++c)
+    if (kBlockShapes[c].tx == tx && kBlockShapes[c].ty == ty) return c + 1;
+  return 0;
+}
+inline int block_class_lanes(int cls) { return cls ? kBlockShapes[cls - 1].tx * kBlockShapes[cls - 1].ty : 256; }
+
 #define VH_DECLARE_PROFILE(ns)                                                                      \
   namespace ns {                                                                                    \
   /* dtype 0: double state; 1: float state (pressure floor in double). */                          \
@@ -37,9 +50,9 @@ inline TileShape tile_for(int rank, int variant) {
                         const int32_t* axis, double gamma, void* F);                                \
   void launch_link_flux(int dtype, int nfaces, hipStream_t s, const void* U, const BlocksDev& B,    \
                         void* F);                                                                   \
-  void launch_block_stage(int dtype, int rank, bool stage2, int wgs, hipStream_t s, const void* Us, \
-                          const void* U0, void* Uout, const double* dtp, double* partials,          \
-                          const BlocksDev& B);                                                      \
+  void launch_block_stage(int dtype, int rank, int cls, bool stage2, int wgs, hipStream_t s,         \
+                          const void* Us, const void* U0, void* Uout, const double* dtp,            \
+                          double* partials, const BlocksDev& B);                                    \
   void launch_block_cfl(int dtype, int rank, int wgs, hipStream_t s, const void* U,                 \
                         const BlocksDev& B, const int* blist, const long long* prefix, int nb,      \
                         double* partials);                                                          \

@@ -129,37 +129,51 @@ void launch_link_flux(int dtype, int nfaces, hipStream_t s, const void* U, const
                        (float*)F);
 }
 
-template <typename T, int RANK, bool FLOOR64, bool GRAV>
+template <typename T, int RANK, bool FLOOR64, bool GRAV, int CTX = 0, int CTY = 0>
 static void block_stage_tg(bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
                            const double* dtp, double* partials, const BlocksDev& B) {
+  constexpr int NT = CTX ? CTX * CTY : BNT;
   if (stage2)
-    hipLaunchKernelGGL((block_stage_kernel<T, RANK, true, FLOOR64, GRAV>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
-                       (const T*)U0, (T*)Uout, dtp, partials, B);
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, true, FLOOR64, GRAV, CTX, CTY>), dim3(wgs), dim3(NT), 0, s,
+                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, B);
   else
-    hipLaunchKernelGGL((block_stage_kernel<T, RANK, false, FLOOR64, GRAV>), dim3(wgs), dim3(BNT), 0, s, (const T*)Us,
-                       (const T*)U0, (T*)Uout, dtp, partials, B);
+    hipLaunchKernelGGL((block_stage_kernel<T, RANK, false, FLOOR64, GRAV, CTX, CTY>), dim3(wgs), dim3(NT), 0, s,
+                       (const T*)Us, (const T*)U0, (T*)Uout, dtp, partials, B);
 }
 
 // With self-gravity (B.gravg set) the stage adds the gravity source of the
-// step's g field to every cell's flux sum.
+// step's g field to every cell's flux sum (runtime-shape kernel). Tile class cls
+// (block_shape_class) picks the compile-time shape of rank-3 tiles.
 template <typename T, int RANK, bool FLOOR64>
-static void block_stage_t(bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
+static void block_stage_t(int cls, bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0, void* Uout,
                           const double* dtp, double* partials, const BlocksDev& B) {
-  if (B.gravg) block_stage_tg<T, RANK, FLOOR64, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
-  else block_stage_tg<T, RANK, FLOOR64, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+  if (B.gravg) {
+    block_stage_tg<T, RANK, FLOOR64, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    return;
+  }
+  if constexpr (RANK == 3) {
+    switch (cls) {
+      case 1: block_stage_tg<T, 3, FLOOR64, false, 32, 8>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B); return;
+      case 2: block_stage_tg<T, 3, FLOOR64, false, 16, 16>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B); return;
+      case 3: block_stage_tg<T, 3, FLOOR64, false, 24, 8>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B); return;
+      case 4: block_stage_tg<T, 3, FLOOR64, false, 8, 8>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B); return;
+      default: break;
+    }
+  }
+  block_stage_tg<T, RANK, FLOOR64, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
 }
 
-void launch_block_stage(int dtype, int rank, bool stage2, int wgs, hipStream_t s, const void* Us, const void* U0,
-                        void* Uout, const double* dtp, double* partials, const BlocksDev& B) {
+void launch_block_stage(int dtype, int rank, int cls, bool stage2, int wgs, hipStream_t s, const void* Us,
+                        const void* U0, void* Uout, const double* dtp, double* partials, const BlocksDev& B) {
   if (wgs <= 0) return;
   if (dtype == 0) {
-    if (rank == 3) block_stage_t<double, 3, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
-    else if (rank == 2) block_stage_t<double, 2, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
-    else block_stage_t<double, 1, false>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    if (rank == 3) block_stage_t<double, 3, false>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else if (rank == 2) block_stage_t<double, 2, false>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else block_stage_t<double, 1, false>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
   } else {
-    if (rank == 3) block_stage_t<float, 3, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
-    else if (rank == 2) block_stage_t<float, 2, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
-    else block_stage_t<float, 1, true>(stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    if (rank == 3) block_stage_t<float, 3, true>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else if (rank == 2) block_stage_t<float, 2, true>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
+    else block_stage_t<float, 1, true>(cls, stage2, wgs, s, Us, U0, Uout, dtp, partials, B);
   }
 }
 
