@@ -385,8 +385,11 @@ int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_blo
  * mesh outgrows twice an earlier one. With self-gravity on, the operator is
  * rebuilt for the new mesh and φ is carried over by the same injection (the next
  * solve's warm start). Invalid tables leave the context unchanged; a HIP error
- * after that leaves it fit only for vh_blocks_destroy. Returns after the new
- * state is in place. */
+ * after that leaves it fit only for vh_blocks_destroy. The upload and the remap
+ * are queued on the context's stream behind the work already there and the call
+ * returns without waiting for them (it waits when the state buffers grow or
+ * self-gravity is on): later calls on the context are ordered after them, and
+ * vh_blocks_sync waits for them. */
 int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const int32_t* dims,
                      const int64_t* origin, const int32_t* bc, const int32_t* src_block);
 

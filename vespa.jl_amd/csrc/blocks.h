@@ -448,34 +448,34 @@ __global__ void __launch_bounds__(CTX ? CTX * CTY : BNT, sizeof(T) == 4 ? VH_BLO
           for (int d = 0; d < RANK; ++d)
             if (kind[d] == 2) acc_add(acc, flux(2 * d), -ar[d]);
         } else {
-          const LinkCellDev* __restrict__ R = B.linkcells + lc;
-          const int npre = R->npre, ngrp = R->ngrp;
+          const LinkCellDev R = B.linkcells[lc];
+          const int npre = lc_npre(R), ngrp = lc_ngrp(R);
           // faces emitted by cells of preceding blocks
           for (int q = 0; q < npre; ++q) {
-            const int sl = R->grp[q];
-            add_link_group(acc, B, LF, R->start[sl], R->count[sl], c);
+            const int sl = lc_grp(R, q);
+            add_link_group(acc, B, LF, lc_start(R, sl), lc_count(R, sl), c);
           }
           // interior lo faces (partner earlier in the block), z, y, x
 #pragma unroll
           for (int d = RANK - 1; d >= 0; --d)
-            if (R->type[2 * d] == 0 && kind[d] == 0) acc_add(acc, flux(2 * d), -ar[d]);
+            if (lc_type(R, 2 * d) == 0 && kind[d] == 0) acc_add(acc, flux(2 * d), -ar[d]);
           // per axis: the hi face, then a boundary lo face or an own coarse-side lo link
 #pragma unroll
           for (int d = 0; d < RANK; ++d) {
-            const int th = R->type[2 * d + 1], tlo = R->type[2 * d];
+            const int th = lc_type(R, 2 * d + 1), tlo = lc_type(R, 2 * d);
             if (th == 0) acc_add(acc, flux(2 * d + 1), ar[d]);
-            else if (th == 1) add_link_group(acc, B, LF, R->start[2 * d + 1], 1, c);
+            else if (th == 1) add_link_group(acc, B, LF, lc_start(R, 2 * d + 1), 1, c);
             if (tlo == 0 && kind[d] == 1) acc_add(acc, flux(2 * d), -ar[d]);
-            else if (tlo == 1) add_link_group(acc, B, LF, R->start[2 * d], 1, c);
+            else if (tlo == 1) add_link_group(acc, B, LF, lc_start(R, 2 * d), 1, c);
           }
           // lo faces wrapped by a periodic boundary (partner later in the block)
 #pragma unroll
           for (int d = 0; d < RANK; ++d)
-            if (R->type[2 * d] == 0 && kind[d] == 2) acc_add(acc, flux(2 * d), -ar[d]);
+            if (lc_type(R, 2 * d) == 0 && kind[d] == 2) acc_add(acc, flux(2 * d), -ar[d]);
           // faces emitted by cells of following blocks
           for (int q = npre; q < ngrp; ++q) {
-            const int sl = R->grp[q];
-            add_link_group(acc, B, LF, R->start[sl], R->count[sl], c);
+            const int sl = lc_grp(R, q);
+            add_link_group(acc, B, LF, lc_start(R, sl), lc_count(R, sl), c);
           }
         }
 

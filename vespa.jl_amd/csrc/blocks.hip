@@ -342,17 +342,23 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
     long long emax[3] = {1, 1, 1};
     for (int d = 0; d < rank; ++d) {
       const int e1 = d == 0 ? 1 : 0;
-      at[d][0].resize(nb);
-      at[d][1].resize(nb);
-      for (int c = 0; c < nb; ++c) {
-        const long long o1 = e1 < rank ? T.origin[3 * c + e1] : 0;
-        at[d][0][c] = {T.origin[3 * c + d], o1, c};
-        at[d][1][c] = {T.origin[3 * c + d] + (long long)T.dims[3 * c + d] * T.span(c), o1, c};
+      for (int c = 0; c < nb; ++c)
         if (e1 < rank) emax[d] = std::max(emax[d], (long long)T.dims[3 * c + e1] * T.span(c));
-      }
-      std::sort(at[d][0].begin(), at[d][0].end());
-      std::sort(at[d][1].begin(), at[d][1].end());
     }
+    // the 2·rank key lists are filled and sorted side by side
+    HostPool::run(nb < 256 ? 1 : 2 * rank, [&](int q) {
+      for (int j = (nb < 256 ? 0 : q); j < 2 * rank; j += (nb < 256 ? 1 : 2 * rank)) {
+        const int d = j >> 1, side = j & 1, e1 = d == 0 ? 1 : 0;
+        std::vector<Key>& v = at[d][side];
+        v.resize(nb);
+        for (int c = 0; c < nb; ++c) {
+          const long long o1 = e1 < rank ? T.origin[3 * c + e1] : 0;
+          const long long pl = T.origin[3 * c + d] + (side ? (long long)T.dims[3 * c + d] * T.span(c) : 0);
+          v[c] = {pl, o1, c};
+        }
+        std::sort(v.begin(), v.end());
+      }
+    });
     const int ncc = nb < 256 ? 1 : std::min(4 * nthr, nb);
     std::vector<std::vector<int>> cids(ncc);          // candidates of the chunk's sides in (block, slot) order
     std::vector<int> counts(6 * (size_t)nb, 0);
@@ -644,10 +650,10 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
                   bgrp.push_back(-1);
                 }
               }
-              brec[r].type[slot] = type;
+              brec[r].tc |= (unsigned)type << (2 * slot);
               bslots[6 * r + slot] = fl;
               bgrp[6 * r + slot] = emitter_block;
-              if (kind == NB_FINER) brec[r].finer |= 1 << slot;
+              if (kind == NB_FINER) brec[r].gr |= 1u << (24 + slot);
               for (int i = 1; i <= fl[0]; ++i) {   // each listed face must touch this cell
                 const LinkFaceDev& F = H.faces[fl[i]];
                 if (F.iL != g && F.iR != g)
@@ -661,25 +667,25 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
         LinkCellDev R = brec[r];
         std::pair<int, int> grp[6];   // (emitting block, slot), at most one per slot
         int ng = 0;
+        R.start = (int)o.slot_faces.size();   // chunk-local until joined
         for (int s = 0; s < 6; ++s) {
           const SlotList& sl = bslots[6 * r + s];
-          R.start[s] = (int)o.slot_faces.size();   // chunk-local until joined
-          R.count[s] = sl[0];
+          R.tc |= (unsigned)sl[0] << (12 + 3 * s);
           for (int i = 1; i <= sl[0]; ++i) o.slot_faces.push_back(sl[i]);
-          if (R.type[s] == 2) grp[ng++] = {bgrp[6 * r + s], s};
+          if (lc_type(R, s) == 2) grp[ng++] = {bgrp[6 * r + s], s};
         }
         std::sort(grp, grp + ng);
-        R.ngrp = ng;
-        R.npre = 0;
-        for (int q = 0; q < 6; ++q) R.grp[q] = -1;
-        for (int q = 0; q < ng; ++q) {
-          if (q > 0 && grp[q].first == grp[q - 1].first)
+        int npre = 0;
+        unsigned gbits = 0;
+        for (int q = 0; q < 6; ++q) {
+          if (q > 0 && q < ng && grp[q].first == grp[q - 1].first)
             return (void)o.set_error(fail(-1, "a cell touches block %s%lld through two faces", "", grp[q].first));
-          R.grp[q] = grp[q].second;
-          if (grp[q].first < b) R.npre++;
+          gbits |= (q < ng ? (unsigned)grp[q].second : 7u) << (3 * q);
+          if (q < ng && grp[q].first < b) npre++;
         }
+        R.gr |= gbits | ((unsigned)npre << 18) | ((unsigned)ng << 21);
         o.recs.push_back(R);
-        if (R.finer) bfiner.push_back({bcell[r], r0 + (int)r});
+        if (lc_finer(R)) bfiner.push_back({bcell[r], r0 + (int)r});
       }
       // link-cell maps of the block's faces (every record sits on one of them)
       bk.haslinks = 0;
@@ -737,7 +743,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
     const int rb = (int)rec_base[c], sb = (int)sf_base[c], lb = (int)lc_base[c];
     for (size_t r = 0; r < o.recs.size(); ++r) {
       LinkCellDev R = o.recs[r];
-      for (int s = 0; s < 6; ++s) R.start[s] += sb;
+      R.start += sb;
       H.linkcells[rb + r] = R;
     }
     std::copy(o.slot_faces.begin(), o.slot_faces.end(), H.slot_faces.begin() + sb);
@@ -934,11 +940,13 @@ __global__ void __launch_bounds__(256) capture_kernel(const int* __restrict__ ce
   const T s = T(sd);
   T acc[5];
   for (int f = 0; f < 5; ++f) acc[f] = delta[5 * (long long)e + f];
-  for (int q = 0; q < R.ngrp; ++q) {
-    const int slot = R.grp[q];
-    if (!((R.finer >> slot) & 1)) continue;
-    for (int i = 0; i < R.count[slot]; ++i) {
-      const int fi = B.slot_faces[R.start[slot] + i];
+  const int finer = lc_finer(R), ngrp = lc_ngrp(R);
+  for (int q = 0; q < ngrp; ++q) {
+    const int slot = lc_grp(R, q);
+    if (!((finer >> slot) & 1)) continue;
+    const int st = lc_start(R, slot), cnt = lc_count(R, slot);
+    for (int i = 0; i < cnt; ++i) {
+      const int fi = B.slot_faces[st + i];
       const LinkFaceDev L = B.faces[fi];
       const T aT = T(L.area);
       const T* Fv = F + 5 * (long long)fi;
@@ -1061,6 +1069,12 @@ struct vh_blocks {
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_np = -1;
+  // A regrid does not wait for the stream: the graph it replaces is kept here
+  // until the next regrid has waited for stage_ev (recorded after the upload
+  // that followed the graph's last launch), and stage_ev guards the pinned
+  // staging against a copy still reading it.
+  hipGraphExec_t retired = nullptr;
+  hipEvent_t stage_ev = nullptr;
   struct BlockGrav* grav = nullptr;   // self-gravity (vh_blocks_enable_gravity)
 };
 
@@ -1341,6 +1355,7 @@ static int alloc_partials(vh_blocks* m) {
   need = std::max(need, m->H.block_tile_start[m->T.nb]);
   need = std::max(need, list_wgs(m, 0));
   if (need > m->npartials) {
+    VH_HIP(hipStreamSynchronize(m->stream));   // queued stages may still write the old partials
     if (m->partials) VH_HIP(hipFree(m->partials));
     m->partials = nullptr;
     VH_HIP(hipMalloc(&m->partials, sizeof(double) * need * 5));
@@ -1377,7 +1392,8 @@ static void free_tables(vh_blocks* m) {
 // need when too small) through the pinned staging buffer in one copy, zeroes the flux-
 // register accumulators and points the context's tables there. extra/extra_bytes
 // (optional) is packed behind the tables; *extra_dev receives its address.
-// Stream-ordered: the caller synchronises before the staging is reused.
+// Stream-ordered; the staging is rewritten only after stage_ev, recorded behind
+// the last copy out of it, has completed.
 static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, const void* extra = nullptr,
                          size_t extra_bytes = 0, void** extra_dev = nullptr) {
   struct Part {
@@ -1418,6 +1434,7 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     o_delta[l] = add(nullptr, m->esize * 5 * std::max<size_t>(1, H.regs[l].cell.size()));
   const size_t zero_to = total;
   if (total > m->arena_cap[a]) {
+    VH_HIP(hipStreamSynchronize(m->stream));   // queued work may still read the old slot
     (void)hipFree(m->arena[a]);
     m->arena[a] = nullptr;
     m->arena_cap[a] = 0;
@@ -1425,6 +1442,7 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     VH_HIP(hipMalloc(&m->arena[a], cap));
     m->arena_cap[a] = cap;
   }
+  if (m->stage_ev) VH_HIP(hipEventSynchronize(m->stage_ev));   // the last copy out of the staging is done
   if (staged > m->stage_cap) {
     if (m->stage) (void)hipHostFree(m->stage);
     m->stage = nullptr;
@@ -1438,6 +1456,8 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     if (p.src && p.bytes && p.off < staged) memcpy(st + p.off, p.src, p.bytes);
   char* base = (char*)m->arena[a];
   VH_HIP(hipMemcpyAsync(base, st, staged, hipMemcpyHostToDevice, m->stream));
+  if (!m->stage_ev) VH_HIP(hipEventCreateWithFlags(&m->stage_ev, hipEventDisableTiming));
+  VH_HIP(hipEventRecord(m->stage_ev, m->stream));
   const long long nz = (long long)((zero_to - zero_from) / sizeof(float));
   if (nz > 0)
     hipLaunchKernelGGL(zero_kernel<float>, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, m->stream,
@@ -1645,7 +1665,26 @@ int vh_blocks_table_digest(int32_t rank, int32_t nblocks, int32_t maxlevel, cons
   digest_vec(h, H.lcmap);
   digest_vec(h, H.faces);
   digest_vec(h, H.slot_faces);
-  digest_vec(h, H.linkcells);
+  {
+    // the records in their unpacked form (type, start, count and grp per slot,
+    // npre, ngrp, finer, 0), the layout the digests were first recorded with
+    std::vector<std::array<int, 28>> wide(H.linkcells.size());
+    for (size_t i = 0; i < wide.size(); ++i) {
+      const LinkCellDev& R = H.linkcells[i];
+      std::array<int, 28>& w = wide[i];
+      for (int s = 0; s < 6; ++s) {
+        w[s] = lc_type(R, s);
+        w[6 + s] = lc_start(R, s);
+        w[12 + s] = lc_count(R, s);
+        w[18 + s] = lc_grp(R, s);
+      }
+      w[24] = lc_npre(R);
+      w[25] = lc_ngrp(R);
+      w[26] = lc_finer(R);
+      w[27] = 0;
+    }
+    digest_vec(h, wide);
+  }
   for (const RegList& r : H.regs) {
     digest_vec(h, r.cell);
     digest_vec(h, r.lc);
@@ -1705,6 +1744,8 @@ int vh_blocks_destroy(vh_blocks* m) {
   if (!m) return 0;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   drop_graph(m);
+  if (m->retired) (void)hipGraphExecDestroy(m->retired);
+  if (m->stage_ev) (void)hipEventDestroy(m->stage_ev);
   free_grav(m);
   free_tables(m);
   (void)hipFree(m->U);
@@ -2118,8 +2159,15 @@ int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const 
   if (!rc) rc = build_remap(N, m->T, src_block, map);
   if (rc) return rc;
   VH_HIP(hipSetDevice(m->device));
-  VH_HIP(hipStreamSynchronize(m->stream));   // the staging and the spare arena are free
-  drop_graph(m);                             // captured steps hold the old tables
+  // No wait for the stream: the new tables go to the spare arena, which the
+  // steps since the last regrid do not read, and everything below is enqueued
+  // behind them. The graph of the old tables is retired (destroyed once the
+  // next regrid has seen the upload after its last launch complete).
+  if (m->stage_ev) VH_HIP(hipEventSynchronize(m->stage_ev));
+  if (m->retired) (void)hipGraphExecDestroy(m->retired);
+  m->retired = m->graph;
+  m->graph = nullptr;
+  m->graph_np = -1;
   const long long n_old = m->T.off[m->T.nb], n_new = N.off[N.nb];
   const BlockDev* old_blocks = m->d_blocks;
   const int old_arena = m->arena_cur;
@@ -2153,7 +2201,8 @@ int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const 
     regrid_remap(m->stream, m->dtype, N.nb, m->d_blocks, old_blocks, (const RemapDev*)dmap, m->grav->B.phi, n_old,
                  phi_new, n_new, 1);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  // the old state is freed (growth) and φ carried over (gravity) only after the remap
+  if (e == hipSuccess && (fresh || grav)) e = hipStreamSynchronize(m->stream);
   if (e != hipSuccess) {
     if (fresh) (void)hipFree(dst);
     (void)hipFree(phi_new);

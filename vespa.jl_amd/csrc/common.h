@@ -90,21 +90,37 @@ struct LinkFaceDev {
   double area;
 };
 
-// Gather record of a cell that touches link faces.
-// type[s]: 0 flux computed by the tile (or none), 1 one link face emitted by
+// Gather record of a cell that touches link faces (16 bytes, packed; read it
+// through the accessors below).
+// type(s): 0 flux computed by the tile (or none), 1 one link face emitted by
 // this cell at its own walk position, 2 a group of link faces emitted by cells
-// of another block. start/count: the slot's faces in slot_faces (walk order).
-// grp[0..ngrp): the type-2 slots in increasing emitting-block order; the first
-// npre come from blocks that precede this one. finer: bit s set when slot s is
-// a group of finer cells (the coarse side of a flux register).
+// of another block. start(s)/count(s): the slot's faces in slot_faces (walk
+// order); the slots' lists follow each other from `start`, so start(s) is
+// start + the counts of the slots before s. grp(0..ngrp): the type-2 slots in
+// increasing emitting-block order; the first npre come from blocks that precede
+// this one. finer: bit s set when slot s is a group of finer cells (the coarse
+// side of a flux register).
+//   tc: bits 2s..2s+1 type(s), bits 12+3s..14+3s count(s) (at most 4)
+//   gr: bits 3q..3q+2 grp(q) (7: none), bits 18-20 npre, 21-23 ngrp, 24-29 finer
 struct LinkCellDev {
-  int type[6];
-  int start[6], count[6];
-  int grp[6];
-  int npre, ngrp;
-  int finer;
+  int start;
+  unsigned tc, gr;
   int pad;
 };
+__host__ __device__ inline int lc_type(const LinkCellDev& R, int s) { return (int)((R.tc >> (2 * s)) & 3u); }
+__host__ __device__ inline int lc_count(const LinkCellDev& R, int s) { return (int)((R.tc >> (12 + 3 * s)) & 7u); }
+__host__ __device__ inline int lc_start(const LinkCellDev& R, int s) {
+  int o = R.start;
+  for (int q = 0; q < s; ++q) o += lc_count(R, q);
+  return o;
+}
+__host__ __device__ inline int lc_grp(const LinkCellDev& R, int q) {
+  const int v = (int)((R.gr >> (3 * q)) & 7u);
+  return v == 7 ? -1 : v;
+}
+__host__ __device__ inline int lc_npre(const LinkCellDev& R) { return (int)((R.gr >> 18) & 7u); }
+__host__ __device__ inline int lc_ngrp(const LinkCellDev& R) { return (int)((R.gr >> 21) & 7u); }
+__host__ __device__ inline int lc_finer(const LinkCellDev& R) { return (int)((R.gr >> 24) & 63u); }
 
 // Everything a block kernel needs, passed by value.
 struct BlocksDev {
