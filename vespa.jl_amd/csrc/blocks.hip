@@ -1451,9 +1451,20 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     VH_HIP(hipHostMalloc(&m->stage, cap, 0));
     m->stage_cap = cap;
   }
+  // the tables are packed into the staging by the builder's threads, in pieces
+  // of at most 256 KiB
   char* st = (char*)m->stage;
+  struct Piece {
+    const char* src;
+    char* dst;
+    size_t bytes;
+  };
+  std::vector<Piece> pieces;
   for (const Part& p : parts)
-    if (p.src && p.bytes && p.off < staged) memcpy(st + p.off, p.src, p.bytes);
+    if (p.src && p.bytes && p.off < staged)
+      for (size_t o = 0; o < p.bytes; o += (size_t)1 << 18)
+        pieces.push_back({(const char*)p.src + o, st + p.off + o, std::min(p.bytes - o, (size_t)1 << 18)});
+  HostPool::run((int)pieces.size(), [&](int i) { memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes); });
   char* base = (char*)m->arena[a];
   VH_HIP(hipMemcpyAsync(base, st, staged, hipMemcpyHostToDevice, m->stream));
   if (!m->stage_ev) VH_HIP(hipEventCreateWithFlags(&m->stage_ev, hipEventDisableTiming));
