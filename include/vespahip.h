@@ -381,8 +381,9 @@ int vh_blocks_remap(vh_blocks* dst, const vh_blocks* src, const int32_t* src_blo
  * block b). The stream, dtype, arithmetic profile, launch shape and clock stay;
  * the tables are built by the host builder's threads and uploaded in one copy
  * into a second device arena, and the state buffers are reused when large enough
- * (allocated at twice the need otherwise), so a regrid allocates only when the
- * mesh outgrows twice an earlier one. With self-gravity on, the operator is
+ * (allocated at twice the need, at least 4 MiB, otherwise; the arenas and the
+ * pinned staging likewise, at least 8 and 4 MiB), so a regrid allocates only
+ * when the mesh outgrows twice an earlier one. With self-gravity on, the operator is
  * rebuilt for the new mesh and φ is carried over by the same injection (the next
  * solve's warm start). Invalid tables leave the context unchanged; a HIP error
  * after that leaves it fit only for vh_blocks_destroy. The upload and the remap

@@ -1438,7 +1438,9 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     (void)hipFree(m->arena[a]);
     m->arena[a] = nullptr;
     m->arena_cap[a] = 0;
-    const size_t cap = 2 * total;   // regrids grow the mesh: keep room for the next ones
+    // regrids grow the mesh: keep room for the next ones (at least 8 MiB, so a
+    // mesh growing from a few blocks does not reallocate at every regrid)
+    const size_t cap = std::max<size_t>(2 * total, (size_t)8 << 20);
     VH_HIP(hipMalloc(&m->arena[a], cap));
     m->arena_cap[a] = cap;
   }
@@ -1447,7 +1449,7 @@ static int upload_tables(vh_blocks* m, const HostTables& H, int nblocks, int a, 
     if (m->stage) (void)hipHostFree(m->stage);
     m->stage = nullptr;
     m->stage_cap = 0;
-    const size_t cap = 2 * staged;
+    const size_t cap = std::max<size_t>(2 * staged, (size_t)4 << 20);
     VH_HIP(hipHostMalloc(&m->stage, cap, 0));
     m->stage_cap = cap;
   }
@@ -2202,7 +2204,7 @@ int vh_blocks_regrid(vh_blocks* m, int32_t nblocks, const int32_t* level, const 
   void* dst = m->U1;
   bool fresh = false;
   if (need > m->state_cap) {
-    const size_t cap = 2 * need;
+    const size_t cap = std::max<size_t>(2 * need, (size_t)4 << 20);
     VH_HIP(hipMalloc(&dst, cap));
     fresh = true;
     m->state_cap = cap;

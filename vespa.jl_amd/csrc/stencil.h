@@ -42,9 +42,10 @@
 #ifndef VH_MIN_WAVES_F32
 #define VH_MIN_WAVES_F32 4
 #endif
-// 1: in the periodic form, issue the loads of the cell's own state at the start
-// of the plane (more latency hidden, 10-20 more registers live across the
-// plane, which the boundary forms would spill); 0: load it in the update phase.
+// 1: in stage 1 of the periodic Float64 form, issue the loads of the cell's own
+// state at the start of the plane (more latency hidden, 10 more registers live
+// across the plane, which the boundary forms and stage 2 need elsewhere; the
+// Float32 form measured 3% slower with it); 0: load it in the update phase.
 #ifndef VH_EARLY_OWN
 #define VH_EARLY_OWN 1
 #endif
@@ -395,14 +396,20 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     }
   }
 
+  // Periodic form: the z face of the plane is computed in P0 behind the halo
+  // loads (stage 1 -3 to -8%, stage 2 -1.5% at 256³ FP64); the boundary forms
+  // keep it in PB, where they have the registers for it.
+  constexpr bool ZEARLY = PER && !PREFETCH && NHX + NHY <= NT;
   for (int k = kz0; k < kz1; ++k) {
     const long long c = (long long)k * plane + (long long)gy * nx + gx;
+    Cons<T> fzhi;   // z face at k+½ of the lane's column
+    Prim<T> sz1;    // its z slope at k+1
     // Early loads: the next z plane of the column and this cell's own state.
     Cons<T> unext, uown, u0own;
     if constexpr (RANK == 3) {
       if (k + 1 < kz1) unext = load_cons(Us, n, column_src<PER>(G, col, k + 3));
     }
-    constexpr bool EARLY = VH_EARLY_OWN && PER;
+    constexpr bool EARLY = VH_EARLY_OWN && PER && !STAGE2 && sizeof(T) == 8;
     if constexpr (EARLY) {
       if (active) {
         uown = load_cons(Us, n, c);
@@ -441,10 +448,34 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
         uhalo = load_cons(Us, n, halo_src(lx, ly, k + 1));
       }
     } else {
-      for (int h = lt; h < NHX + NHY; h += NT) {
-        int lx, ly;
-        halo_pos(h, lx, ly);
-        stage_halo(lx, ly, k, load_cons(Us, n, halo_src(lx, ly, k)));
+      if constexpr (ZEARLY) {
+        // At most one halo item per lane: its loads are issued first and the
+        // z face of the lane's column (registers only) is computed while they
+        // are in flight.
+        int lx = 0, ly = 0;
+        Cons<T> uh;
+        const bool hv = lt < NHX + NHY;
+        if (hv) {
+          halo_pos(lt, lx, ly);
+          uh = load_cons(Us, n, halo_src(lx, ly, k));
+        }
+        if constexpr (RANK == 3) {
+          sz1 = slope(wz0, wz1, wz2);
+          const int bz = G.bc[4];
+          const Prim<T> L = face_value(wz0, sz0, T(0.5));
+          if (!PER && bz != BC_PERIODIC && k == nz - 1) {
+            fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
+            if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
+          } else
+            fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
+        }
+        if (hv) stage_halo(lx, ly, k, uh);
+      } else {
+        for (int h = lt; h < NHX + NHY; h += NT) {
+          int lx, ly;
+          halo_pos(h, lx, ly);
+          stage_halo(lx, ly, k, load_cons(Us, n, halo_src(lx, ly, k)));
+        }
       }
     }
     __syncthreads();
@@ -537,18 +568,18 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
       }
     }
 
-    // z face at k+½ for the lane's column.
-    Cons<T> fzhi;
-    Prim<T> sz1;
-    if constexpr (RANK == 3) {
-      sz1 = slope(wz0, wz1, wz2);
-      const int bz = G.bc[4];
-      const Prim<T> L = face_value(wz0, sz0, T(0.5));
-      if (!PER && bz != BC_PERIODIC && k == nz - 1) {
-        fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
-        if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
-      } else
-        fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
+    // z face at k+½ for the lane's column (here unless computed in P0).
+    if constexpr (!ZEARLY) {
+      if constexpr (RANK == 3) {
+        sz1 = slope(wz0, wz1, wz2);
+        const int bz = G.bc[4];
+        const Prim<T> L = face_value(wz0, sz0, T(0.5));
+        if (!PER && bz != BC_PERIODIC && k == nz - 1) {
+          fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
+          if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
+        } else
+          fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
+      }
     }
     __syncthreads();
 
