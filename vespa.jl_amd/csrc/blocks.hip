@@ -28,6 +28,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -177,6 +178,7 @@ struct RegList {
   std::vector<int> cell, lc, block;
 };
 
+struct BuildScratch;
 struct HostTables {
   std::vector<BlockDev> blocks;
   std::vector<int> repmap, lcmap;
@@ -195,6 +197,9 @@ struct HostTables {
   std::vector<int> cfl_blocks;
   std::vector<long long> cfl_prefix;
   std::vector<int> cfl_list_off;         // into cfl_blocks; prefix list l starts at cfl_list_off[l] + l
+  // the builder's working buffers, kept with the tables so that a rebuild
+  // (every regrid) reuses their memory instead of allocating and faulting it in
+  std::shared_ptr<BuildScratch> scratch;
 };
 
 static int balanced(int n, int maxT) {
@@ -308,6 +313,27 @@ struct ChunkOut {
   std::vector<LinkCellDev> recs;
   std::vector<int> slot_faces, lcmap;
   std::vector<RegList> regs;   // per level
+  // working buffers of the record phase
+  std::vector<int> loc;
+  std::vector<long long> bcell;
+  std::vector<LinkCellDev> brec;
+  std::vector<std::array<int, 5>> bslots;
+  std::vector<int> bgrp;
+  std::vector<std::pair<long long, int>> bfiner;
+  void reset(int nlev) {
+    rc = 0;
+    err.clear();
+    faces.clear();
+    recs.clear();
+    slot_faces.clear();
+    lcmap.clear();
+    if ((int)regs.size() < nlev) regs.resize(nlev);
+    for (RegList& r : regs) {
+      r.cell.clear();
+      r.lc.clear();
+      r.block.clear();
+    }
+  }
   int set_error(int code) {
     if (!rc) {
       rc = code;
@@ -315,6 +341,25 @@ struct ChunkOut {
     }
     return code;
   }
+};
+
+// Candidate-search key: a block's lo (or hi) plane along an axis and its lo
+// corner along the first other axis.
+struct CandKey {
+  long long plane, e1lo;
+  int b;
+  bool operator<(const CandKey& o) const {
+    return plane != o.plane ? plane < o.plane : (e1lo != o.e1lo ? e1lo < o.e1lo : b < o.b);
+  }
+};
+
+struct BuildScratch {
+  std::vector<CandKey> at[3][2];
+  std::vector<std::vector<int>> cids;
+  std::vector<int> counts, perr, fmap, seen, cls, idx;
+  std::vector<ChunkOut> ch;
+  std::vector<TileDev> base;
+  std::vector<long long> cost;
 };
 
 // Builds every table of a mesh. Returns 0 or a negative code (message set).
@@ -330,15 +375,11 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   // Blocks are searched in parallel chunks; each side lists its candidates in
   // ascending block order.
   const int nthr = nthreads > 0 ? nthreads : HostPool::threads();
+  if (!H.scratch) H.scratch = std::make_shared<BuildScratch>();
+  BuildScratch& X = *H.scratch;
   {
-    struct Key {
-      long long plane, e1lo;
-      int b;
-      bool operator<(const Key& o) const {
-        return plane != o.plane ? plane < o.plane : (e1lo != o.e1lo ? e1lo < o.e1lo : b < o.b);
-      }
-    };
-    std::vector<Key> at[3][2];   // [axis][0: lo planes, 1: hi planes]
+    using Key = CandKey;
+    auto& at = X.at;   // [axis][0: lo planes, 1: hi planes]
     long long emax[3] = {1, 1, 1};
     for (int d = 0; d < rank; ++d) {
       const int e1 = d == 0 ? 1 : 0;
@@ -360,9 +401,13 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
       }
     });
     const int ncc = nb < 256 ? 1 : std::min(4 * nthr, nb);
-    std::vector<std::vector<int>> cids(ncc);          // candidates of the chunk's sides in (block, slot) order
-    std::vector<int> counts(6 * (size_t)nb, 0);
-    std::vector<int> perr(ncc, -1);                    // block with a periodic-axis contact
+    auto& cids = X.cids;                               // candidates of the chunk's sides in (block, slot) order
+    if ((int)cids.size() < ncc) cids.resize(ncc);
+    for (int q = 0; q < ncc; ++q) cids[q].clear();
+    auto& counts = X.counts;
+    counts.assign(6 * (size_t)nb, 0);
+    auto& perr = X.perr;
+    perr.assign(ncc, -1);                              // block with a periodic-axis contact
     HostPool::run(ncc, [&](int q) {
       const int b0 = (int)((long long)nb * q / ncc), b1 = (int)((long long)nb * (q + 1) / ncc);
       int tmp[64];
@@ -455,12 +500,16 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   }
   H.repmap.assign((size_t)nplane, -1);
   // fmap runs beside repmap: the link face a face cell emits through that side, or -1
-  std::vector<int> fmap((size_t)nplane, -1);
+  std::vector<int>& fmap = X.fmap;
+  fmap.assign((size_t)nplane, -1);
 
   // chunks: contiguous block ranges of similar face-plane work
   int nchunk = std::min(4 * nthr, nb);
   if (nb < 64 || nplane < 4096 || nthreads == 1) nchunk = 1;
-  std::vector<ChunkOut> ch(nchunk);
+  // chunk outputs keep their capacity between builds: the vector only grows
+  std::vector<ChunkOut>& ch = X.ch;
+  if ((int)ch.size() < nchunk) ch.resize(nchunk);
+  for (int c = 0; c < nchunk; ++c) ch[c].reset(T.maxlevel + 1);
   {
     long long acc = 0;
     int c = 0;
@@ -477,13 +526,12 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
       }
     }
     ch[c].b1 = nb;
-    ch.resize(c + 1);
     nchunk = c + 1;
   }
   auto for_chunks = [&](const std::function<void(ChunkOut&)>& f) {
     HostPool::run(nchunk, [&](int i) { f(ch[i]); });
-    for (ChunkOut& o : ch)
-      if (o.rc) return fail(o.rc, "%s", o.err.c_str());
+    for (int c = 0; c < nchunk; ++c)
+      if (ch[c].rc) return fail(ch[c].rc, "%s", ch[c].err.c_str());
     return 0;
   };
 
@@ -582,13 +630,13 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   rc = for_chunks([&](ChunkOut& o) {
     long long cmax = 1;
     for (int b = o.b0; b < o.b1; ++b) cmax = std::max(cmax, T.off[b + 1] - T.off[b]);
-    std::vector<int> loc((size_t)cmax, -1);   // cell of the current block -> its record (block-local)
-    std::vector<long long> bcell;             // block-local record -> cell
-    std::vector<LinkCellDev> brec;
-    std::vector<SlotList> bslots;             // block-local record * 6 + slot
-    std::vector<int> bgrp;                    // block-local record * 6 + slot -> emitting block of a type-2 slot
-    std::vector<std::pair<long long, int>> bfiner;
-    o.regs.assign(T.maxlevel + 1, RegList{});
+    std::vector<int>& loc = o.loc;            // cell of the current block -> its record (block-local)
+    loc.assign((size_t)cmax, -1);
+    std::vector<long long>& bcell = o.bcell;  // block-local record -> cell
+    std::vector<LinkCellDev>& brec = o.brec;
+    std::vector<SlotList>& bslots = o.bslots; // block-local record * 6 + slot
+    std::vector<int>& bgrp = o.bgrp;          // block-local record * 6 + slot -> emitting block of a type-2 slot
+    std::vector<std::pair<long long, int>>& bfiner = o.bfiner;
     for (int b = o.b0; b < o.b1 && !o.rc; ++b) {
       BlockDev& bk = H.blocks[b];
       bcell.clear();
@@ -752,7 +800,12 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
       for (int s = 0; s < 6; ++s)
         if (H.blocks[b].lcoff[s] >= 0) H.blocks[b].lcoff[s] += lb;
   });
-  H.regs.assign(T.maxlevel + 1, RegList{});
+  if ((int)H.regs.size() != T.maxlevel + 1) H.regs.resize(T.maxlevel + 1);
+  for (RegList& r : H.regs) {
+    r.cell.clear();
+    r.lc.clear();
+    r.block.clear();
+  }
   for (int c = 0; c < nchunk; ++c)
     for (int l = 0; l <= T.maxlevel; ++l) {
       const RegList& s = ch[c].regs[l];
@@ -764,7 +817,8 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   // every link face must be gathered by both of its cells (fails when a fine
   // block is not aligned to the coarse cells it touches)
   {
-    std::vector<int> seen(H.faces.size(), 0);
+    std::vector<int>& seen = X.seen;
+    seen.assign(H.faces.size(), 0);
     for (int f : H.slot_faces) seen[f]++;
     for (size_t f = 0; f < seen.size(); ++f)
       if (seen[f] != 2)
@@ -773,8 +827,10 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   }
 
   // ---- tiles ----
-  std::vector<TileDev> base;
-  std::vector<long long> cost;
+  std::vector<TileDev>& base = X.base;
+  std::vector<long long>& cost = X.cost;
+  base.clear();
+  cost.clear();
   H.block_tile_start.assign(nb + 1, 0);
   for (int b = 0; b < nb; ++b) {
     const BlockDev& bk = H.blocks[b];
@@ -815,10 +871,12 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   const int nt = (int)base.size();
   // A list holds its tiles grouped by shape class (one launch per class), each
   // group in decreasing cost.
-  std::vector<int> cls(nt);
+  std::vector<int>& cls = X.cls;
+  cls.resize(nt);
   for (int i = 0; i < nt; ++i) cls[i] = block_shape_class(rank, base[i].tx, base[i].ty);
-  auto sorted_list = [&](int lev) {
-    std::vector<int> idx;
+  auto sorted_list = [&](int lev) -> const std::vector<int>& {
+    std::vector<int>& idx = X.idx;
+    idx.clear();
     for (int i = 0; i < nt; ++i)
       if (lev < 0 || T.level[base[i].block] == lev) idx.push_back(i);
     std::stable_sort(idx.begin(), idx.end(),
@@ -830,7 +888,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   H.tile_class_off.clear();
   H.tile_list_off.push_back(nt);   // list 0: all, starts after the block-order copy
   for (int l = -1; l <= T.maxlevel; ++l) {
-    const std::vector<int> idx = sorted_list(l);
+    const std::vector<int>& idx = sorted_list(l);
     for (int c = 0, q = 0; c <= kBlockClasses; ++c) {
       while (c < kBlockClasses && q < (int)idx.size() && cls[idx[q]] < c) ++q;
       H.tile_class_off.push_back(c < kBlockClasses ? q : (int)idx.size());
