@@ -399,9 +399,10 @@ def sedov_leg(V, args):
     import numpy as np
     from vespa_jl_amd import amr
     out = {}
-    # Untimed warm-up: a short run of the same problem. It also waits out the
-    # teardown of the previous leg (freeing its ~1 GB of device buffers keeps the
-    # host busy for ~0.1 s, which slowed the first regrids of the next run 5x).
+    # Untimed warm-up: a short run of the same problem. Measured right after the
+    # adjoint leg, the first ~0.1 s of the next run went 5x slower on the host
+    # (table builds and the pure-Python tree edit alike), which the warm-up
+    # absorbs.
     amr.run_sedov(tfinal=0.0125, dtype=np.float64, mode=args.mode)
     for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
         t0 = time.perf_counter()
