@@ -356,7 +356,7 @@ struct CandKey {
 struct BuildScratch {
   std::vector<CandKey> at[3][2];
   std::vector<std::vector<int>> cids;
-  std::vector<int> counts, perr, fmap, seen, cls, idx;
+  std::vector<int> counts, perr, fmap, seen, cls, idx, chunk_of;
   std::vector<ChunkOut> ch;
   std::vector<TileDev> base;
   std::vector<long long> cost;
@@ -528,6 +528,10 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
     ch[c].b1 = nb;
     nchunk = c + 1;
   }
+  std::vector<int>& chunk_of = X.chunk_of;   // block -> its chunk
+  chunk_of.resize(nb);
+  for (int c = 0; c < nchunk; ++c)
+    for (int b = ch[c].b0; b < ch[c].b1; ++b) chunk_of[b] = c;
   auto for_chunks = [&](const std::function<void(ChunkOut&)>& f) {
     HostPool::run(nchunk, [&](int i) { f(ch[i]); });
     for (int c = 0; c < nchunk; ++c)
@@ -585,33 +589,22 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
                   nbr[s] = k2 == NB_BOUNDARY ? -1 - T.bc[6 * cb + 2 * d + s] : (int)r2;
                 }
               }
-              fmap[pi] = (int)o.faces.size();   // chunk-local until joined
+              fmap[pi] = (int)o.faces.size();   // chunk-local: + face_base[chunk_of[b]]
               o.faces.push_back(f);
             }
         }
     }
   });
   if (rc) return rc;
+  // faces are numbered in chunk order: global = chunk-local + face_base[chunk]
+  // (the chunks' faces are copied into H.faces with the records below)
   std::vector<int> face_base(nchunk + 1, 0);
   for (int c = 0; c < nchunk; ++c) face_base[c + 1] = face_base[c] + (int)ch[c].faces.size();
   H.faces.resize(face_base[nchunk]);
-  HostPool::run(nchunk, [&](int c) {
-    const ChunkOut& o = ch[c];
-    std::copy(o.faces.begin(), o.faces.end(), H.faces.begin() + face_base[c]);
-    const int b = o.b0, e = o.b1;
-    if (e > b) {
-      // the chunk's plane maps are one contiguous run of fmap
-      int lo = -1, hi = -1;
-      for (int q = b; q < e; ++q)
-        for (int s = 0; s < 6; ++s)
-          if (H.blocks[q].repoff[s] >= 0) {
-            if (lo < 0) lo = H.blocks[q].repoff[s];
-            hi = H.blocks[q].repoff[s] + plane_size(H.blocks[q].n, s >> 1);
-          }
-      for (int i = lo; i >= 0 && i < hi; ++i)
-        if (fmap[i] >= 0) fmap[i] += face_base[c];
-    }
-  });
+  auto face_ref = [&](int f) -> const LinkFaceDev& {
+    const int c = (int)(std::upper_bound(face_base.begin(), face_base.begin() + nchunk + 1, f) - face_base.begin()) - 1;
+    return ch[c].faces[f - face_base[c]];
+  };
 
   // the link face that cell q of block blk emits through side sl, or -1
   auto face_at = [&](int blk, const long long q[3], int sl) -> int {
@@ -619,7 +612,8 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
     if (ob.repoff[sl] < 0) return -1;
     const int d = sl >> 1;
     if (q[d] != ((sl & 1) ? ob.n[d] - 1 : 0)) return -1;
-    return fmap[ob.repoff[sl] + plane_idx(ob.n, d, q[0], q[1], q[2])];
+    const int v = fmap[ob.repoff[sl] + plane_idx(ob.n, d, q[0], q[1], q[2])];
+    return v < 0 ? -1 : v + face_base[chunk_of[blk]];
   };
 
   // ---- link-cell records, their per-face maps and the flux-register lists (per chunk) ----
@@ -665,7 +659,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
               SlotList fl{0, -1, -1, -1, -1};
               if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
                 type = 1;
-                fl[++fl[0]] = fmap[pi];
+                fl[++fl[0]] = fmap[pi] + face_base[chunk_of[b]];
               } else if (side == 0 && kind == NB_SAME) {
                 type = 2;
                 emitter_block = nbb;
@@ -703,7 +697,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
               bgrp[6 * r + slot] = emitter_block;
               if (kind == NB_FINER) brec[r].gr |= 1u << (24 + slot);
               for (int i = 1; i <= fl[0]; ++i) {   // each listed face must touch this cell
-                const LinkFaceDev& F = H.faces[fl[i]];
+                const LinkFaceDev& F = face_ref(fl[i]);
                 if (F.iL != g && F.iR != g)
                   return (void)o.set_error(fail(-1, "link face bookkeeping error at block %s%lld", "", b));
               }
@@ -788,6 +782,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   H.lcmap.resize(lc_base[nchunk]);
   HostPool::run(nchunk, [&](int c) {
     const ChunkOut& o = ch[c];
+    std::copy(o.faces.begin(), o.faces.end(), H.faces.begin() + face_base[c]);
     const int rb = (int)rec_base[c], sb = (int)sf_base[c], lb = (int)lc_base[c];
     for (size_t r = 0; r < o.recs.size(); ++r) {
       LinkCellDev R = o.recs[r];
