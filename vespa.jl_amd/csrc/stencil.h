@@ -404,6 +404,19 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
     const long long c = (long long)k * plane + (long long)gy * nx + gx;
     Cons<T> fzhi;   // z face at k+½ of the lane's column
     Prim<T> sz1;    // its z slope at k+1
+    // the z face at k+½ of the lane's column and the z slope of plane k+1
+    auto z_face = [&]() {
+      if constexpr (RANK == 3) {
+        sz1 = slope(wz0, wz1, wz2);
+        const int bz = G.bc[4];
+        const Prim<T> L = face_value(wz0, sz0, T(0.5));
+        if (!PER && bz != BC_PERIODIC && k == nz - 1) {
+          fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
+          if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
+        } else
+          fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
+      }
+    };
     // Early loads: the next z plane of the column and this cell's own state.
     Cons<T> unext, uown, u0own;
     if constexpr (RANK == 3) {
@@ -459,16 +472,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
           halo_pos(lt, lx, ly);
           uh = load_cons(Us, n, halo_src(lx, ly, k));
         }
-        if constexpr (RANK == 3) {
-          sz1 = slope(wz0, wz1, wz2);
-          const int bz = G.bc[4];
-          const Prim<T> L = face_value(wz0, sz0, T(0.5));
-          if (!PER && bz != BC_PERIODIC && k == nz - 1) {
-            fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
-            if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
-          } else
-            fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
-        }
+        z_face();
         if (hv) stage_halo(lx, ly, k, uh);
       } else {
         for (int h = lt; h < NHX + NHY; h += NT) {
@@ -570,16 +574,7 @@ __global__ void __launch_bounds__(TX* TY, sizeof(T) == 4 ? VH_MIN_WAVES_F32 : VH
 
     // z face at k+½ for the lane's column (here unless computed in P0).
     if constexpr (!ZEARLY) {
-      if constexpr (RANK == 3) {
-        sz1 = slope(wz0, wz1, wz2);
-        const int bz = G.bc[4];
-        const Prim<T> L = face_value(wz0, sz0, T(0.5));
-        if (!PER && bz != BC_PERIODIC && k == nz - 1) {
-          fzhi = hllc(L, bghost<PG>(L, G, 2, 1, col), g, igm1, 3);
-          if (G.bflux && active) bflux_add(G, 2, 1, (long long)gy * nx + gx, fzhi, T(0.5 * dt), a2);
-        } else
-          fzhi = hllc(L, face_value(wz1, sz1, T(-0.5)), g, igm1, 3);
-      }
+      z_face();
     }
     __syncthreads();
 
