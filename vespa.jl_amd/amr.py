@@ -47,18 +47,18 @@ class QuadBlocks:
 
     def mesh(self) -> BlockMesh:
         ml = self.maxlevel
-        lv = [b[0] for b in self.blocks]
-        dims = [[self.B, self.B, 1]] * len(self.blocks)
-        org = []
-        bc = []
-        for (l, ix, iy) in self.blocks:
-            span = self.B * (1 << (ml - l))          # lattice units per block edge
-            org.append([ix * span, iy * span, 0])
-            n_l = self.nroot << l                     # blocks per axis at this level
-            bc.append([self.bc, self.bc, self.bc, self.bc, 1, 1])
-            assert 0 <= ix < n_l and 0 <= iy < n_l
+        a = np.array(self.blocks, dtype=np.int64).reshape(-1, 3)
+        lv, ix, iy = a[:, 0], a[:, 1], a[:, 2]
+        n_l = np.int64(self.nroot) << lv                  # blocks per axis at each block's level
+        assert ((0 <= ix) & (ix < n_l) & (0 <= iy) & (iy < n_l)).all()
+        span = np.int64(self.B) << (ml - lv)              # lattice units per block edge
+        org = np.zeros((len(a), 3), dtype=np.int64)
+        org[:, 0] = ix * span
+        org[:, 1] = iy * span
+        dims = np.tile(np.array([self.B, self.B, 1], dtype=np.int32), (len(a), 1))
+        bc = np.tile(np.array([self.bc] * 4 + [1, 1], dtype=np.int32), (len(a), 1))
         h0 = self.L / (self.nroot * self.B)
-        return BlockMesh(2, ml, (h0, h0), self.gamma, lv, dims, org, bc, x0=(self.x0, self.x0))
+        return BlockMesh(2, ml, (h0, h0), self.gamma, lv.astype(np.int32), dims, org, bc, x0=(self.x0, self.x0))
 
     def cell_centers(self):
         """(x, y) of every cell in leaf order (block by block, x fastest)."""
@@ -110,63 +110,57 @@ class QuadBlocks:
                         tags.add(img)
         return tags
 
-    def _leaf_at(self, leafset: dict, level: int, fx: int, fy: int):
-        """Leaf block containing finest-lattice block position (fx, fy) (in units
-        of finest blocks), searched from `level` down to 0."""
-        for l in range(self.maxlevel, -1, -1):
-            s = 1 << (self.maxlevel - l)
-            key = (l, fx // s, fy // s)
-            if key in leafset:
-                return key
-        return None
-
     def refine(self, tags: set) -> int:
         """Refine the tagged blocks, then restore 2:1 balance across faces.
-        Returns the number of blocks split."""
-        leaves = {b: True for b in self.blocks}
-        todo = set(tags)
+        Returns the number of blocks split.
+
+        Work-list form: a leaf at level l whose face neighbour region (the
+        level-l slot across one of its faces, inside the square) lies in a leaf
+        coarser than l - 1 forces that leaf to split; only leaves created by a
+        split can break the balance, so each new leaf is checked once (and its
+        own check repeats until the slot's leaf is fine enough). Every split is
+        forced, so the result is the smallest balanced refinement that splits
+        the tags, independent of the order of the work list."""
+        leaves = set(self.blocks)
+        queue = []
+
+        def split(b):
+            l, ix, iy = b
+            leaves.discard(b)
+            for cy in (0, 1):
+                for cx in (0, 1):
+                    c = (l + 1, 2 * ix + cx, 2 * iy + cy)
+                    leaves.add(c)
+                    queue.append(c)
+
         nsplit = 0
-        while todo:
-            nxt = set()
-            for b in sorted(todo):
-                if b not in leaves:
-                    continue
-                l, ix, iy = b
-                del leaves[b]
+        for b in sorted(tags):
+            if b in leaves:
+                split(b)
                 nsplit += 1
-                for cy in (0, 1):
-                    for cx in (0, 1):
-                        leaves[(l + 1, 2 * ix + cx, 2 * iy + cy)] = True
-            # 2:1 balance: a leaf whose face neighbour is more than one level finer must refine
-            for b in list(leaves):
-                l, ix, iy = b
-                if l >= self.maxlevel:
+        while queue:
+            c = queue.pop()
+            if c not in leaves:
+                continue                        # split since: its children are queued
+            lc, cx, cy = c
+            if lc < 2:
+                continue
+            n = self.nroot << lc
+            for px, py in ((cx - 1, cy), (cx + 1, cy), (cx, cy - 1), (cx, cy + 1)):
+                if not (0 <= px < n and 0 <= py < n):
                     continue
-                s = 1 << (self.maxlevel - l)                # finest blocks per edge of b
-                fx0, fy0 = ix * s, iy * s
-                nmax = self.nroot << self.maxlevel
-                need = False
-                for side in range(4):
-                    for t in range(s):
-                        if side == 0:
-                            px, py = fx0 - 1, fy0 + t
-                        elif side == 1:
-                            px, py = fx0 + s, fy0 + t
-                        elif side == 2:
-                            px, py = fx0 + t, fy0 - 1
-                        else:
-                            px, py = fx0 + t, fy0 + s
-                        if not (0 <= px < nmax and 0 <= py < nmax):
-                            continue
-                        nb = self._leaf_at(leaves, l, px, py)
-                        if nb is not None and nb[0] > l + 1:
-                            need = True
+                while True:
+                    # the leaf holding slot (lc, px, py): itself or an ancestor
+                    for up in range(lc + 1):
+                        a = (lc - up, px >> up, py >> up)
+                        if a in leaves:
                             break
-                    if need:
+                    else:
+                        break                   # the slot is split finer than lc
+                    if up < 2:
                         break
-                if need:
-                    nxt.add(b)
-            todo = nxt
+                    split(a)
+                    nsplit += 1
         # order: by level, then row-major, for a deterministic layout
         self.blocks = sorted(leaves, key=lambda b: (b[0], b[2], b[1]))
         return nsplit
@@ -176,14 +170,17 @@ class QuadBlocks:
         of the ancestor it was split from: the table the remap injects by."""
         where = {b: q for q, b in enumerate(old_blocks)}
         out = np.empty(len(self.blocks), dtype=np.int32)
-        for q, (l, ix, iy) in enumerate(self.blocks):
-            for up in range(0, l + 1):
-                a = (l - up, ix >> up, iy >> up)
-                if a in where:
-                    out[q] = where[a]
-                    break
-            else:
-                raise ValueError(f"block {(l, ix, iy)} has no ancestor in the old mesh")
+        for q, b in enumerate(self.blocks):
+            s = where.get(b)
+            if s is None:
+                l, ix, iy = b
+                for up in range(1, l + 1):
+                    s = where.get((l - up, ix >> up, iy >> up))
+                    if s is not None:
+                        break
+                else:
+                    raise ValueError(f"block {b} has no ancestor in the old mesh")
+            out[q] = s
         return out
 
 
