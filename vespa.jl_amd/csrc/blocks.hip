@@ -356,7 +356,7 @@ struct CandKey {
 struct BuildScratch {
   std::vector<CandKey> at[3][2];
   std::vector<std::vector<int>> cids;
-  std::vector<int> counts, perr, fmap, seen, cls, idx, chunk_of;
+  std::vector<int> counts, perr, fmap, nbk, seen, cls, idx, chunk_of;
   std::vector<ChunkOut> ch;
   std::vector<TileDev> base;
   std::vector<long long> cost;
@@ -502,6 +502,10 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
   // fmap runs beside repmap: the link face a face cell emits through that side, or -1
   std::vector<int>& fmap = X.fmap;
   fmap.assign((size_t)nplane, -1);
+  // nbk beside them: the neighbour block across that side and the kind, 8·block + kind
+  // (set by the face walk, read back by the record walk)
+  std::vector<int>& nbk = X.nbk;
+  nbk.resize((size_t)nplane);
 
   // chunks: contiguous block ranges of similar face-plane work
   int nchunk = std::min(4 * nthr, nb);
@@ -570,6 +574,7 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
                                               "(fine blocks must be aligned to coarse cells)", "", b));
               const int pi = bk.repoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2]);
               H.repmap[pi] = kind == NB_BOUNDARY ? -1 : (int)rep;
+              nbk[pi] = 8 * nbb + kind;
               const bool emit = (side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER);
               if (!emit) continue;
               LinkFaceDev f{};
@@ -652,9 +657,16 @@ static int build_tables(Topo& T, int zchunk, HostTables& H, int max_tx = 32, int
               const int pi = bk.repoff[slot] + plane_idx(bk.n, d, ijk[0], ijk[1], ijk[2]);
               if (H.repmap[pi] < 0) continue;   // a domain boundary
               const long long g = T.index(b, ijk);
-              long long rep = -1, nijk[3];
-              int nbb = -1;
-              const int kind = T.neighbor_at(b, ijk, d, side, &rep, nullptr, nullptr, &nbb, nijk);
+              // the face walk's neighbour: block, kind and (from the representative) its cell
+              const int kind = nbk[pi] & 7, nbb = nbk[pi] >> 3;
+              long long nijk[3];
+              {
+                const int* nn = &T.dims[3 * nbb];
+                const long long t = H.repmap[pi] - T.off[nbb];
+                nijk[0] = t % nn[0];
+                nijk[1] = (t / nn[0]) % nn[1];
+                nijk[2] = t / ((long long)nn[0] * nn[1]);
+              }
               int type = 0, emitter_block = -1;
               SlotList fl{0, -1, -1, -1, -1};
               if ((side == 1 && (kind == NB_SAME || kind == NB_COARSER)) || (side == 0 && kind == NB_COARSER)) {
