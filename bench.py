@@ -399,11 +399,16 @@ def sedov_leg(V, args):
     import numpy as np
     from vespa_jl_amd import amr
     out = {}
-    # Untimed warm-up: one whole run of the same problem. Measured right after the
-    # adjoint leg, the first ~0.1 s or more of the next run went 5x slower on the
-    # host (table builds and the pure-Python tree edit alike); a quarter-length
-    # warm-up did not absorb it.
-    amr.run_sedov(tfinal=0.05, dtype=np.float64, mode=args.mode)
+    # Untimed warm-up: whole runs of the same problem for at least half a second.
+    # Measured right after the adjoint leg, the first ~0.1 s or more of host work
+    # went 5x slower (table builds and the pure-Python tree edit alike): a
+    # quarter-length warm-up did not absorb it, and once a whole run took 0.045 s
+    # one run did not either (tables 10.3 ms against 4.9 ms).
+    tw = time.perf_counter()
+    while True:
+        amr.run_sedov(tfinal=0.05, dtype=np.float64, mode=args.mode)
+        if time.perf_counter() - tw >= 0.5:
+            break
     for dtype, key in ((np.float64, "f64"), (np.float32, "f32")):
         t0 = time.perf_counter()
         r = amr.run_sedov(tfinal=0.05, dtype=dtype, snapshots=(0.0125, 0.025, 0.05), mode=args.mode)
