@@ -90,3 +90,59 @@ def test_shock_radius_of_a_ring():
     X, Y = np.meshgrid(c, c)
     rho = 1.0 + np.exp(-((np.hypot(X, Y) - 0.2) / 0.01) ** 2)
     assert abs(amr.shock_radius(rho) - 0.2) < 2.0 / N
+
+
+def _sweep_refine(nroot, maxlevel, leaves, tags):
+    """2:1 balance by sweeps: split the tags, then every leaf that has a face
+    neighbour more than one level finer (found through the finest-lattice
+    positions along its faces), until none does. Slow, and the definition the
+    work-list form in QuadBlocks.refine has to reproduce."""
+    leaves = set(leaves)
+    todo = set(tags)
+    nmax = nroot << maxlevel
+
+    def leaf_at(fx, fy):
+        for l in range(maxlevel, -1, -1):
+            s = 1 << (maxlevel - l)
+            if (l, fx // s, fy // s) in leaves:
+                return (l, fx // s, fy // s)
+        return None
+
+    while todo:
+        for (l, ix, iy) in todo:
+            if (l, ix, iy) in leaves:
+                leaves.discard((l, ix, iy))
+                leaves.update((l + 1, 2 * ix + cx, 2 * iy + cy) for cx in (0, 1) for cy in (0, 1))
+        todo = set()
+        for (l, ix, iy) in leaves:
+            s = 1 << (maxlevel - l)
+            edge = [(ix * s - 1, iy * s + t) for t in range(s)] + [(ix * s + s, iy * s + t) for t in range(s)] + \
+                   [(ix * s + t, iy * s - 1) for t in range(s)] + [(ix * s + t, iy * s + s) for t in range(s)]
+            for (px, py) in edge:
+                if 0 <= px < nmax and 0 <= py < nmax:
+                    nb = leaf_at(px, py)
+                    if nb is not None and nb[0] > l + 1:
+                        todo.add((l, ix, iy))
+                        break
+    return sorted(leaves, key=lambda b: (b[0], b[2], b[1]))
+
+
+def test_refine_matches_sweep_balance():
+    """The work-list balance gives the same leaves as splitting by sweeps, on
+    random tag sequences over small quadtrees (1-4 roots per axis, up to 4
+    levels); each split counts once."""
+    rng = np.random.default_rng(7)
+    for _ in range(60):
+        nroot, ml = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+        qb = amr.QuadBlocks(nroot, 4, ml)
+        for _ in range(int(rng.integers(1, 5))):
+            cand = [b for b in qb.blocks if b[0] < ml]
+            if not cand:
+                break
+            pick = rng.choice(len(cand), size=int(rng.integers(1, min(len(cand), 5) + 1)), replace=False)
+            tags = {cand[i] for i in pick}
+            want = _sweep_refine(nroot, ml, qb.blocks, tags)
+            n0 = len(qb.blocks)
+            nsplit = qb.refine(tags)
+            assert qb.blocks == want
+            assert len(qb.blocks) == n0 + 3 * nsplit
